@@ -1,0 +1,225 @@
+"""GPU parity: libtsunami_hip.so (HIP, gfx950) vs the CPU oracle, bit-exact.
+
+Every test calls the C-ABI through tsunami_pact_amd (ctypes) and compares
+integer scores (and CIGARs) with oracle/ on the same seeded inputs.  Sizes
+are those the oracle finishes in seconds; the full-size configuration is
+checked through size-independent properties plus a seeded sample.
+"""
+import json
+import os
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+HERE = os.path.dirname(os.path.abspath(__file__))
+SEED = 0x5453554E
+
+
+def say(msg):
+    print(f"\n[gpu] {msg}", flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def fixtures():
+    with open(os.path.join(HERE, "golden", "pairs_scores_cigars.json")) as f:
+        return json.load(f)["cases"]
+
+
+@pytest.fixture(scope="module")
+def aligners(tsu):
+    cache = {}
+
+    def get(pen):
+        if pen not in cache:
+            cache[pen] = tsu.Aligner(*pen)
+        return cache[pen]
+    yield get
+    for a in cache.values():
+        a.close()
+
+
+def test_device_is_gfx950(aligner):
+    info = aligner.device_info()
+    say(f"device {info}")
+    assert "gfx950" in info["name"]
+    assert info["cu_count"] > 0
+
+
+def test_plain_config1_all_pairs(aligner, tsu, oracle):
+    """Config 1: 10,000 pairs of 100 bp, 2% mixed edits, W = 100."""
+    ds = tsu.Dataset.generate(10000, seed=SEED + 1, len_min=100, err_min=0.02)
+    say("plain config1 10000x100 W=100")
+    b = aligner.batch(ds)
+    got, t = aligner.align_plain(b, 100)
+    want, _, _ = oracle.wfa_batch(ds.pairs(), 4, 6, 2)
+    assert (got == want).all(), np.nonzero(got != want)[0][:10]
+    assert t["kernel_ms"] > 0
+
+
+def test_plain_config2_sample(aligner, tsu, oracle):
+    """Config-2 shape: 1000 bp, 5% mixed edits (3000 seeded pairs)."""
+    ds = tsu.Dataset.generate(3000, seed=SEED + 2, len_min=1000, err_min=0.05)
+    want, _, cnt = oracle.wfa_batch(ds.pairs(), 4, 6, 2)
+    W = 2 * ((int(want.max()) - 6) // 2) + 3
+    say(f"plain config2 sample 3000x1000 W={W}")
+    b = aligner.batch(ds)
+    got, _ = aligner.align_plain(b, W)
+    assert (got == want).all()
+
+
+def test_plain_long_reads(aligner, tsu, oracle):
+    """Config-3 shape: 10,000 bp at 10% (12 seeded pairs), ring of ~7k diagonals."""
+    ds = tsu.Dataset.generate(12, seed=SEED + 3, len_min=10000, err_min=0.10)
+    want, _, _ = oracle.wfa_batch(ds.pairs(), 4, 6, 2)
+    W = 2 * ((int(want.max()) - 6) // 2) + 3
+    say(f"plain long 12x10000 W={W}")
+    got, _ = aligner.align_plain(aligner.batch(ds), W)
+    assert (got == want).all()
+
+
+def test_capacity_variants_agree(aligner, tsu, oracle):
+    """Different diagonal capacities select 1..16 waves per pair; same scores."""
+    ds = tsu.Dataset.generate(400, seed=SEED + 5, len_min=600, err_min=0.08)
+    want, _, _ = oracle.wfa_batch(ds.pairs(), 4, 6, 2)
+    b = aligner.batch(ds)
+    for W in (400, 700, 1300, 2500, 5000):
+        say(f"capacity W={W}")
+        got, _ = aligner.align_plain(b, W)
+        assert (got == want).all(), W
+
+
+def test_golden_fixture_scores(aligners, tsu, fixtures):
+    by_pen = {}
+    for c in fixtures:
+        by_pen.setdefault(tuple(c["penalties"]), []).append(c)
+    for pen, cases in by_pen.items():
+        say(f"fixtures penalties={pen} n={len(cases)}")
+        a = aligners(pen)
+        b = a.batch([(c["pattern"], c["text"]) for c in cases])
+        W = max(len(c["pattern"]) + len(c["text"]) + 3 for c in cases)
+        got, _ = a.align_plain(b, W)
+        assert [int(v) for v in got] == [c["score"] for c in cases]
+
+
+def test_penalty_sets_random(aligners, tsu, oracle):
+    ds = tsu.Dataset.generate(600, seed=SEED + 6, len_min=30, len_max=700, err_min=0.0, err_max=0.25)
+    pairs = ds.pairs()
+    for pen in [(1, 1, 1), (2, 3, 1), (5, 2, 3), (3, 0, 1), (6, 4, 2), (7, 11, 3)]:
+        say(f"random penalties={pen}")
+        want, _, _ = oracle.wfa_batch(pairs, *pen)
+        a = aligners(pen)
+        got, _ = a.align_plain(a.batch(ds), 1500)
+        assert (got == want).all(), pen
+
+
+def test_overflow_is_reported_not_corrupting(aligner, tsu, oracle):
+    good = tsu.Dataset.generate(50, seed=1, len_min=200, err_min=0.01).pairs()
+    rng = np.random.default_rng(2)
+    bad = ("".join(rng.choice(list("ACGT"), 300)), "".join(rng.choice(list("ACGT"), 300)))
+    pairs = good[:25] + [bad] + good[25:]
+    say("overflow W=24")
+    b = aligner.batch(pairs)
+    got, _ = aligner.align_plain(b, 24, raise_on_error=False)
+    assert got[25] == tsu.TSU_EKOVERFLOW
+    want, _, _ = oracle.wfa_batch(pairs, 4, 6, 2)
+    ok = [i for i in range(len(pairs)) if i != 25]
+    assert (got[ok] == want[ok]).all()
+    with pytest.raises(tsu.TsuError) as ei:
+        aligner.align_plain(b, 24)
+    assert ei.value.status == tsu.TSU_EKOVERFLOW
+
+
+def test_int32_offsets_long_pair(aligner, tsu, oracle):
+    """Sequences longer than 32,000 bases switch the ring to int32 offsets."""
+    ds = tsu.Dataset.generate(2, seed=SEED + 7, len_min=40000, err_min=0.004)
+    want, _, _ = oracle.wfa_batch(ds.pairs(), 4, 6, 2)
+    W = 2 * ((int(want.max()) - 6) // 2) + 3
+    say(f"int32 2x40000 W={W}")
+    got, _ = aligner.align_plain(aligner.batch(ds), W)
+    assert (got == want).all()
+
+
+def test_fixed_layout_entry_point(aligner, tsu, oracle):
+    """tsu_align_fixed takes the reference's fixed-slot char arrays."""
+    ds = tsu.Dataset.generate(500, seed=SEED + 8, len_min=128, err_min=0.02, edit_mode=1)
+    pairs = ds.pairs()
+    say("fixed layout 500x128")
+    got, _ = aligner.align_fixed([p for p, _ in pairs], [t for _, t in pairs], 128, 42)
+    want, _, _ = oracle.wfa_batch(pairs, 4, 6, 2)
+    assert (got == want).all()
+
+
+def test_batched_matches_oracle_mixed(aligner, tsu, oracle):
+    """Config-4 shape (small): lengths log-uniform 100..5000, 1-15% error."""
+    ds = tsu.Dataset.generate(1500, seed=SEED + 4, len_min=100, len_max=5000, err_min=0.01, err_max=0.15)
+    want, _, _ = oracle.wfa_batch(ds.pairs(), 4, 6, 2)
+    say("batched mixed 1500")
+    got, t = aligner.align_batched(aligner.batch(ds))
+    assert (got == want).all()
+    assert t["launches"] >= 1
+
+
+def test_cigars_match_oracle(aligner, tsu, oracle, fixtures):
+    ds = tsu.Dataset.generate(300, seed=SEED + 9, len_min=50, len_max=3000, err_min=0.0, err_max=0.15)
+    pairs = ds.pairs()
+    say("cigar mixed 300")
+    got, cig, _ = aligner.align_cigar(aligner.batch(ds))
+    want, ops, _ = oracle.wfa_batch(pairs, 4, 6, 2, cigar=True)
+    assert (got == want).all()
+    for i, (p, t) in enumerate(pairs):
+        assert cig[i] == oracle.rle(ops[i]), i
+        assert oracle.rescore(oracle.expand(cig[i]), p, t, 4, 6, 2) == got[i]
+    cases = [c for c in fixtures if tuple(c["penalties"]) == (4, 6, 2)]
+    say(f"cigar fixtures {len(cases)}")
+    got, cig, _ = aligner.align_cigar(aligner.batch([(c["pattern"], c["text"]) for c in cases]))
+    assert [int(v) for v in got] == [c["score"] for c in cases]
+    assert cig == [c["cigar"] for c in cases]
+
+
+def test_cigar_wfa2lib_readme_example(aligner):
+    say("cigar readme example")
+    got, cig, _ = aligner.align_cigar(aligner.batch([("TCTTTACTCGCGCGTTGGAGAAATACAATAGT",
+                                                        "TCTATACTGCGCGTTTGGAGAAATAAAATAGT")]))
+    assert int(got[0]) == 24
+    assert cig[0] == "3M1X4M1D7M1I9M1X6M"
+
+
+def test_full_size_config2_properties(aligner, tsu, oracle):
+    """1,000,000 pairs of 1000 bp at 5%: every score is non-negative, a
+    multiple of gcd(4,6,2)=2 and within the all-mismatch-plus-gap bound; a
+    seeded sample of 2,000 pairs equals the oracle; re-running is identical."""
+    n = 1_000_000
+    ds = tsu.Dataset.generate(n, seed=SEED + 2, len_min=1000, err_min=0.05)
+    say("full config2 1Mx1000")
+    b = aligner.batch(ds)
+    got, t = aligner.align_plain(b, 448)
+    assert (got >= 0).all() and (got % 2 == 0).all()
+    pl, tl = ds.lengths()
+    bound = 4 * np.minimum(pl, tl) + np.where(pl != tl, 6 + 2 * np.abs(pl.astype(int) - tl), 0)
+    assert (got <= bound).all()
+    idx = np.random.default_rng(0).choice(n, 2000, replace=False)
+    want, _, _ = oracle.wfa_batch([ds.pair(int(i)) for i in idx], 4, 6, 2)
+    assert (got[idx] == want).all()
+    again, _ = aligner.align_plain(b, 448)
+    assert (again == got).all()
+    say(f"full config2 kernel_ms={t['kernel_ms']:.2f}")
+
+
+def test_wfa_gpu_cli_check_mode(repo, tsu, tmp_path):
+    """bin/wfa_gpu with check=1 prints the reference's success banner."""
+    ds = tsu.Dataset.generate(2000, seed=SEED + 10, len_min=128, err_min=0.02)
+    ds.write(str(tmp_path / "sequences.txt"), 2)
+    say("cli check 2000x128")
+    r = subprocess.run([os.path.join(repo, "bin", "wfa_gpu"), "4", "6", "2", "sequences.txt", "42", "1"],
+                       cwd=tmp_path, capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    lines = r.stdout.splitlines()
+    assert lines[0].startswith("GPU Time: ")
+    assert lines[1].startswith("Estimated GCUPS GPU: : ")
+    assert lines[2].startswith("CPU Time: ")
+    assert lines[3].startswith("Estimated GCUPS SW: ")
+    assert "|ALIGNEMENT FINISHED CORRECTLY|" in r.stdout

@@ -1,0 +1,158 @@
+"""Pins the CPU oracle (oracle/wfa_oracle.c) before it is trusted.
+
+Checks, each chosen so that a plausible mistake fails one of them:
+  - the WFA2-lib README example (published score 24, CIGAR 3M1X4M1D7M1I9M1X6M);
+  - Gotoh == exhaustive enumeration of every alignment on tiny inputs;
+  - closed forms (identity, one substitution, one gap of length L, empty);
+  - WFA score == Gotoh on every fixture and on all 10,000 config-1 pairs;
+  - CIGARs re-score to the score, consume both sequences, M/X are true
+    matches/mismatches; swap symmetry; the all-mismatch-plus-gap bound;
+  - the committed golden fixtures.
+"""
+import functools
+import json
+import os
+import random
+
+import pytest
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+PENALTIES = [(4, 6, 2), (1, 1, 1), (2, 3, 1), (5, 2, 3), (3, 0, 1), (1, 0, 1), (6, 4, 2)]
+
+
+def brute(p, t, x, o, e):
+    """Minimum cost over every alignment path (exhaustive recursion)."""
+    @functools.lru_cache(None)
+    def f(i, j, last):  # last: 0 none/M/X, 1 I, 2 D
+        if i == len(p) and j == len(t):
+            return 0
+        best = 1 << 60
+        if i < len(p) and j < len(t):
+            best = min(best, (0 if p[i] == t[j] else x) + f(i + 1, j + 1, 0))
+        if j < len(t):
+            best = min(best, (e if last == 1 else o + e) + f(i, j + 1, 1))
+        if i < len(p):
+            best = min(best, (e if last == 2 else o + e) + f(i + 1, j, 2))
+        return best
+    return f(0, 0, 0)
+
+
+def rand_seq(rng, n):
+    return "".join(rng.choice("ACGT") for _ in range(n))
+
+
+def test_wfa2lib_readme_example(oracle):
+    p = "TCTTTACTCGCGCGTTGGAGAAATACAATAGT"
+    t = "TCTATACTGCGCGTTTGGAGAAATAAAATAGT"
+    s, ops, _ = oracle.wfa(p, t, 4, 6, 2, cigar=True)
+    assert s == 24
+    assert oracle.rle(ops) == "3M1X4M1D7M1I9M1X6M"
+    assert oracle.gotoh(p, t, 4, 6, 2) == 24
+
+
+@pytest.mark.parametrize("pen", PENALTIES)
+def test_gotoh_and_wfa_equal_brute_force(oracle, pen):
+    rng = random.Random(hash(pen) & 0xFFFF)
+    for _ in range(400):
+        p = rand_seq(rng, rng.randint(0, 7))
+        t = rand_seq(rng, rng.randint(0, 7))
+        b = brute(p, t, *pen)
+        assert oracle.gotoh(p, t, *pen) == b, (p, t)
+        s, ops, _ = oracle.wfa(p, t, *pen, cigar=True)
+        assert s == b, (p, t, pen)
+        assert oracle.rescore(ops, p, t, *pen) == b
+
+
+def test_closed_forms(oracle):
+    rng = random.Random(3)
+    for x, o, e in PENALTIES:
+        for L in (1, 5, 37, 100):
+            s = rand_seq(rng, 60)
+            assert oracle.wfa(s, s, x, o, e)[0] == 0
+            # one substitution costs min(x, 2(o+e)) (a sub can also be a D+I pair)
+            i = rng.randrange(len(s))
+            sub = s[:i] + "ACGT"[("ACGT".index(s[i]) + 1) % 4] + s[i + 1:]
+            assert oracle.wfa(s, sub, x, o, e)[0] == min(x, 2 * (o + e))
+            # empty against L bases: one gap
+            t = rand_seq(rng, L)
+            assert oracle.wfa("", t, x, o, e)[0] == o + L * e
+            assert oracle.wfa(t, "", x, o, e)[0] == o + L * e
+        assert oracle.wfa("", "", x, o, e)[0] == 0
+
+
+def test_single_indel_closed_form(oracle):
+    # a homopolymer-free insertion of L bases into a sequence costs o + L*e
+    base = "ACGTTGCAAGCTTCGA" * 4
+    for L in (1, 2, 3, 8):
+        ins = base[:20] + "C" * L + base[20:]
+        s, ops, _ = oracle.wfa(base, ins, 4, 6, 2, cigar=True)
+        assert s == 6 + 2 * L
+        assert oracle.rescore(ops, base, ins, 4, 6, 2) == s
+
+
+def test_gcd_divisibility_and_symmetry(oracle):
+    rng = random.Random(11)
+    for _ in range(200):
+        p = rand_seq(rng, rng.randint(0, 60))
+        t = list(p)
+        for _ in range(rng.randint(0, 10)):
+            pos = rng.randrange(len(t) + 1)
+            op = rng.randrange(3)
+            if op == 0 and pos < len(t):
+                t[pos] = rng.choice("ACGT")
+            elif op == 1:
+                t.insert(pos, rng.choice("ACGT"))
+            elif pos < len(t):
+                del t[pos]
+        t = "".join(t)
+        s = oracle.wfa(p, t, 4, 6, 2)[0]
+        assert s % 2 == 0
+        assert oracle.wfa(t, p, 4, 6, 2)[0] == s  # I <-> D symmetric
+        bound = 4 * min(len(p), len(t)) + (6 + 2 * abs(len(p) - len(t)) if len(p) != len(t) else 0)
+        assert s <= bound
+
+
+def test_config1_all_pairs_wfa_equals_gotoh(oracle, tsu):
+    ds = tsu.Dataset.generate(10000, seed=0x5453554E + 1, len_min=100, err_min=0.02)
+    pairs = ds.pairs()
+    scores, _, cnt = oracle.wfa_batch(pairs, 4, 6, 2)
+    for i in range(0, len(pairs)):
+        p, t = pairs[i]
+        assert scores[i] == oracle.gotoh(p, t, 4, 6, 2), i
+    assert cnt["cells"] > 0 and cnt["extend_words"] > 0
+
+
+def test_golden_fixtures(oracle):
+    with open(os.path.join(HERE, "golden", "pairs_scores_cigars.json")) as f:
+        cases = json.load(f)["cases"]
+    assert len(cases) > 300
+    for c in cases:
+        pen = tuple(c["penalties"])
+        assert oracle.gotoh(c["pattern"], c["text"], *pen) == c["score"], c["name"]
+        s, ops, _ = oracle.wfa(c["pattern"], c["text"], *pen, cigar=True)
+        assert s == c["score"], c["name"]
+        assert oracle.rle(ops) == c["cigar"], c["name"]
+
+
+def test_batch_matches_single_and_cigars_valid(oracle, tsu):
+    ds = tsu.Dataset.generate(300, seed=9, len_min=80, len_max=900, err_min=0.0, err_max=0.2)
+    pairs = ds.pairs()
+    scores, ops, _ = oracle.wfa_batch(pairs, 4, 6, 2, threads=4, cigar=True)
+    for i, (p, t) in enumerate(pairs):
+        s1, o1, _ = oracle.wfa(p, t, 4, 6, 2, cigar=True)
+        assert scores[i] == s1
+        assert ops[i] == o1
+        assert oracle.rescore(ops[i], p, t, 4, 6, 2) == s1
+
+
+def test_rescore_catches_a_planted_mistake(oracle):
+    # a plausible backtrace bug (one op dropped / swapped) must be detected
+    p = "TCTTTACTCGCGCGTTGGAGAAATACAATAGT"
+    t = "TCTATACTGCGCGTTTGGAGAAATAAAATAGT"
+    _, ops, _ = oracle.wfa(p, t, 4, 6, 2, cigar=True)
+    bad = ops.replace("D", "", 1)
+    with pytest.raises(Exception):
+        oracle.rescore(bad, p, t, 4, 6, 2)
+    swapped = ops.replace("I", "Q", 1).replace("D", "I", 1).replace("Q", "D", 1)
+    with pytest.raises(Exception):
+        oracle.rescore(swapped, p, t, 4, 6, 2)

@@ -1,0 +1,737 @@
+// api.cpp -- host side of libtsunami_hip.so: contexts, device batches, the
+// plain / batched / CIGAR alignment entry points and the autotuner.
+//
+// Replaces the inline host code of the reference's main() (wfa_gpu.cu:43-148):
+// argument handling, cudaMalloc/cudaMemcpy, the <<<N,32>>> launch, kernel-only
+// timing and the score copy.  Unlike the reference, device memory is sized for
+// the workgroups that are resident at once (the rings live in LDS), not for
+// all N pairs (wfa_gpu.cu:100-108 allocate num_wf*wf_length*N shorts x3).
+#include <hip/hip_runtime_api.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <new>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "tsu_internal.h"
+
+using tsu::KArgs;
+using tsu::KPair;
+using tsu::Shape;
+
+struct tsu_ctx {
+    int device = 0;
+    int32_t x = 0, o = 0, e = 0;        // penalties as given
+    int32_t xs = 0, os = 0, es = 0, g = 1;  // divided by gcd
+    int32_t rm = 0, ri = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    int cu_count = 0;
+    int64_t max_lds = 0;
+    char name[256] = {0};
+    std::string err;
+    std::vector<tsu_tune_entry> tune;   // per-bucket shapes (autotuner)
+    // scratch reused across calls
+    int32_t* d_queue = nullptr;
+};
+
+struct tsu_batch {
+    tsu_ctx* ctx = nullptr;
+    uint32_t* d_words = nullptr;
+    KPair* d_pairs = nullptr;
+    int32_t* d_scores = nullptr;
+    int32_t n = 0;
+    int64_t n_words = 0;
+    std::vector<tsu_pair> pairs;   // host copy of the descriptors
+    int32_t max_len = 0;
+    double cells = 0.0;
+};
+
+struct tsu_cigars {
+    std::vector<uint32_t> runs;
+    std::vector<int64_t> off;
+    std::vector<int32_t> n;
+};
+
+namespace {
+
+int64_t gcd64(int64_t a, int64_t b) {
+    while (b) { int64_t t = a % b; a = b; b = t; }
+    return a < 0 ? -a : a;
+}
+
+double now_ms() {
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+int fail(tsu_ctx* ctx, int status, const std::string& msg) {
+    if (ctx) ctx->err = msg;
+    return status;
+}
+
+int hip_fail(tsu_ctx* ctx, hipError_t e, const char* what) {
+    return fail(ctx, TSU_EHIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define HIPCHK(call, what)                                   \
+    do {                                                     \
+        hipError_t _e = (call);                              \
+        if (_e != hipSuccess) return hip_fail(ctx, _e, what); \
+    } while (0)
+
+// Device buffer that frees itself.
+template <typename T>
+struct DevBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    ~DevBuf() { if (p) (void)hipFree(p); }
+    hipError_t alloc(size_t count) {
+        if (p) { (void)hipFree(p); p = nullptr; }
+        n = count;
+        if (count == 0) return hipSuccess;
+        return hipMalloc(reinterpret_cast<void**>(&p), count * sizeof(T));
+    }
+};
+
+// Diagonal capacity that holds every k-range up to score bound s (scaled):
+// |k| <= (s - o) / e + 1 on each side (+ the union's +-1).
+int32_t capacity_for_score(const tsu_ctx* ctx, int64_t s_scaled) {
+    int64_t half = (s_scaled - ctx->os) / ctx->es + 2;
+    if (half < 4) half = 4;
+    int64_t w = 2 * half + 1;
+    return (int32_t)std::min<int64_t>(w, 1 << 26);
+}
+
+// Diagonal capacity no pair of lengths (p, t) can exceed: the range lies in
+// [-p-1, t+1].
+int32_t capacity_full(int32_t p, int32_t t) { return p + t + 3; }
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// context
+// ---------------------------------------------------------------------------
+extern "C" int tsu_create(tsu_ctx** out, int device, int32_t mismatch, int32_t gap_opening, int32_t gap_extension) {
+    if (!out) return TSU_EINVAL;
+    *out = nullptr;
+    if (mismatch <= 0 || gap_opening < 0 || gap_extension <= 0) return TSU_EINVAL;
+    if (mismatch > (1 << 20) || gap_opening > (1 << 20) || gap_extension > (1 << 20)) return TSU_EINVAL;
+    tsu_ctx* ctx = new (std::nothrow) tsu_ctx();
+    if (!ctx) return TSU_ENOMEM;
+    ctx->device = device;
+    ctx->x = mismatch; ctx->o = gap_opening; ctx->e = gap_extension;
+    ctx->g = (int32_t)gcd64(gcd64(mismatch, gap_opening), gap_extension);
+    ctx->xs = mismatch / ctx->g; ctx->os = gap_opening / ctx->g; ctx->es = gap_extension / ctx->g;
+    ctx->rm = std::max(ctx->xs, ctx->os + ctx->es) + 1;
+    ctx->ri = ctx->es + 1;
+    if (ctx->rm > 4096) { delete ctx; return TSU_EINVAL; }
+    int ndev = 0;
+    hipError_t e = hipGetDeviceCount(&ndev);
+    if (e != hipSuccess || ndev <= 0) { delete ctx; return TSU_EHIP; }
+    if (device < 0 || device >= ndev) { delete ctx; return TSU_EINVAL; }
+    if ((e = hipSetDevice(device)) != hipSuccess) { delete ctx; return TSU_EHIP; }
+    hipDeviceProp_t prop;
+    if ((e = hipGetDeviceProperties(&prop, device)) != hipSuccess) { delete ctx; return TSU_EHIP; }
+    ctx->cu_count = prop.multiProcessorCount;
+    ctx->max_lds = (int64_t)prop.sharedMemPerBlock;
+    std::snprintf(ctx->name, sizeof(ctx->name), "%.180s (%.60s)", prop.name, prop.gcnArchName);
+    if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&ctx->ev0) != hipSuccess || hipEventCreate(&ctx->ev1) != hipSuccess ||
+        hipMalloc(reinterpret_cast<void**>(&ctx->d_queue), 64 * sizeof(int32_t)) != hipSuccess) {
+        tsu_destroy(ctx);
+        return TSU_EHIP;
+    }
+    *out = ctx;
+    return TSU_OK;
+}
+
+extern "C" void tsu_destroy(tsu_ctx* ctx) {
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    if (ctx->d_queue) (void)hipFree(ctx->d_queue);
+    if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
+    if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+extern "C" const char* tsu_last_error(const tsu_ctx* ctx) { return ctx ? ctx->err.c_str() : "no context"; }
+
+extern "C" int tsu_device_info(tsu_ctx* ctx, int32_t* cu_count, int32_t* lds_bytes, char* name, int32_t name_cap) {
+    if (!ctx) return TSU_EINVAL;
+    if (cu_count) *cu_count = ctx->cu_count;
+    if (lds_bytes) *lds_bytes = (int32_t)ctx->max_lds;
+    if (name && name_cap > 0) std::snprintf(name, (size_t)name_cap, "%s", ctx->name);
+    return TSU_OK;
+}
+
+extern "C" int tsu_sync(tsu_ctx* ctx) {
+    if (!ctx) return TSU_EINVAL;
+    HIPCHK(hipSetDevice(ctx->device), "hipSetDevice");
+    HIPCHK(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+    return TSU_OK;
+}
+
+// ---------------------------------------------------------------------------
+// batches
+// ---------------------------------------------------------------------------
+static int batch_upload(tsu_ctx* ctx, const uint32_t* words, int64_t n_words, const tsu_pair* pairs, int32_t n,
+                        tsu_batch** out) {
+    *out = nullptr;
+    for (int32_t i = 0; i < n; ++i) {
+        const tsu_pair& p = pairs[i];
+        if (p.p_len < 0 || p.t_len < 0 || p.p_off < 0 || p.t_off < 0 ||
+            p.p_off + tsu_packed_words(p.p_len) > n_words || p.t_off + tsu_packed_words(p.t_len) > n_words)
+            return fail(ctx, TSU_EINVAL, "pair " + std::to_string(i) + " lies outside the packed arena");
+        if (p.p_len > (1 << 26) || p.t_len > (1 << 26))
+            return fail(ctx, TSU_EINVAL, "pair " + std::to_string(i) + " longer than 2^26 bases");
+    }
+    tsu_batch* b = new (std::nothrow) tsu_batch();
+    if (!b) return fail(ctx, TSU_ENOMEM, "host allocation");
+    b->ctx = ctx;
+    b->n = n;
+    b->n_words = n_words;
+    try { b->pairs.assign(pairs, pairs + n); } catch (...) { delete b; return fail(ctx, TSU_ENOMEM, "host allocation"); }
+    for (int32_t i = 0; i < n; ++i) {
+        b->max_len = std::max(b->max_len, std::max(pairs[i].p_len, pairs[i].t_len));
+        b->cells += (double)pairs[i].p_len * (double)pairs[i].t_len;
+    }
+    hipError_t e;
+    if ((e = hipSetDevice(ctx->device)) != hipSuccess ||
+        (e = hipMalloc(reinterpret_cast<void**>(&b->d_words), (size_t)std::max<int64_t>(n_words, 1) * 4)) != hipSuccess ||
+        (e = hipMalloc(reinterpret_cast<void**>(&b->d_pairs), (size_t)std::max(n, 1) * sizeof(KPair))) != hipSuccess ||
+        (e = hipMalloc(reinterpret_cast<void**>(&b->d_scores), (size_t)std::max(n, 1) * sizeof(int32_t))) != hipSuccess) {
+        tsu_batch_free(b);
+        return hip_fail(ctx, e, "hipMalloc (batch)");
+    }
+    if ((n_words > 0 && (e = hipMemcpy(b->d_words, words, (size_t)n_words * 4, hipMemcpyHostToDevice)) != hipSuccess) ||
+        (n > 0 && (e = hipMemcpy(b->d_pairs, pairs, (size_t)n * sizeof(KPair), hipMemcpyHostToDevice)) != hipSuccess)) {
+        tsu_batch_free(b);
+        return hip_fail(ctx, e, "hipMemcpy H2D (batch)");
+    }
+    *out = b;
+    return TSU_OK;
+}
+
+extern "C" int tsu_batch_create(tsu_ctx* ctx, const uint32_t* words, int64_t n_words, const tsu_pair* pairs,
+                                int32_t n_pairs, tsu_batch** out) {
+    if (!ctx || !out || n_pairs < 0 || n_words < 0 || (n_pairs > 0 && !pairs) || (n_words > 0 && !words))
+        return fail(ctx, TSU_EINVAL, "tsu_batch_create: bad arguments");
+    return batch_upload(ctx, words, n_words, pairs, n_pairs, out);
+}
+
+extern "C" int tsu_batch_from_dataset(tsu_ctx* ctx, const tsu_dataset* ds, int64_t first, int64_t count,
+                                      tsu_batch** out) {
+    if (!ctx || !ds || !out || first < 0 || count < 0 || first + count > (int64_t)ds->p_len.size() || count > INT32_MAX)
+        return fail(ctx, TSU_EINVAL, "tsu_batch_from_dataset: bad range");
+    std::vector<uint32_t> words;
+    std::vector<tsu_pair> pairs;
+    int r = tsu_pack_dataset(ds, first, count, words, pairs);
+    if (r != TSU_OK) return fail(ctx, r, "packing failed (non-ACGT base?)");
+    return batch_upload(ctx, words.data(), (int64_t)words.size(), pairs.data(), (int32_t)count, out);
+}
+
+extern "C" int32_t tsu_batch_size(const tsu_batch* b) { return b ? b->n : 0; }
+
+extern "C" int64_t tsu_batch_bytes(const tsu_batch* b) {
+    if (!b) return 0;
+    return b->n_words * 4 + (int64_t)b->n * (int64_t)(sizeof(KPair) + sizeof(int32_t));
+}
+
+extern "C" void tsu_batch_free(tsu_batch* b) {
+    if (!b) return;
+    if (b->ctx) (void)hipSetDevice(b->ctx->device);
+    if (b->d_words) (void)hipFree(b->d_words);
+    if (b->d_pairs) (void)hipFree(b->d_pairs);
+    if (b->d_scores) (void)hipFree(b->d_scores);
+    delete b;
+}
+
+// ---------------------------------------------------------------------------
+// shapes and launches
+// ---------------------------------------------------------------------------
+namespace {
+
+KArgs base_args(const tsu_ctx* ctx) {
+    KArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.x = ctx->xs; a.o = ctx->os; a.e = ctx->es; a.g = ctx->g;
+    a.rm = ctx->rm; a.ri = ctx->ri;
+    return a;
+}
+
+int32_t seq_words_for(int32_t max_len) { return (int32_t)((max_len + 15) / 16 + 2); }
+
+// Waves per pair for a diagonal capacity: enough chunks of 64 diagonals per
+// wave to amortise the per-step bookkeeping, at most 16 waves.
+int default_waves(int32_t W) {
+    const int chunks = (W + 63) / 64;
+    int nw = 1;
+    while (nw < 16 && chunks >= nw * 2 * 4) nw *= 2;
+    return nw;
+}
+
+}  // namespace
+
+// Runs one launch of `shape` over the queue entries `order` (or pairs [0,n)
+// when order is null and the launch is plain).  Scores land in b->d_scores.
+static int run_shape(tsu_ctx* ctx, tsu_batch* b, Shape sh, int32_t W, const int32_t* d_order, int32_t n_entries,
+                     int32_t max_len, int64_t grid_cap, KArgs* extra, double* kernel_ms) {
+    KArgs a = extra ? *extra : base_args(ctx);
+    a.x = ctx->xs; a.o = ctx->os; a.e = ctx->es; a.g = ctx->g; a.rm = ctx->rm; a.ri = ctx->ri;
+    a.words = b->d_words;
+    a.pairs = b->d_pairs;
+    a.order = d_order;
+    a.n = n_entries;
+    a.scores = b->d_scores;
+    a.W = W;
+    a.seq_words = seq_words_for(max_len);
+    a.queue = ctx->d_queue;
+    const int64_t lds = tsu::lds_bytes(sh, a);
+    if (lds > ctx->max_lds) return fail(ctx, TSU_EINVAL, "shape does not fit in LDS");
+    int64_t grid = n_entries;
+    DevBuf<uint8_t> ring;
+    if (sh.queue) {
+        int per_cu = 0;
+        int r = tsu::occupancy(sh, lds, &per_cu);
+        if (r != 0) return hip_fail(ctx, (hipError_t)r, "occupancy query");
+        if (per_cu <= 0) return fail(ctx, TSU_EINVAL, "shape has zero occupancy");
+        grid = std::min<int64_t>((int64_t)per_cu * ctx->cu_count, n_entries);
+        if (grid_cap > 0) grid = std::min(grid, grid_cap);
+        HIPCHK(hipMemsetAsync(ctx->d_queue, 0, sizeof(int32_t), ctx->stream), "hipMemsetAsync (queue)");
+        if (!sh.lds_ring) {
+            a.ring_stride = tsu::ring_bytes(sh, W, ctx->rm, ctx->ri);
+            HIPCHK(ring.alloc((size_t)(a.ring_stride * std::max<int64_t>(grid, 1))), "hipMalloc (ring)");
+            a.ring_g = ring.p;
+        }
+    } else if (!sh.lds_ring || sh.bt) {
+        return fail(ctx, TSU_EINVAL, "plain launches keep the ring in LDS");
+    }
+    if (grid <= 0) { if (kernel_ms) *kernel_ms = 0; return TSU_OK; }
+    HIPCHK(hipEventRecord(ctx->ev0, ctx->stream), "hipEventRecord");
+    int r = tsu::launch_wfa(sh, a, grid, ctx->stream);
+    if (r != 0) return hip_fail(ctx, (hipError_t)r, "kernel launch");
+    HIPCHK(hipEventRecord(ctx->ev1, ctx->stream), "hipEventRecord");
+    HIPCHK(hipEventSynchronize(ctx->ev1), "hipEventSynchronize");
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1), "hipEventElapsedTime");
+    if (kernel_ms) *kernel_ms = ms;
+    return TSU_OK;
+}
+
+static Shape plain_shape(const tsu_ctx* ctx, int32_t W, int32_t max_len, bool* fits) {
+    Shape sh{default_waves(W), max_len <= 32000, true, false, false};
+    KArgs a = base_args(ctx);
+    a.W = W;
+    a.seq_words = seq_words_for(max_len);
+    *fits = tsu::lds_bytes(sh, a) <= ctx->max_lds;
+    return sh;
+}
+
+extern "C" int tsu_align_plain(tsu_ctx* ctx, tsu_batch* b, int32_t wf_length, int32_t* scores, tsu_timing* t) {
+    if (!ctx || !b || !scores || b->ctx != ctx) return fail(ctx, TSU_EINVAL, "tsu_align_plain: bad arguments");
+    if (wf_length < 8) wf_length = 8;
+    // a capacity beyond p+t+3 can never be used
+    int32_t W = std::min<int64_t>(wf_length, 2ll * b->max_len + 3);
+    W = std::max(W, 8);
+    HIPCHK(hipSetDevice(ctx->device), "hipSetDevice");
+    const double w0 = now_ms();
+    bool fits = false;
+    Shape sh = plain_shape(ctx, W, b->max_len, &fits);
+    double kms = 0;
+    int r;
+    if (fits) {
+        r = run_shape(ctx, b, sh, W, nullptr, b->n, b->max_len, 0, nullptr, &kms);
+    } else {
+        // ring too wide for LDS: same one-workgroup-per-pair work, ring in HBM,
+        // resident workgroups fed from the queue
+        sh.queue = true;
+        sh.lds_ring = false;
+        sh.nw = 16;
+        r = run_shape(ctx, b, sh, W, nullptr, b->n, b->max_len, 0, nullptr, &kms);
+    }
+    if (r != TSU_OK) return r;
+    const double w1 = now_ms();
+    HIPCHK(hipMemcpy(scores, b->d_scores, (size_t)b->n * sizeof(int32_t), hipMemcpyDeviceToHost), "hipMemcpy D2H (scores)");
+    const double w2 = now_ms();
+    if (t) {
+        std::memset(t, 0, sizeof(*t));
+        t->kernel_ms = kms; t->wall_ms = w1 - w0; t->d2h_ms = w2 - w1;
+        t->pairs = b->n; t->cells = b->cells; t->launches = 1;
+    }
+    for (int32_t i = 0; i < b->n; ++i) {
+        if (scores[i] == TSU_EKOVERFLOW) return fail(ctx, TSU_EKOVERFLOW, "pair " + std::to_string(i) + ": diagonal range exceeded wf_length " + std::to_string(wf_length));
+    }
+    for (int32_t i = 0; i < b->n; ++i) {
+        if (scores[i] < 0) return fail(ctx, scores[i], "pair " + std::to_string(i) + ": " + tsu_strerror(scores[i]));
+    }
+    return TSU_OK;
+}
+
+extern "C" int tsu_align_fixed(tsu_ctx* ctx, const char* patterns, const char* texts, int32_t num_couples,
+                               int32_t seq_len, int32_t wf_length, int32_t* scores, tsu_timing* t) {
+    if (!ctx || num_couples < 0 || seq_len < 0 || (num_couples > 0 && (!patterns || !texts || !scores)))
+        return fail(ctx, TSU_EINVAL, "tsu_align_fixed: bad arguments");
+    const int64_t wps = tsu_packed_words(seq_len);
+    std::vector<uint32_t> words;
+    std::vector<tsu_pair> pairs((size_t)num_couples);
+    try { words.assign((size_t)(2 * wps * (int64_t)num_couples), 0u); } catch (...) { return fail(ctx, TSU_ENOMEM, "host allocation"); }
+    int bad = 0;
+#pragma omp parallel for schedule(static, 1024) reduction(+:bad)
+    for (int32_t i = 0; i < num_couples; ++i) {
+        tsu_pair& pr = pairs[(size_t)i];
+        pr.p_off = 2 * wps * (int64_t)i; pr.t_off = pr.p_off + wps;
+        pr.p_len = seq_len; pr.t_len = seq_len;
+        if (tsu_pack_seq(patterns + (int64_t)i * seq_len, seq_len, words.data() + pr.p_off) != TSU_OK) bad++;
+        if (tsu_pack_seq(texts + (int64_t)i * seq_len, seq_len, words.data() + pr.t_off) != TSU_OK) bad++;
+    }
+    if (bad) return fail(ctx, TSU_EBADINPUT, "non-ACGT base in input");
+    const double h0 = now_ms();
+    tsu_batch* b = nullptr;
+    int r = batch_upload(ctx, words.data(), (int64_t)words.size(), pairs.data(), num_couples, &b);
+    if (r != TSU_OK) return r;
+    const double h1 = now_ms();
+    r = tsu_align_plain(ctx, b, wf_length, scores, t);
+    if (t) t->h2d_ms = h1 - h0;
+    tsu_batch_free(b);
+    return r;
+}
+
+// ---------------------------------------------------------------------------
+// batched form: buckets, persistent queue launches, overflow widening
+// ---------------------------------------------------------------------------
+namespace {
+
+struct Bucket {
+    int32_t bound;                // max(p, t) of the bucket
+    std::vector<int32_t> idx;     // pair indices, longest first
+};
+
+// Bucket bounds on max(p, t).
+const int32_t kBucketBounds[] = {128, 256, 512, 1024, 2048, 4096, 8192, 16384, 32000, 1 << 26};
+
+// Initial capacity for a bucket: enough for about 15% divergence at the
+// bucket's length (score ~ 6.7 * 0.15 * L for 4/6/2 mixed edits), clamped to
+// what the lengths allow.  Pairs that overflow are re-run wider.
+int32_t initial_capacity(const tsu_ctx* ctx, int32_t bound) {
+    const double per_base = 0.15 * (double)std::max(ctx->xs, ctx->os + ctx->es);
+    const int64_t s_est = (int64_t)std::ceil(per_base * bound) + ctx->os + ctx->es;
+    int32_t W = capacity_for_score(ctx, s_est);
+    W = std::min<int32_t>(W, capacity_full(bound, bound));
+    return std::max<int32_t>(W, 16);
+}
+
+Shape batched_shape(const tsu_ctx* ctx, int32_t W, int32_t max_len, int nw_hint) {
+    Shape sh{nw_hint > 0 ? nw_hint : default_waves(W), max_len <= 32000, true, false, true};
+    KArgs a = base_args(ctx);
+    a.W = W;
+    a.seq_words = seq_words_for(max_len);
+    if (tsu::lds_bytes(sh, a) > ctx->max_lds) {
+        sh.lds_ring = false;
+        if (nw_hint <= 0) sh.nw = 16;
+    }
+    return sh;
+}
+
+}  // namespace
+
+static const tsu_tune_entry* tune_lookup(const tsu_ctx* ctx, int32_t bound) {
+    for (const tsu_tune_entry& te : ctx->tune)
+        if (te.bucket_bound == bound) return &te;
+    return nullptr;
+}
+
+// Aligns the pairs listed in `sel` (pair indices) with per-bucket persistent
+// launches, widening the capacity of pairs that overflow.  Scores are left in
+// b->d_scores and copied to `scores` (full batch order) for the selected pairs.
+static int batched_core(tsu_ctx* ctx, tsu_batch* b, const std::vector<int32_t>& sel, int32_t* scores,
+                        tsu_timing* t, bool use_tune) {
+    const int nb = (int)(sizeof(kBucketBounds) / sizeof(kBucketBounds[0]));
+    std::vector<Bucket> buckets((size_t)nb);
+    for (int i = 0; i < nb; ++i) buckets[(size_t)i].bound = kBucketBounds[i];
+    for (int32_t i : sel) {
+        const tsu_pair& pr = b->pairs[(size_t)i];
+        const int32_t L = std::max(pr.p_len, pr.t_len);
+        int bi = 0;
+        while (bi < nb - 1 && L > kBucketBounds[bi]) ++bi;
+        buckets[(size_t)bi].idx.push_back(i);
+    }
+    DevBuf<int32_t> d_order;
+    HIPCHK(d_order.alloc(std::max<size_t>(sel.size(), 1)), "hipMalloc (order)");
+    double kms_total = 0;
+    int launches = 0, retried = 0;
+    // longest buckets first: their tails overlap nothing, so start them early
+    for (int bi = nb - 1; bi >= 0; --bi) {
+        Bucket& bk = buckets[(size_t)bi];
+        if (bk.idx.empty()) continue;
+        std::stable_sort(bk.idx.begin(), bk.idx.end(), [&](int32_t a1, int32_t a2) {
+            const tsu_pair& x1 = b->pairs[(size_t)a1];
+            const tsu_pair& x2 = b->pairs[(size_t)a2];
+            return (int64_t)x1.p_len * x1.t_len > (int64_t)x2.p_len * x2.t_len;
+        });
+        int32_t max_len = 0;
+        for (int32_t i : bk.idx) max_len = std::max(max_len, std::max(b->pairs[(size_t)i].p_len, b->pairs[(size_t)i].t_len));
+        const tsu_tune_entry* te = use_tune ? tune_lookup(ctx, bk.bound) : nullptr;
+        int32_t W = (te && te->capacity > 0) ? te->capacity : initial_capacity(ctx, std::min(bk.bound, max_len));
+        const int nw_hint = te ? te->waves_per_pair : 0;
+        std::vector<int32_t> todo = bk.idx;
+        for (int attempt = 0; !todo.empty(); ++attempt) {
+            W = std::min<int32_t>(W, capacity_full(max_len, max_len));
+            Shape sh = batched_shape(ctx, W, max_len, nw_hint);
+            HIPCHK(hipMemcpyAsync(d_order.p, todo.data(), todo.size() * sizeof(int32_t), hipMemcpyHostToDevice, ctx->stream),
+                   "hipMemcpyAsync (order)");
+            double kms = 0;
+            int r = run_shape(ctx, b, sh, W, d_order.p, (int32_t)todo.size(), max_len, 0, nullptr, &kms);
+            if (r != TSU_OK) return r;
+            kms_total += kms;
+            ++launches;
+            // scores live at their pair index
+            std::vector<int32_t> tmp((size_t)b->n);
+            HIPCHK(hipMemcpy(tmp.data(), b->d_scores, (size_t)b->n * sizeof(int32_t), hipMemcpyDeviceToHost), "hipMemcpy D2H (scores)");
+            std::vector<int32_t> again;
+            for (int32_t i : todo) {
+                scores[i] = tmp[(size_t)i];
+                if (tmp[(size_t)i] == TSU_EKOVERFLOW) again.push_back(i);
+            }
+            if (!again.empty()) {
+                if (W >= capacity_full(max_len, max_len)) return fail(ctx, TSU_EINTERNAL, "overflow at full capacity");
+                retried += (int)again.size();
+                W = (int32_t)std::min<int64_t>(2ll * W, capacity_full(max_len, max_len));
+            }
+            todo.swap(again);
+        }
+    }
+    if (t) { t->kernel_ms += kms_total; t->launches += launches; t->retried += retried; }
+    return TSU_OK;
+}
+
+extern "C" int tsu_align_batched(tsu_ctx* ctx, tsu_batch* b, int32_t* scores, tsu_timing* t) {
+    if (!ctx || !b || !scores || b->ctx != ctx) return fail(ctx, TSU_EINVAL, "tsu_align_batched: bad arguments");
+    HIPCHK(hipSetDevice(ctx->device), "hipSetDevice");
+    if (t) std::memset(t, 0, sizeof(*t));
+    std::vector<int32_t> sel((size_t)b->n);
+    std::iota(sel.begin(), sel.end(), 0);
+    const double w0 = now_ms();
+    int r = batched_core(ctx, b, sel, scores, t, true);
+    const double w1 = now_ms();
+    if (t) { t->wall_ms = w1 - w0; t->pairs = b->n; t->cells = b->cells; }
+    if (r != TSU_OK) return r;
+    for (int32_t i = 0; i < b->n; ++i)
+        if (scores[i] < 0) return fail(ctx, scores[i], "pair " + std::to_string(i) + ": " + tsu_strerror(scores[i]));
+    return TSU_OK;
+}
+
+// Autotuner: for each populated bucket, time the candidate waves-per-pair on
+// a sample of the bucket and keep the fastest.
+extern "C" int tsu_autotune(tsu_ctx* ctx, tsu_batch* b, int32_t sample_pairs) {
+    if (!ctx || !b || b->ctx != ctx) return fail(ctx, TSU_EINVAL, "tsu_autotune: bad arguments");
+    HIPCHK(hipSetDevice(ctx->device), "hipSetDevice");
+    if (sample_pairs <= 0) sample_pairs = 4096;
+    const int nb = (int)(sizeof(kBucketBounds) / sizeof(kBucketBounds[0]));
+    std::vector<std::vector<int32_t>> members((size_t)nb);
+    for (int32_t i = 0; i < b->n; ++i) {
+        const int32_t L = std::max(b->pairs[(size_t)i].p_len, b->pairs[(size_t)i].t_len);
+        int bi = 0;
+        while (bi < nb - 1 && L > kBucketBounds[bi]) ++bi;
+        members[(size_t)bi].push_back(i);
+    }
+    std::vector<tsu_tune_entry> table;
+    std::vector<int32_t> scratch((size_t)b->n);
+    for (int bi = 0; bi < nb; ++bi) {
+        std::vector<int32_t>& m = members[(size_t)bi];
+        if (m.empty()) continue;
+        // deterministic stride sample
+        std::vector<int32_t> sample;
+        const size_t step = std::max<size_t>(1, m.size() / (size_t)sample_pairs);
+        for (size_t j = 0; j < m.size() && sample.size() < (size_t)sample_pairs; j += step) sample.push_back(m[j]);
+        double best = 1e300;
+        int best_nw = 0;
+        for (int nw : {1, 2, 4, 8, 16}) {
+            ctx->tune.clear();
+            tsu_tune_entry probe{nw, 0, kBucketBounds[bi], 0};
+            ctx->tune.push_back(probe);
+            tsu_timing tt;
+            std::memset(&tt, 0, sizeof(tt));
+            int r = batched_core(ctx, b, sample, scratch.data(), &tt, true);
+            if (r != TSU_OK) continue;
+            if (tt.kernel_ms < best) { best = tt.kernel_ms; best_nw = nw; }
+        }
+        if (best_nw > 0) table.push_back(tsu_tune_entry{best_nw, 0, kBucketBounds[bi], 0});
+    }
+    ctx->tune = table;
+    return TSU_OK;
+}
+
+extern "C" int tsu_tune_table(const tsu_ctx* ctx, tsu_tune_entry* entries, int32_t cap, int32_t* n) {
+    if (!ctx || !n) return TSU_EINVAL;
+    *n = (int32_t)ctx->tune.size();
+    for (int32_t i = 0; i < *n && i < cap; ++i) entries[i] = ctx->tune[(size_t)i];
+    return TSU_OK;
+}
+
+// ---------------------------------------------------------------------------
+// CIGAR form
+// ---------------------------------------------------------------------------
+extern "C" int tsu_align_cigar(tsu_ctx* ctx, tsu_batch* b, int32_t* scores, tsu_cigars** out, tsu_timing* t) {
+    if (!ctx || !b || !scores || !out || b->ctx != ctx) return fail(ctx, TSU_EINVAL, "tsu_align_cigar: bad arguments");
+    *out = nullptr;
+    HIPCHK(hipSetDevice(ctx->device), "hipSetDevice");
+    if (t) std::memset(t, 0, sizeof(*t));
+    const double w0 = now_ms();
+    // 1) scores first (batched form): they size the backtrace buffers exactly
+    std::vector<int32_t> sel((size_t)b->n);
+    std::iota(sel.begin(), sel.end(), 0);
+    int r = batched_core(ctx, b, sel, scores, t, true);
+    if (r != TSU_OK) return r;
+    for (int32_t i = 0; i < b->n; ++i)
+        if (scores[i] < 0) return fail(ctx, scores[i], "pair " + std::to_string(i) + ": " + tsu_strerror(scores[i]));
+    tsu_cigars* cg = new (std::nothrow) tsu_cigars();
+    if (!cg) return fail(ctx, TSU_ENOMEM, "host allocation");
+    cg->off.assign((size_t)b->n, 0);
+    cg->n.assign((size_t)b->n, 0);
+    // 2) per pair: steps = s/g + 1, records <= sum of widths <= steps * W(s)
+    const int32_t unit = std::min(ctx->xs, ctx->os + ctx->es);
+    std::vector<int32_t> order((size_t)b->n);
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](int32_t a1, int32_t a2) { return scores[a1] > scores[a2]; });
+    int64_t runs_total = 0;
+    for (int32_t i = 0; i < b->n; ++i) runs_total += 2ll * (scores[i] / ctx->g / unit) + 2;
+    DevBuf<uint32_t> d_runs;
+    DevBuf<unsigned long long> d_used;
+    DevBuf<int64_t> d_run_off;
+    DevBuf<int32_t> d_run_n;
+    DevBuf<int32_t> d_order;
+    HIPCHK(d_runs.alloc((size_t)std::max<int64_t>(runs_total, 1)), "hipMalloc (runs)");
+    HIPCHK(d_used.alloc(1), "hipMalloc");
+    HIPCHK(d_run_off.alloc((size_t)std::max(b->n, 1)), "hipMalloc");
+    HIPCHK(d_run_n.alloc((size_t)std::max(b->n, 1)), "hipMalloc");
+    HIPCHK(d_order.alloc((size_t)std::max(b->n, 1)), "hipMalloc");
+    HIPCHK(hipMemsetAsync(d_used.p, 0, sizeof(unsigned long long), ctx->stream), "hipMemsetAsync");
+    // group pairs into launches of similar score so buffers fit the widest
+    size_t pos = 0;
+    double kms_total = 0;
+    int launches = 0;
+    size_t free_b = 0, total_b = 0;
+    HIPCHK(hipMemGetInfo(&free_b, &total_b), "hipMemGetInfo");
+    const int64_t budget = (int64_t)(0.5 * (double)free_b);
+    while (pos < order.size()) {
+        const int32_t s0 = scores[order[pos]] / ctx->g;  // widest in this group (sorted desc)
+        int32_t max_len = 0;
+        size_t end = pos;
+        // all pairs with score within this group share W and buffers
+        const int32_t W = std::max<int32_t>(capacity_for_score(ctx, s0), 16);
+        const int32_t steps = s0 + 2;
+        int64_t recs = 0;
+        for (int64_t s = 0; s <= s0; ++s)
+            recs += std::min<int64_t>(W, 2 * (std::max<int64_t>(0, s - ctx->os) / ctx->es + 2) + 1);
+        recs += 64;
+        Shape sh{default_waves(W), true, true, true, true};
+        KArgs a = base_args(ctx);
+        for (end = pos; end < order.size(); ++end) {
+            const tsu_pair& pr = b->pairs[(size_t)order[end]];
+            max_len = std::max(max_len, std::max(pr.p_len, pr.t_len));
+            if (end - pos >= 65536) break;
+        }
+        sh.off16 = max_len <= 32000;
+        a.W = W;
+        a.seq_words = seq_words_for(max_len);
+        if (tsu::lds_bytes(sh, a) > ctx->max_lds) { sh.lds_ring = false; sh.nw = 16; }
+        int per_cu = 0;
+        int rr = tsu::occupancy(sh, tsu::lds_bytes(sh, a), &per_cu);
+        if (rr != 0 || per_cu <= 0) { tsu_cigars_free(cg); return fail(ctx, TSU_EINTERNAL, "backtrace shape has no occupancy"); }
+        const int64_t per_wg = recs * 4 + (int64_t)steps * 12 + (sh.lds_ring ? 0 : tsu::ring_bytes(sh, W, ctx->rm, ctx->ri));
+        int64_t grid = std::min<int64_t>((int64_t)per_cu * ctx->cu_count, (int64_t)(end - pos));
+        grid = std::max<int64_t>(1, std::min<int64_t>(grid, budget / std::max<int64_t>(per_wg, 1)));
+        DevBuf<uint32_t> d_rec;
+        DevBuf<int32_t> d_slo;
+        DevBuf<int64_t> d_soff;
+        HIPCHK(d_rec.alloc((size_t)(recs * grid)), "hipMalloc (records)");
+        HIPCHK(d_slo.alloc((size_t)((int64_t)steps * grid)), "hipMalloc (steps)");
+        HIPCHK(d_soff.alloc((size_t)((int64_t)steps * grid)), "hipMalloc (steps)");
+        std::vector<int32_t> ids(order.begin() + (ptrdiff_t)pos, order.begin() + (ptrdiff_t)end);
+        HIPCHK(hipMemcpyAsync(d_order.p, ids.data(), ids.size() * sizeof(int32_t), hipMemcpyHostToDevice, ctx->stream), "hipMemcpyAsync");
+        a.rec_g = d_rec.p; a.rec_cap = recs;
+        a.step_lo_g = d_slo.p; a.step_off_g = d_soff.p; a.step_cap = steps;
+        a.runs = d_runs.p; a.runs_cap = std::max<int64_t>(runs_total, 1); a.runs_used = d_used.p;
+        a.run_off = d_run_off.p; a.run_n = d_run_n.p;
+        double kms = 0;
+        r = run_shape(ctx, b, sh, W, d_order.p, (int32_t)ids.size(), max_len, grid, &a, &kms);
+        if (r != TSU_OK) { tsu_cigars_free(cg); return r; }
+        kms_total += kms;
+        ++launches;
+        pos = end;
+    }
+    std::vector<int32_t> sc2((size_t)b->n);
+    HIPCHK(hipMemcpy(sc2.data(), b->d_scores, (size_t)b->n * sizeof(int32_t), hipMemcpyDeviceToHost), "hipMemcpy D2H");
+    unsigned long long used = 0;
+    HIPCHK(hipMemcpy(&used, d_used.p, sizeof(used), hipMemcpyDeviceToHost), "hipMemcpy D2H");
+    try { cg->runs.resize((size_t)std::max<unsigned long long>(used, 1)); } catch (...) { tsu_cigars_free(cg); return fail(ctx, TSU_ENOMEM, "host allocation"); }
+    if (used) HIPCHK(hipMemcpy(cg->runs.data(), d_runs.p, (size_t)used * 4, hipMemcpyDeviceToHost), "hipMemcpy D2H (runs)");
+    if (b->n) {
+        HIPCHK(hipMemcpy(cg->off.data(), d_run_off.p, (size_t)b->n * 8, hipMemcpyDeviceToHost), "hipMemcpy D2H");
+        HIPCHK(hipMemcpy(cg->n.data(), d_run_n.p, (size_t)b->n * 4, hipMemcpyDeviceToHost), "hipMemcpy D2H");
+    }
+    const double w1 = now_ms();
+    if (t) { t->kernel_ms += kms_total; t->launches += launches; t->wall_ms = w1 - w0; t->pairs = b->n; t->cells = b->cells; }
+    for (int32_t i = 0; i < b->n; ++i) {
+        if (sc2[(size_t)i] != scores[i]) {
+            const int st = sc2[(size_t)i] < 0 ? sc2[(size_t)i] : TSU_EINTERNAL;
+            tsu_cigars_free(cg);
+            return fail(ctx, st, "pair " + std::to_string(i) + ": backtrace pass returned " + std::to_string(sc2[(size_t)i]) +
+                                     " (score pass " + std::to_string(scores[i]) + ")");
+        }
+    }
+    *out = cg;
+    return TSU_OK;
+}
+
+extern "C" int tsu_cigars_get(const tsu_cigars* c, int64_t i, const uint32_t** runs, int32_t* n_runs) {
+    if (!c || i < 0 || i >= (int64_t)c->n.size()) return TSU_EINVAL;
+    if (runs) *runs = c->runs.data() + c->off[(size_t)i];
+    if (n_runs) *n_runs = c->n[(size_t)i];
+    return TSU_OK;
+}
+
+extern "C" int tsu_cigars_string(const tsu_cigars* c, int64_t i, char* buf, int32_t cap) {
+    if (!c || i < 0 || i >= (int64_t)c->n.size()) return TSU_EINVAL;
+    static const char ops[4] = {'M', 'X', 'I', 'D'};
+    std::string s;
+    const uint32_t* r = c->runs.data() + c->off[(size_t)i];
+    for (int32_t j = 0; j < c->n[(size_t)i]; ++j) {
+        s += std::to_string(r[j] >> 2);
+        s += ops[r[j] & 3u];
+    }
+    if (buf && cap > 0) std::snprintf(buf, (size_t)cap, "%s", s.c_str());
+    return (int)s.size();
+}
+
+extern "C" void tsu_cigars_free(tsu_cigars* c) { delete c; }
+
+// ---------------------------------------------------------------------------
+// calibration
+// ---------------------------------------------------------------------------
+extern "C" int tsu_calibrate_copy(tsu_ctx* ctx, int64_t bytes, double* kernel_ms) {
+    if (!ctx || bytes <= 0) return TSU_EINVAL;
+    HIPCHK(hipSetDevice(ctx->device), "hipSetDevice");
+    bytes = bytes / 16 * 16;
+    DevBuf<uint8_t> src, dst;
+    HIPCHK(src.alloc((size_t)bytes), "hipMalloc");
+    HIPCHK(dst.alloc((size_t)bytes), "hipMalloc");
+    HIPCHK(hipMemsetAsync(src.p, 1, (size_t)bytes, ctx->stream), "hipMemsetAsync");
+    HIPCHK(hipEventRecord(ctx->ev0, ctx->stream), "hipEventRecord");
+    int r = tsu::launch_copy(src.p, dst.p, bytes, ctx->stream);
+    if (r != 0) return hip_fail(ctx, (hipError_t)r, "copy launch");
+    HIPCHK(hipEventRecord(ctx->ev1, ctx->stream), "hipEventRecord");
+    HIPCHK(hipEventSynchronize(ctx->ev1), "hipEventSynchronize");
+    float ms = 0;
+    HIPCHK(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1), "hipEventElapsedTime");
+    if (kernel_ms) *kernel_ms = ms;
+    return TSU_OK;
+}

@@ -1,0 +1,86 @@
+// tsu_internal.h -- structures shared by the host side of libtsunami_hip.so
+// and the launchers of the HIP kernels.
+#pragma once
+
+#include <stdint.h>
+#include <string>
+#include <vector>
+
+#include "../../include/tsunami_wfa.h"
+
+struct tsu_dataset {
+    std::vector<char> chars;
+    std::vector<int64_t> p_off, t_off;
+    std::vector<int32_t> p_len, t_len;
+    int32_t hdr_num = 0, hdr_len = 0, hdr_err = 0;
+};
+
+int tsu_pack_seq(const char* seq, int32_t len, uint32_t* out);
+bool tsu_is_acgt(const char* seq, int64_t len, int64_t* bad_pos);
+int tsu_pack_dataset(const tsu_dataset* ds, int64_t first, int64_t count, std::vector<uint32_t>& words,
+                     std::vector<tsu_pair>& pairs);
+
+namespace tsu {
+
+// Kernel-side view of a pair (same layout as tsu_pair).
+struct KPair {
+    int64_t p_off;
+    int64_t t_off;
+    int32_t p_len;
+    int32_t t_len;
+};
+static_assert(sizeof(KPair) == sizeof(tsu_pair), "KPair must mirror tsu_pair");
+
+// Arguments of every alignment kernel.
+struct KArgs {
+    const uint32_t* words;     // packed arena (HBM)
+    const KPair* pairs;        // descriptors (HBM)
+    const int32_t* order;      // queue entry -> pair index (nullable: identity)
+    int32_t n;                 // queue entries / pairs
+    int32_t* scores;           // per pair: score (unscaled) or negative status
+    int32_t x, o, e, g;        // penalties divided by g = gcd(x, o, e)
+    int32_t W;                 // diagonal capacity of a ring slot
+    int32_t seq_words;         // LDS words reserved per sequence
+    int32_t rm, ri;            // ring depths: M keeps max(x, o+e)+1 scores, I/D keep e+1
+    int32_t* queue;            // work counter (persistent launches)
+    // ring in global memory (when it does not fit in LDS): one region per workgroup
+    uint8_t* ring_g;
+    int64_t ring_stride;       // bytes per workgroup
+    // backtrace (CIGAR) state: one region per workgroup
+    uint32_t* rec_g;           // per-cell records
+    int64_t rec_cap;           // records per workgroup
+    int32_t* step_lo_g;        // per score step: lowest diagonal recorded
+    int64_t* step_off_g;       // per score step: offset of its first record
+    int32_t step_cap;          // score steps per workgroup
+    uint32_t* runs;            // CIGAR run pool
+    int64_t runs_cap;
+    unsigned long long* runs_used;
+    int64_t* run_off;          // per pair
+    int32_t* run_n;            // per pair
+};
+
+struct Shape {
+    int nw;          // waves per pair (1, 2, 4, 8, 16)
+    bool off16;      // int16 offsets in the ring
+    bool lds_ring;   // ring in LDS (else global memory)
+    bool bt;         // record backtrace + emit CIGAR
+    bool queue;      // persistent launch fed by a work counter
+};
+
+// LDS bytes a launch of this shape needs.
+int64_t lds_bytes(const Shape& sh, const KArgs& a);
+// Bytes of one ring (all slots) for W diagonals.
+int64_t ring_bytes(const Shape& sh, int32_t W, int32_t rm, int32_t ri);
+int32_t ring_stride_entries(const Shape& sh, int32_t W);
+// Launches the kernel of the given shape on stream with grid blocks.
+// Returns a hipError_t value (0 on success).
+int launch_wfa(const Shape& sh, const KArgs& a, int64_t grid, void* stream);
+// Streaming copy used to calibrate HBM byte counters.
+int launch_copy(const void* src, void* dst, int64_t bytes, void* stream);
+// Max dynamic LDS bytes the kernel of this shape may use (after raising the
+// attribute); 0 on error.
+int64_t max_lds_for(const Shape& sh);
+// Resident workgroups of this shape per CU for a given LDS size.
+int occupancy(const Shape& sh, int64_t lds, int* blocks_per_cu);
+
+}  // namespace tsu
