@@ -1,0 +1,186 @@
+"""Benchmark of the gap-affine WFA hot path on one MI355X.
+
+Default workload (the configuration the headline metric is quoted on):
+1,000,000 pairs of 1000 bp, text = pattern with 50 mixed edits (substitution /
+insertion / deletion, 1/3 each), penalties 4/6/2, plain form (one workgroup
+per pair), sequences 2-bit packed and resident in HBM before timing starts.
+
+A step is one pass of the plain kernel over all pairs plus the copy of the
+int32 scores to the host.  value = pairs * steps / wall time of the timed
+steps (alignments/s).  The kernel's own duration comes from HIP events
+recorded on the stream it runs on; the roofline figure divides the
+algorithmic HBM bytes of a launch (2-bit sequences read once + 4-byte score
+written: 8*(ceil(p/32)+ceil(t/32))+4 per pair) by that duration.
+
+The CPU baseline is the oracle's exact WFA (C, OpenMP over pairs, every host
+thread), timed on the first pairs of the same workload, and its scores double
+as a bit-exactness check of the timed GPU run.
+
+    python bench.py [--steps K] [--warmup W] [--form plain|batched|cigar|served]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, REPO)
+
+SEED = 0x5453554E
+HBM_PEAK_GBS = 8000.0
+PEN = (4, 6, 2)
+METRIC = "alignments/sec + equiv. GCUPS at 1 GPU, bit-exact; p50/p99 latency when served"
+
+WORKLOADS = {
+    # name: (pairs, len_min, len_max, err_min, err_max, seed offset)
+    "cfg1": (10_000, 100, 100, 0.02, 0.02, 1),
+    "cfg2": (1_000_000, 1000, 1000, 0.05, 0.05, 2),
+    "cfg3": (50_000, 10_000, 10_000, 0.10, 0.10, 3),
+    "cfg4": (500_000, 100, 20_000, 0.01, 0.15, 4),
+}
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def algorithmic_bytes(pl, tl):
+    return int((8 * ((pl.astype(np.int64) + 31) // 32 + (tl.astype(np.int64) + 31) // 32) + 4).sum())
+
+
+def cpu_baseline(pairs_fn, n_total, target_cpu_s=20.0):
+    """Oracle WFA on the first pairs of the workload, every host thread."""
+    from oracle import oracle
+    threads = oracle.max_threads()
+    n = min(n_total, 2000)
+    t0 = time.perf_counter()
+    sample = pairs_fn(0, n)
+    scores, _, _ = oracle.wfa_batch(sample, *PEN, threads=threads)
+    dt = time.perf_counter() - t0
+    # grow the sample to about target_cpu_s of CPU work (threads x wall)
+    per_pair = dt * threads / max(n, 1)
+    n2 = int(min(n_total, max(n, target_cpu_s / max(per_pair, 1e-9))))
+    if n2 > n:
+        sample = pairs_fn(0, n2)
+        t0 = time.perf_counter()
+        scores, _, cnt = oracle.wfa_batch(sample, *PEN, threads=threads)
+        dt = time.perf_counter() - t0
+        n = n2
+    return {"value": n / dt, "unit": "alignments/s", "cores": threads, "kind": "port",
+            "sample": f"first {n} pairs of the workload, oracle exact WFA (C, OpenMP, {threads} threads), "
+                      f"{dt:.2f} s wall"}, scores
+
+
+def read_traffic(name):
+    path = os.path.join(REPO, "profiles", f"traffic_{name}.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f).get("bytes_per_launch")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--form", default="plain", choices=["plain", "batched", "cigar"])
+    ap.add_argument("--workload", default="cfg2", choices=sorted(WORKLOADS))
+    ap.add_argument("--pairs", type=int, default=0, help="override the pair count (testing only)")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
+              "(pairs are independent; a multi-GPU run would shard them by estimated work)", file=sys.stderr)
+        sys.exit(2)
+
+    import tsunami_pact_amd as tsu
+
+    n, lmin, lmax, emin, emax, soff = WORKLOADS[args.workload]
+    if args.pairs:
+        n = args.pairs
+    log(f"generating {n} pairs ({args.workload})")
+    ds = tsu.Dataset.generate(n, seed=SEED + soff, len_min=lmin, len_max=lmax, err_min=emin, err_max=emax)
+    pl, tl = ds.lengths()
+    x, o, e = PEN
+    # capacity from the score bound: <= round(rate*L) edits of cost <= o+e
+    edits = int(round(emax * lmax))
+    s_bound = edits * (o + e)
+    W = 2 * ((s_bound - o) // e) + 3
+
+    a = tsu.Aligner(*PEN)
+    info = a.device_info()
+    log(f"device {info}")
+    b = a.batch(ds)
+    log(f"batch resident: {b.nbytes / 1e6:.1f} MB")
+
+    def step():
+        if args.form == "plain":
+            return a.align_plain(b, W)
+        if args.form == "batched":
+            return a.align_batched(b)
+        s, _, t = a.align_cigar(b)
+        return s, t
+
+    for i in range(args.warmup):
+        scores, _ = step()
+        log(f"warmup {i} done")
+    a.sync()
+    kms = []
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        scores, t = step()
+        kms.append(t["kernel_ms"])
+        log(f"step {i}: kernel {t['kernel_ms']:.3f} ms")
+    a.sync()
+    wall = time.perf_counter() - t0
+
+    value = n * args.steps / wall
+    cells = float((pl.astype(np.float64) * tl).sum())
+    kernel_ms = float(np.mean(kms))
+    algo_bytes = algorithmic_bytes(pl, tl)
+    achieved = algo_bytes / (kernel_ms / 1e3) / 1e9
+    traffic = read_traffic(f"wfa_{args.form}_{args.workload}")
+
+    cpu = None
+    exact = None
+    if not args.no_cpu:
+        log("cpu baseline")
+        cpu, cpu_scores = cpu_baseline(lambda s, m: [ds.pair(i) for i in range(s, s + m)], n)
+        m = len(cpu_scores)
+        exact = bool((scores[:m] == cpu_scores).all())
+
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "alignments/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int32",
+        "data": "synthetic",
+        "config": {"workload": f"{args.workload}: {n} pairs, length "
+                               f"{lmin if lmin == lmax else f'{lmin}-{lmax} log-uniform'} bp, "
+                               f"{emin * 100:g}{'' if emin == emax else f'-{emax * 100:g}'}% mixed edits, "
+                               f"penalties 4/6/2, {args.form} form",
+                   "pairs": n, "form": args.form, "wf_length": W, "penalties": list(PEN), "seed": SEED + soff},
+        "gcups": cells / (wall / args.steps) / 1e9,
+        "kernel_ms": kernel_ms,
+        "bit_exact_sample": exact,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic},
+        "cpu_baseline": cpu,
+        "device": info["name"],
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

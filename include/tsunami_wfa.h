@@ -136,6 +136,8 @@ int tsu_dataset_from_strings(const char* const* patterns, const int32_t* p_len,
 int64_t tsu_dataset_size(const tsu_dataset* ds);
 int tsu_dataset_pair(const tsu_dataset* ds, int64_t i, const char** pattern, int32_t* p_len,
                      const char** text, int32_t* t_len);
+/* Lengths of all pairs: p_len[i], t_len[i] for i < tsu_dataset_size(ds). */
+int tsu_dataset_lengths(const tsu_dataset* ds, int32_t* p_len, int32_t* t_len);
 /* Header fields as read (or 0 for generated data). */
 int tsu_dataset_header(const tsu_dataset* ds, int32_t* num, int32_t* len, int32_t* err);
 void tsu_dataset_free(tsu_dataset* ds);

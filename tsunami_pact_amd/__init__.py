@@ -40,7 +40,7 @@ EXPORTS = (
     "tsu_create", "tsu_destroy", "tsu_last_error", "tsu_strerror", "tsu_device_info",
     "tsu_packed_words", "tsu_pack",
     "tsu_dataset_read", "tsu_dataset_write", "tsu_dataset_generate", "tsu_dataset_from_strings",
-    "tsu_dataset_size", "tsu_dataset_pair", "tsu_dataset_header", "tsu_dataset_free",
+    "tsu_dataset_size", "tsu_dataset_pair", "tsu_dataset_lengths", "tsu_dataset_header", "tsu_dataset_free",
     "tsu_batch_from_dataset", "tsu_batch_create", "tsu_batch_size", "tsu_batch_bytes", "tsu_batch_free",
     "tsu_align_plain", "tsu_align_fixed", "tsu_align_batched", "tsu_autotune", "tsu_tune_table",
     "tsu_align_cigar", "tsu_cigars_get", "tsu_cigars_string", "tsu_cigars_free",
@@ -119,6 +119,7 @@ def library() -> ctypes.CDLL:
         "tsu_dataset_from_strings": (ctypes.c_int, [P(ctypes.c_char_p), P(i32), P(ctypes.c_char_p), P(i32), i64, P(vp)]),
         "tsu_dataset_size": (i64, [vp]),
         "tsu_dataset_pair": (ctypes.c_int, [vp, i64, P(ctypes.c_void_p), P(i32), P(ctypes.c_void_p), P(i32)]),
+        "tsu_dataset_lengths": (ctypes.c_int, [vp, P(i32), P(i32)]),
         "tsu_dataset_header": (ctypes.c_int, [vp, P(i32), P(i32), P(i32)]),
         "tsu_dataset_free": (None, [vp]),
         "tsu_batch_from_dataset": (ctypes.c_int, [vp, vp, i64, i64, P(vp)]),
@@ -233,13 +234,12 @@ class Dataset:
 
     def lengths(self) -> Tuple[np.ndarray, np.ndarray]:
         n = len(self)
-        pl = np.empty(n, np.int32)
-        tl = np.empty(n, np.int32)
-        a, b = ctypes.c_int32(), ctypes.c_int32()
-        for i in range(n):
-            self._lib.tsu_dataset_pair(self._h, i, None, ctypes.byref(a), None, ctypes.byref(b))
-            pl[i], tl[i] = a.value, b.value
-        return pl, tl
+        pl = np.empty(max(n, 1), np.int32)
+        tl = np.empty(max(n, 1), np.int32)
+        r = self._lib.tsu_dataset_lengths(self._h, _i32p(pl), _i32p(tl))
+        if r != TSU_OK:
+            raise TsuError(r, "tsu_dataset_lengths")
+        return pl[:n], tl[:n]
 
     def close(self) -> None:
         if self._h:

@@ -19,45 +19,18 @@
 #include <string>
 #include <vector>
 
-#include "tsu_internal.h"
+#include "tsu_host.h"
 
-using tsu::KArgs;
-using tsu::KPair;
-using tsu::Shape;
+using tsu_host::fail;
 
-struct tsu_ctx {
-    int device = 0;
-    int32_t x = 0, o = 0, e = 0;        // penalties as given
-    int32_t xs = 0, os = 0, es = 0, g = 1;  // divided by gcd
-    int32_t rm = 0, ri = 0;
-    hipStream_t stream = nullptr;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    int cu_count = 0;
-    int64_t max_lds = 0;
-    char name[256] = {0};
-    std::string err;
-    std::vector<tsu_tune_entry> tune;   // per-bucket shapes (autotuner)
-    // scratch reused across calls
-    int32_t* d_queue = nullptr;
-};
+namespace tsu_host {
 
-struct tsu_batch {
-    tsu_ctx* ctx = nullptr;
-    uint32_t* d_words = nullptr;
-    KPair* d_pairs = nullptr;
-    int32_t* d_scores = nullptr;
-    int32_t n = 0;
-    int64_t n_words = 0;
-    std::vector<tsu_pair> pairs;   // host copy of the descriptors
-    int32_t max_len = 0;
-    double cells = 0.0;
-};
+int fail(tsu_ctx* ctx, int status, const std::string& msg) {
+    if (ctx) ctx->err = msg;
+    return status;
+}
 
-struct tsu_cigars {
-    std::vector<uint32_t> runs;
-    std::vector<int64_t> off;
-    std::vector<int32_t> n;
-};
+}  // namespace tsu_host
 
 namespace {
 
@@ -68,11 +41,6 @@ int64_t gcd64(int64_t a, int64_t b) {
 
 double now_ms() {
     return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
-}
-
-int fail(tsu_ctx* ctx, int status, const std::string& msg) {
-    if (ctx) ctx->err = msg;
-    return status;
 }
 
 int hip_fail(tsu_ctx* ctx, hipError_t e, const char* what) {
@@ -108,11 +76,12 @@ int32_t capacity_for_score(const tsu_ctx* ctx, int64_t s_scaled) {
     return (int32_t)std::min<int64_t>(w, 1 << 26);
 }
 
+}  // namespace
+
 // Diagonal capacity no pair of lengths (p, t) can exceed: the range lies in
 // [-p-1, t+1].
-int32_t capacity_full(int32_t p, int32_t t) { return p + t + 3; }
-
-}  // namespace
+int32_t tsu_host::capacity_full(int32_t p, int32_t t) { return p + t + 3; }
+using tsu_host::capacity_full;
 
 // ---------------------------------------------------------------------------
 // context
@@ -257,9 +226,7 @@ extern "C" void tsu_batch_free(tsu_batch* b) {
 // ---------------------------------------------------------------------------
 // shapes and launches
 // ---------------------------------------------------------------------------
-namespace {
-
-KArgs base_args(const tsu_ctx* ctx) {
+KArgs tsu_host::base_args(const tsu_ctx* ctx) {
     KArgs a;
     std::memset(&a, 0, sizeof(a));
     a.x = ctx->xs; a.o = ctx->os; a.e = ctx->es; a.g = ctx->g;
@@ -267,7 +234,12 @@ KArgs base_args(const tsu_ctx* ctx) {
     return a;
 }
 
-int32_t seq_words_for(int32_t max_len) { return (int32_t)((max_len + 15) / 16 + 2); }
+int32_t tsu_host::seq_words_for(int32_t max_len) { return (int32_t)((max_len + 15) / 16 + 2); }
+
+using tsu_host::base_args;
+using tsu_host::seq_words_for;
+
+namespace {
 
 // Waves per pair for a diagonal capacity: enough chunks of 64 diagonals per
 // wave to amortise the per-step bookkeeping, at most 16 waves.
@@ -279,6 +251,14 @@ int default_waves(int32_t W) {
 }
 
 }  // namespace
+
+int64_t tsu_host::resident_grid(const tsu_ctx* ctx, const Shape& sh, const KArgs& a, int* status) {
+    int per_cu = 0;
+    const int r = tsu::occupancy(sh, tsu::lds_bytes(sh, a), &per_cu);
+    *status = r;
+    if (r != 0 || per_cu <= 0) return 0;
+    return (int64_t)per_cu * ctx->cu_count;
+}
 
 // Runs one launch of `shape` over the queue entries `order` (or pairs [0,n)
 // when order is null and the launch is plain).  Scores land in b->d_scores.
@@ -417,10 +397,20 @@ struct Bucket {
 // Bucket bounds on max(p, t).
 const int32_t kBucketBounds[] = {128, 256, 512, 1024, 2048, 4096, 8192, 16384, 32000, 1 << 26};
 
+}  // namespace
+
+int tsu_host::bucket_count() { return (int)(sizeof(kBucketBounds) / sizeof(kBucketBounds[0])); }
+int32_t tsu_host::bucket_bound(int i) { return kBucketBounds[i]; }
+int tsu_host::bucket_of(int32_t len) {
+    int bi = 0;
+    while (bi < bucket_count() - 1 && len > kBucketBounds[bi]) ++bi;
+    return bi;
+}
+
 // Initial capacity for a bucket: enough for about 15% divergence at the
-// bucket's length (score ~ 6.7 * 0.15 * L for 4/6/2 mixed edits), clamped to
-// what the lengths allow.  Pairs that overflow are re-run wider.
-int32_t initial_capacity(const tsu_ctx* ctx, int32_t bound) {
+// bucket's length (score ~ 0.15 * L * max(x, o+e)), clamped to what the
+// lengths allow.  Pairs that overflow are re-run wider.
+int32_t tsu_host::initial_capacity(const tsu_ctx* ctx, int32_t bound) {
     const double per_base = 0.15 * (double)std::max(ctx->xs, ctx->os + ctx->es);
     const int64_t s_est = (int64_t)std::ceil(per_base * bound) + ctx->os + ctx->es;
     int32_t W = capacity_for_score(ctx, s_est);
@@ -428,7 +418,7 @@ int32_t initial_capacity(const tsu_ctx* ctx, int32_t bound) {
     return std::max<int32_t>(W, 16);
 }
 
-Shape batched_shape(const tsu_ctx* ctx, int32_t W, int32_t max_len, int nw_hint) {
+Shape tsu_host::batched_shape(const tsu_ctx* ctx, int32_t W, int32_t max_len, int nw_hint) {
     Shape sh{nw_hint > 0 ? nw_hint : default_waves(W), max_len <= 32000, true, false, true};
     KArgs a = base_args(ctx);
     a.W = W;
@@ -440,7 +430,14 @@ Shape batched_shape(const tsu_ctx* ctx, int32_t W, int32_t max_len, int nw_hint)
     return sh;
 }
 
-}  // namespace
+int tsu_host::tuned_waves(const tsu_ctx* ctx, int32_t bound) {
+    for (const tsu_tune_entry& te : ctx->tune)
+        if (te.bucket_bound == bound) return te.waves_per_pair;
+    return 0;
+}
+
+using tsu_host::batched_shape;
+using tsu_host::initial_capacity;
 
 static const tsu_tune_entry* tune_lookup(const tsu_ctx* ctx, int32_t bound) {
     for (const tsu_tune_entry& te : ctx->tune)

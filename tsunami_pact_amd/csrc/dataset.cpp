@@ -351,6 +351,13 @@ extern "C" int tsu_dataset_pair(const tsu_dataset* ds, int64_t i, const char** p
     return TSU_OK;
 }
 
+extern "C" int tsu_dataset_lengths(const tsu_dataset* ds, int32_t* p_len, int32_t* t_len) {
+    if (!ds || !p_len || !t_len) return TSU_EINVAL;
+    std::memcpy(p_len, ds->p_len.data(), ds->p_len.size() * sizeof(int32_t));
+    std::memcpy(t_len, ds->t_len.data(), ds->t_len.size() * sizeof(int32_t));
+    return TSU_OK;
+}
+
 extern "C" int tsu_dataset_header(const tsu_dataset* ds, int32_t* num, int32_t* len, int32_t* err) {
     if (!ds) return TSU_EINVAL;
     if (num) *num = ds->hdr_num;

@@ -1,8 +1,549 @@
-// serve.cpp -- continuous-batching server (placeholder until the server lands).
-#include "tsu_internal.h"
+// serve.cpp -- continuous-batching alignment server on one GPU.
+//
+// Requests (1..N pairs each) are packed on the submitting thread and queued.
+// One worker thread owns the device side:
+//   - device memory is one pre-allocated region of hbm_budget_bytes, carved
+//     by a first-fit allocator; a request is admitted only when its bytes
+//     (packed sequences, descriptors, scores, order, ring spill) fit;
+//   - admitted requests are merged (FIFO, up to max_batch_pairs pairs) into a
+//     flight: one stream, pairs grouped by length bucket, longest first, one
+//     persistent queue-fed launch per bucket, scores copied back async;
+//   - up to max_inflight flights run at once on separate streams, so requests
+//     that arrive while others are on the GPU are launched as soon as a slot
+//     and memory free up and their work overlaps on the device;
+//   - a finished flight releases its memory; pairs whose diagonal range
+//     overflowed are re-launched wider on the same stream first.
+// Each pair's result depends only on that pair, so a request's scores are the
+// same alone and in any mix.
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <mutex>
+#include <new>
+#include <thread>
+#include <unordered_map>
+#include <vector>
 
-extern "C" int tsu_serve_open(tsu_ctx*, const tsu_serve_cfg*, tsu_server** out) { if (out) *out = nullptr; return TSU_EINTERNAL; }
-extern "C" int tsu_serve_submit(tsu_server*, const char* const*, const int32_t*, const char* const*, const int32_t*, int32_t, uint64_t*) { return TSU_EINTERNAL; }
-extern "C" int tsu_serve_poll(tsu_server*, uint64_t, int32_t*, tsu_req_stats*) { return TSU_EINTERNAL; }
-extern "C" int tsu_serve_wait(tsu_server*, uint64_t, int32_t*, tsu_req_stats*) { return TSU_EINTERNAL; }
-extern "C" int tsu_serve_close(tsu_server*) { return TSU_EINTERNAL; }
+#include "tsu_host.h"
+
+namespace {
+
+int64_t mono_ns() {
+    return std::chrono::duration_cast<std::chrono::nanoseconds>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+int64_t align256(int64_t b) { return (b + 255) / 256 * 256; }
+
+// First-fit allocator over one device region.
+struct Pool {
+    uint8_t* base = nullptr;
+    int64_t size = 0;
+    int64_t used = 0;
+    std::map<int64_t, int64_t> free_;  // offset -> bytes
+    void init(uint8_t* b, int64_t s) { base = b; size = s; free_.clear(); free_[0] = s; used = 0; }
+    int64_t alloc(int64_t bytes) {
+        bytes = align256(std::max<int64_t>(bytes, 256));
+        for (auto it = free_.begin(); it != free_.end(); ++it) {
+            if (it->second >= bytes) {
+                const int64_t off = it->first;
+                const int64_t rest = it->second - bytes;
+                free_.erase(it);
+                if (rest > 0) free_[off + bytes] = rest;
+                used += bytes;
+                return off;
+            }
+        }
+        return -1;
+    }
+    void release(int64_t off, int64_t bytes) {
+        bytes = align256(std::max<int64_t>(bytes, 256));
+        used -= bytes;
+        auto it = free_.emplace(off, bytes).first;
+        auto nx = std::next(it);
+        if (nx != free_.end() && it->first + it->second == nx->first) { it->second += nx->second; free_.erase(nx); }
+        if (it != free_.begin()) {
+            auto pv = std::prev(it);
+            if (pv->first + pv->second == it->first) { pv->second += it->second; free_.erase(it); }
+        }
+    }
+};
+
+struct Req {
+    uint64_t ticket = 0;
+    std::vector<uint32_t> words;
+    std::vector<tsu_pair> pairs;
+    std::vector<int32_t> scores;
+    int32_t max_len = 0;
+    int64_t submit_ns = 0, admit_ns = 0, done_ns = 0;
+    int status = TSU_OK;
+    bool done = false;
+};
+
+struct LaunchPlan {
+    int32_t first = 0, count = 0;  // range of the flight's order array
+    int32_t W = 0;
+    int32_t max_len = 0;
+    Shape sh{};
+    int64_t grid = 0;
+};
+
+struct Slot {
+    hipStream_t stream = nullptr;
+    hipEvent_t ev = nullptr;
+    bool active = false;
+    std::vector<Req*> reqs;
+    std::vector<int32_t> req_first;  // first flight pair of each request
+    int32_t n = 0;
+    int64_t off = -1, bytes = 0;
+    uint32_t* d_words = nullptr;
+    KPair* d_pairs = nullptr;
+    int32_t* d_scores = nullptr;
+    int32_t* d_order = nullptr;
+    int32_t* d_queue = nullptr;  // one counter per launch
+    uint8_t* d_ring = nullptr;
+    int64_t ring_bytes = 0;
+    std::vector<LaunchPlan> plan;
+    std::vector<int32_t> order;       // host copy of the launch order (pair indices)
+    std::vector<tsu_pair> fpairs;     // flight descriptors (host copy)
+    // pinned staging
+    uint32_t* h_words = nullptr; int64_t h_words_cap = 0;
+    tsu_pair* h_pairs = nullptr; int64_t h_pairs_cap = 0;
+    int32_t* h_order = nullptr; int64_t h_order_cap = 0;
+    int32_t* h_scores = nullptr; int64_t h_scores_cap = 0;
+};
+
+template <typename T>
+bool grow_pinned(T*& p, int64_t& cap, int64_t need) {
+    if (need <= cap) return true;
+    if (p) (void)hipHostFree(p);
+    p = nullptr;
+    cap = 0;
+    const int64_t n = std::max<int64_t>(need, 1024) * 3 / 2;
+    if (hipHostMalloc(reinterpret_cast<void**>(&p), (size_t)n * sizeof(T), hipHostMallocDefault) != hipSuccess) return false;
+    cap = n;
+    return true;
+}
+
+}  // namespace
+
+struct tsu_server {
+    tsu_ctx* ctx = nullptr;
+    tsu_serve_cfg cfg{};
+    std::mutex mu;
+    std::condition_variable cv;
+    std::deque<Req*> pending;
+    std::unordered_map<uint64_t, Req*> reqs;
+    std::vector<Slot> slots;
+    Pool pool;
+    uint8_t* d_base = nullptr;
+    uint64_t next_ticket = 1;
+    bool stop = false;
+    int fatal = TSU_OK;
+    std::thread worker;
+};
+
+namespace {
+
+// Plans the launches of a flight and returns its device bytes.
+int64_t plan_flight(tsu_server* s, Slot& sl, int* status) {
+    tsu_ctx* ctx = s->ctx;
+    const int nb = tsu_host::bucket_count();
+    std::vector<std::vector<int32_t>> by((size_t)nb);
+    for (int32_t i = 0; i < sl.n; ++i) {
+        const tsu_pair& pr = sl.fpairs[(size_t)i];
+        by[(size_t)tsu_host::bucket_of(std::max(pr.p_len, pr.t_len))].push_back(i);
+    }
+    sl.order.clear();
+    sl.plan.clear();
+    int64_t ring = 0;
+    for (int bi = nb - 1; bi >= 0; --bi) {
+        std::vector<int32_t>& v = by[(size_t)bi];
+        if (v.empty()) continue;
+        std::stable_sort(v.begin(), v.end(), [&](int32_t a, int32_t b) {
+            const tsu_pair& x = sl.fpairs[(size_t)a];
+            const tsu_pair& y = sl.fpairs[(size_t)b];
+            return (int64_t)x.p_len * x.t_len > (int64_t)y.p_len * y.t_len;
+        });
+        LaunchPlan lp;
+        lp.first = (int32_t)sl.order.size();
+        lp.count = (int32_t)v.size();
+        for (int32_t i : v) lp.max_len = std::max(lp.max_len, std::max(sl.fpairs[(size_t)i].p_len, sl.fpairs[(size_t)i].t_len));
+        const int32_t bound = tsu_host::bucket_bound(bi);
+        lp.W = std::min(tsu_host::initial_capacity(ctx, std::min(bound, lp.max_len)),
+                        tsu_host::capacity_full(lp.max_len, lp.max_len));
+        lp.sh = tsu_host::batched_shape(ctx, lp.W, lp.max_len, tsu_host::tuned_waves(ctx, bound));
+        KArgs a = tsu_host::base_args(ctx);
+        a.W = lp.W;
+        a.seq_words = tsu_host::seq_words_for(lp.max_len);
+        lp.grid = std::min<int64_t>(tsu_host::resident_grid(ctx, lp.sh, a, status), lp.count);
+        if (*status != 0 || lp.grid <= 0) { *status = TSU_EINTERNAL; return -1; }
+        if (!lp.sh.lds_ring) ring = std::max(ring, tsu::ring_bytes(lp.sh, lp.W, ctx->rm, ctx->ri) * lp.grid);
+        sl.order.insert(sl.order.end(), v.begin(), v.end());
+        sl.plan.push_back(lp);
+    }
+    sl.ring_bytes = ring;
+    int64_t words = 0;
+    for (const tsu_pair& pr : sl.fpairs) words = std::max<int64_t>(words, std::max(pr.p_off + tsu_packed_words(pr.p_len), pr.t_off + tsu_packed_words(pr.t_len)));
+    *status = 0;
+    return align256(words * 4) + align256((int64_t)sl.n * (int64_t)sizeof(KPair)) + 2 * align256((int64_t)sl.n * 4) +
+           align256((int64_t)sl.plan.size() * 4 + 4) + align256(ring);
+}
+
+int launch_plan(tsu_server* s, Slot& sl) {
+    tsu_ctx* ctx = s->ctx;
+    for (size_t li = 0; li < sl.plan.size(); ++li) {
+        const LaunchPlan& lp = sl.plan[li];
+        KArgs a = tsu_host::base_args(ctx);
+        a.words = sl.d_words;
+        a.pairs = sl.d_pairs;
+        a.order = sl.d_order + lp.first;
+        a.n = lp.count;
+        a.scores = sl.d_scores;
+        a.W = lp.W;
+        a.seq_words = tsu_host::seq_words_for(lp.max_len);
+        a.queue = sl.d_queue + li;
+        if (!lp.sh.lds_ring) {
+            a.ring_g = sl.d_ring;
+            a.ring_stride = tsu::ring_bytes(lp.sh, lp.W, ctx->rm, ctx->ri);
+        }
+        if (hipMemsetAsync(sl.d_queue + li, 0, sizeof(int32_t), sl.stream) != hipSuccess) return TSU_EHIP;
+        if (tsu::launch_wfa(lp.sh, a, lp.grid, sl.stream) != 0) return TSU_EHIP;
+    }
+    return TSU_OK;
+}
+
+// Builds and launches a flight in slot sl from the head of the queue.
+// Returns true when something was admitted.
+bool admit(tsu_server* s, Slot& sl) {
+    sl.reqs.clear();
+    sl.req_first.clear();
+    sl.fpairs.clear();
+    int64_t words = 0;
+    int32_t n = 0;
+    int64_t bytes = 0;
+    // FIFO: take requests while they fit the pair cap and the free memory
+    while (!s->pending.empty()) {
+        Req* r = s->pending.front();
+        const int32_t rn = (int32_t)r->pairs.size();
+        if (n > 0 && n + rn > s->cfg.max_batch_pairs) break;
+        // tentative flight with r added
+        const size_t keep_pairs = sl.fpairs.size();
+        for (const tsu_pair& pr : r->pairs) {
+            tsu_pair q = pr;
+            q.p_off += words;
+            q.t_off += words;
+            sl.fpairs.push_back(q);
+        }
+        sl.n = n + rn;
+        int st = 0;
+        const int64_t need = plan_flight(s, sl, &st);
+        if (st != 0 || need < 0 || need > s->pool.size) {
+            sl.fpairs.resize(keep_pairs);
+            if (n == 0) {  // cannot ever run: fail it
+                s->pending.pop_front();
+                r->status = st != 0 ? TSU_EINTERNAL : TSU_EBUDGET;
+                r->done = true;
+                r->done_ns = mono_ns();
+                s->cv.notify_all();
+                continue;
+            }
+            break;
+        }
+        // does it fit the memory free right now?
+        const int64_t off = s->pool.alloc(need);
+        if (off < 0) {
+            sl.fpairs.resize(keep_pairs);
+            break;
+        }
+        s->pool.release(off, need);
+        s->pending.pop_front();
+        sl.reqs.push_back(r);
+        sl.req_first.push_back(n);
+        words += (int64_t)r->words.size();
+        n += rn;
+        bytes = need;
+    }
+    if (sl.reqs.empty()) return false;
+    sl.n = n;
+    int st = 0;
+    bytes = plan_flight(s, sl, &st);
+    sl.off = s->pool.alloc(bytes);
+    if (st != 0 || sl.off < 0) {  // cannot happen: checked above; put the requests back
+        for (auto it = sl.reqs.rbegin(); it != sl.reqs.rend(); ++it) s->pending.push_front(*it);
+        sl.reqs.clear();
+        return false;
+    }
+    sl.bytes = bytes;
+    uint8_t* p = s->pool.base + sl.off;
+    auto carve = [&](int64_t b) { uint8_t* q = p; p += align256(b); return q; };
+    sl.d_words = reinterpret_cast<uint32_t*>(carve(words * 4));
+    sl.d_pairs = reinterpret_cast<KPair*>(carve((int64_t)n * (int64_t)sizeof(KPair)));
+    sl.d_scores = reinterpret_cast<int32_t*>(carve((int64_t)n * 4));
+    sl.d_order = reinterpret_cast<int32_t*>(carve((int64_t)n * 4));
+    sl.d_queue = reinterpret_cast<int32_t*>(carve((int64_t)sl.plan.size() * 4 + 4));
+    sl.d_ring = sl.ring_bytes ? carve(sl.ring_bytes) : nullptr;
+    // stage into pinned memory, copy, launch, copy back
+    if (!grow_pinned(sl.h_words, sl.h_words_cap, std::max<int64_t>(words, 1)) ||
+        !grow_pinned(sl.h_pairs, sl.h_pairs_cap, n) || !grow_pinned(sl.h_order, sl.h_order_cap, n) ||
+        !grow_pinned(sl.h_scores, sl.h_scores_cap, n)) {
+        s->fatal = TSU_ENOMEM;
+        return false;
+    }
+    int64_t w = 0;
+    for (Req* r : sl.reqs) {
+        std::memcpy(sl.h_words + w, r->words.data(), r->words.size() * 4);
+        w += (int64_t)r->words.size();
+    }
+    std::memcpy(sl.h_pairs, sl.fpairs.data(), (size_t)n * sizeof(tsu_pair));
+    std::memcpy(sl.h_order, sl.order.data(), (size_t)n * 4);
+    const int64_t now = mono_ns();
+    for (Req* r : sl.reqs) r->admit_ns = now;
+    bool ok = hipMemcpyAsync(sl.d_words, sl.h_words, (size_t)words * 4, hipMemcpyHostToDevice, sl.stream) == hipSuccess &&
+              hipMemcpyAsync(sl.d_pairs, sl.h_pairs, (size_t)n * sizeof(KPair), hipMemcpyHostToDevice, sl.stream) == hipSuccess &&
+              hipMemcpyAsync(sl.d_order, sl.h_order, (size_t)n * 4, hipMemcpyHostToDevice, sl.stream) == hipSuccess;
+    ok = ok && launch_plan(s, sl) == TSU_OK;
+    ok = ok && hipMemcpyAsync(sl.h_scores, sl.d_scores, (size_t)n * 4, hipMemcpyDeviceToHost, sl.stream) == hipSuccess &&
+         hipEventRecord(sl.ev, sl.stream) == hipSuccess;
+    if (!ok) { s->fatal = TSU_EHIP; return false; }
+    sl.active = true;
+    return true;
+}
+
+// A flight finished: hand out scores or re-launch overflowed pairs wider.
+void retire(tsu_server* s, Slot& sl) {
+    tsu_ctx* ctx = s->ctx;
+    std::vector<int32_t> again;
+    for (int32_t i = 0; i < sl.n; ++i)
+        if (sl.h_scores[i] == TSU_EKOVERFLOW) again.push_back(i);
+    if (!again.empty()) {
+        // one wider launch over the overflowed pairs, ring in LDS or in the
+        // flight's ring region; fall back to failing them if neither fits
+        int32_t max_len = 0;
+        int32_t W = 0;
+        for (int32_t i : again) max_len = std::max(max_len, std::max(sl.fpairs[(size_t)i].p_len, sl.fpairs[(size_t)i].t_len));
+        for (const LaunchPlan& lp : sl.plan) W = std::max(W, lp.W);
+        const bool can_widen = W < tsu_host::capacity_full(max_len, max_len);
+        W = std::min<int32_t>(2 * W, tsu_host::capacity_full(max_len, max_len));
+        LaunchPlan lp;
+        lp.first = 0;
+        lp.count = (int32_t)again.size();
+        lp.W = W;
+        lp.max_len = max_len;
+        lp.sh = tsu_host::batched_shape(ctx, W, max_len, 0);
+        KArgs a = tsu_host::base_args(ctx);
+        a.W = W;
+        a.seq_words = tsu_host::seq_words_for(max_len);
+        int st = 0;
+        lp.grid = std::min<int64_t>(tsu_host::resident_grid(ctx, lp.sh, a, &st), lp.count);
+        if (!lp.sh.lds_ring && st == 0)
+            lp.grid = std::min<int64_t>(lp.grid, sl.ring_bytes / std::max<int64_t>(1, tsu::ring_bytes(lp.sh, W, ctx->rm, ctx->ri)));
+        const bool widened = can_widen && st == 0 && lp.grid > 0;
+        if (widened) {
+            std::memcpy(sl.h_order, again.data(), again.size() * 4);
+            sl.plan.assign(1, lp);
+            bool ok = hipMemcpyAsync(sl.d_order, sl.h_order, again.size() * 4, hipMemcpyHostToDevice, sl.stream) == hipSuccess;
+            ok = ok && launch_plan(s, sl) == TSU_OK;
+            ok = ok && hipMemcpyAsync(sl.h_scores, sl.d_scores, (size_t)sl.n * 4, hipMemcpyDeviceToHost, sl.stream) == hipSuccess &&
+                 hipEventRecord(sl.ev, sl.stream) == hipSuccess;
+            if (ok) return;  // still in flight
+            s->fatal = TSU_EHIP;
+        }
+    }
+    const int64_t now = mono_ns();
+    for (size_t ri = 0; ri < sl.reqs.size(); ++ri) {
+        Req* r = sl.reqs[ri];
+        const int32_t f = sl.req_first[ri];
+        r->scores.assign(sl.h_scores + f, sl.h_scores + f + (int64_t)r->pairs.size());
+        r->status = TSU_OK;
+        for (int32_t v : r->scores)
+            if (v < 0) { r->status = v; break; }
+        r->done = true;
+        r->done_ns = now;
+    }
+    s->pool.release(sl.off, sl.bytes);
+    sl.off = -1;
+    sl.reqs.clear();
+    sl.active = false;
+    s->cv.notify_all();
+}
+
+void worker_loop(tsu_server* s) {
+    (void)hipSetDevice(s->ctx->device);
+    std::unique_lock<std::mutex> lk(s->mu);
+    for (;;) {
+        bool progressed = false;
+        bool any_active = false;
+        for (Slot& sl : s->slots) {
+            if (!sl.active) continue;
+            const hipError_t q = hipEventQuery(sl.ev);
+            if (q == hipSuccess) { retire(s, sl); progressed = true; }
+            else if (q != hipErrorNotReady) { s->fatal = TSU_EHIP; }
+            if (sl.active) any_active = true;
+        }
+        if (s->fatal == TSU_OK) {
+            for (Slot& sl : s->slots) {
+                if (sl.active || s->pending.empty()) continue;
+                if (admit(s, sl)) { progressed = true; any_active = true; }
+                else break;
+            }
+        }
+        if (s->fatal != TSU_OK) {
+            // fail everything still queued; in-flight work is waited for below
+            for (Req* r : s->pending) { r->status = s->fatal; r->done = true; r->done_ns = mono_ns(); }
+            s->pending.clear();
+            s->cv.notify_all();
+        }
+        if (s->stop && s->pending.empty() && !any_active) break;
+        if (!progressed) {
+            if (any_active) s->cv.wait_for(lk, std::chrono::microseconds(50));
+            else s->cv.wait(lk);
+        }
+    }
+}
+
+}  // namespace
+
+extern "C" int tsu_serve_open(tsu_ctx* ctx, const tsu_serve_cfg* cfg, tsu_server** out) {
+    if (!ctx || !cfg || !out) return TSU_EINVAL;
+    *out = nullptr;
+    if (cfg->hbm_budget_bytes < (1 << 20) || cfg->max_inflight < 1 || cfg->max_inflight > 64 || cfg->max_batch_pairs < 1)
+        return tsu_host::fail(ctx, TSU_EINVAL, "tsu_serve_open: budget >= 1 MiB, 1 <= max_inflight <= 64, max_batch_pairs >= 1");
+    if (hipSetDevice(ctx->device) != hipSuccess) return tsu_host::fail(ctx, TSU_EHIP, "hipSetDevice");
+    tsu_server* s = new (std::nothrow) tsu_server();
+    if (!s) return TSU_ENOMEM;
+    s->ctx = ctx;
+    s->cfg = *cfg;
+    if (hipMalloc(reinterpret_cast<void**>(&s->d_base), (size_t)cfg->hbm_budget_bytes) != hipSuccess) {
+        delete s;
+        return tsu_host::fail(ctx, TSU_ENOMEM, "tsu_serve_open: cannot reserve the HBM budget");
+    }
+    s->pool.init(s->d_base, cfg->hbm_budget_bytes);
+    s->slots.resize((size_t)cfg->max_inflight);
+    for (Slot& sl : s->slots) {
+        if (hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking) != hipSuccess ||
+            hipEventCreateWithFlags(&sl.ev, hipEventDisableTiming) != hipSuccess) {
+            tsu_serve_close(s);
+            return tsu_host::fail(ctx, TSU_EHIP, "tsu_serve_open: stream creation");
+        }
+    }
+    s->worker = std::thread(worker_loop, s);
+    *out = s;
+    return TSU_OK;
+}
+
+extern "C" int tsu_serve_submit(tsu_server* s, const char* const* patterns, const int32_t* p_len,
+                                const char* const* texts, const int32_t* t_len, int32_t n, uint64_t* ticket) {
+    if (!s || !ticket || n < 0 || (n > 0 && (!patterns || !p_len || !texts || !t_len))) return TSU_EINVAL;
+    Req* r = new (std::nothrow) Req();
+    if (!r) return TSU_ENOMEM;
+    r->submit_ns = mono_ns();
+    try {
+        r->pairs.resize((size_t)n);
+        int64_t w = 0;
+        for (int32_t i = 0; i < n; ++i) {
+            if (p_len[i] < 0 || t_len[i] < 0 || p_len[i] > (1 << 26) || t_len[i] > (1 << 26)) { delete r; return TSU_EINVAL; }
+            r->pairs[(size_t)i] = tsu_pair{w, w + tsu_packed_words(p_len[i]), p_len[i], t_len[i]};
+            w += tsu_packed_words(p_len[i]) + tsu_packed_words(t_len[i]);
+            r->max_len = std::max(r->max_len, std::max(p_len[i], t_len[i]));
+        }
+        r->words.assign((size_t)w, 0u);
+        for (int32_t i = 0; i < n; ++i) {
+            if (tsu_pack_seq(patterns[i], p_len[i], r->words.data() + r->pairs[(size_t)i].p_off) != TSU_OK ||
+                tsu_pack_seq(texts[i], t_len[i], r->words.data() + r->pairs[(size_t)i].t_off) != TSU_OK) {
+                delete r;
+                return TSU_EBADINPUT;
+            }
+        }
+    } catch (...) {
+        delete r;
+        return TSU_ENOMEM;
+    }
+    (void)hipSetDevice(s->ctx->device);
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (s->stop) { delete r; return TSU_EINVAL; }
+    if (s->fatal != TSU_OK) { delete r; return s->fatal; }
+    r->ticket = s->next_ticket++;
+    *ticket = r->ticket;
+    s->reqs[r->ticket] = r;
+    if (n == 0) {
+        r->done = true;
+        r->admit_ns = r->done_ns = mono_ns();
+    } else {
+        // a request that can never fit the budget is refused up front
+        Slot probe;
+        probe.fpairs = r->pairs;
+        probe.n = n;
+        int st = 0;
+        const int64_t need = plan_flight(s, probe, &st);
+        if (st != 0 || need > s->pool.size) {
+            s->reqs.erase(r->ticket);
+            delete r;
+            return st != 0 ? TSU_EINTERNAL : TSU_EBUDGET;
+        }
+        s->pending.push_back(r);
+    }
+    s->cv.notify_all();
+    return TSU_OK;
+}
+
+static int take_result(tsu_server* s, Req* r, int32_t* scores, tsu_req_stats* st) {
+    if (scores && !r->scores.empty()) std::memcpy(scores, r->scores.data(), r->scores.size() * 4);
+    if (st) {
+        st->submit_ns = r->submit_ns;
+        st->admit_ns = r->admit_ns;
+        st->done_ns = r->done_ns;
+        st->status = r->status;
+        st->n_pairs = (int32_t)r->pairs.size();
+    }
+    const int status = r->status;
+    s->reqs.erase(r->ticket);
+    delete r;
+    return status;
+}
+
+extern "C" int tsu_serve_poll(tsu_server* s, uint64_t ticket, int32_t* scores, tsu_req_stats* st) {
+    if (!s) return TSU_EINVAL;
+    std::lock_guard<std::mutex> lk(s->mu);
+    auto it = s->reqs.find(ticket);
+    if (it == s->reqs.end()) return TSU_EINVAL;
+    if (!it->second->done) return TSU_EAGAIN;
+    return take_result(s, it->second, scores, st);
+}
+
+extern "C" int tsu_serve_wait(tsu_server* s, uint64_t ticket, int32_t* scores, tsu_req_stats* st) {
+    if (!s) return TSU_EINVAL;
+    std::unique_lock<std::mutex> lk(s->mu);
+    auto it = s->reqs.find(ticket);
+    if (it == s->reqs.end()) return TSU_EINVAL;
+    Req* r = it->second;
+    s->cv.wait(lk, [&] { return r->done; });
+    return take_result(s, r, scores, st);
+}
+
+extern "C" int tsu_serve_close(tsu_server* s) {
+    if (!s) return TSU_EINVAL;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->stop = true;
+    }
+    s->cv.notify_all();
+    if (s->worker.joinable()) s->worker.join();
+    (void)hipSetDevice(s->ctx->device);
+    for (Slot& sl : s->slots) {
+        if (sl.stream) (void)hipStreamSynchronize(sl.stream);
+        if (sl.ev) (void)hipEventDestroy(sl.ev);
+        if (sl.stream) (void)hipStreamDestroy(sl.stream);
+        if (sl.h_words) (void)hipHostFree(sl.h_words);
+        if (sl.h_pairs) (void)hipHostFree(sl.h_pairs);
+        if (sl.h_order) (void)hipHostFree(sl.h_order);
+        if (sl.h_scores) (void)hipHostFree(sl.h_scores);
+    }
+    if (s->d_base) (void)hipFree(s->d_base);
+    const int status = s->fatal;
+    for (auto& kv : s->reqs) delete kv.second;
+    delete s;
+    return status;
+}

@@ -1,0 +1,71 @@
+// tsu_host.h -- host-only internals shared by api.cpp and serve.cpp.
+#pragma once
+
+#include <hip/hip_runtime_api.h>
+
+#include <string>
+#include <vector>
+
+#include "tsu_internal.h"
+
+using tsu::KArgs;
+using tsu::KPair;
+using tsu::Shape;
+
+struct tsu_ctx {
+    int device = 0;
+    int32_t x = 0, o = 0, e = 0;        // penalties as given
+    int32_t xs = 0, os = 0, es = 0, g = 1;  // divided by gcd
+    int32_t rm = 0, ri = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    int cu_count = 0;
+    int64_t max_lds = 0;
+    char name[256] = {0};
+    std::string err;
+    std::vector<tsu_tune_entry> tune;   // per-bucket shapes (autotuner)
+    // scratch reused across calls
+    int32_t* d_queue = nullptr;
+};
+
+struct tsu_batch {
+    tsu_ctx* ctx = nullptr;
+    uint32_t* d_words = nullptr;
+    KPair* d_pairs = nullptr;
+    int32_t* d_scores = nullptr;
+    int32_t n = 0;
+    int64_t n_words = 0;
+    std::vector<tsu_pair> pairs;   // host copy of the descriptors
+    int32_t max_len = 0;
+    double cells = 0.0;
+};
+
+struct tsu_cigars {
+    std::vector<uint32_t> runs;
+    std::vector<int64_t> off;
+    std::vector<int32_t> n;
+};
+
+namespace tsu_host {
+
+// Kernel arguments with the context's scaled penalties and ring depths.
+KArgs base_args(const tsu_ctx* ctx);
+// LDS words per staged sequence for sequences up to max_len bases.
+int32_t seq_words_for(int32_t max_len);
+// Length bucket bounds on max(p, t) and the bucket of a length.
+int bucket_count();
+int32_t bucket_bound(int i);
+int bucket_of(int32_t len);
+// First diagonal capacity tried for a bucket (widened on overflow).
+int32_t initial_capacity(const tsu_ctx* ctx, int32_t bound);
+// Capacity no pair of lengths (p, t) can exceed.
+int32_t capacity_full(int32_t p, int32_t t);
+// Persistent-launch shape for capacity W (ring in HBM when LDS is too small).
+Shape batched_shape(const tsu_ctx* ctx, int32_t W, int32_t max_len, int nw_hint);
+// Tuned waves per pair for a bucket (0 when the autotuner has not run).
+int tuned_waves(const tsu_ctx* ctx, int32_t bound);
+// Resident workgroups of a shape on the whole device.
+int64_t resident_grid(const tsu_ctx* ctx, const Shape& sh, const KArgs& a, int* status);
+int fail(tsu_ctx* ctx, int status, const std::string& msg);
+
+}  // namespace tsu_host
