@@ -82,15 +82,84 @@ def read_traffic(name):
         return json.load(f).get("bytes_per_launch")
 
 
+def run_served(a, tsu, args, cpu_sample):
+    """Open-loop Poisson arrivals at three rates (fractions of the measured
+    batched capacity); each request holds U{1..256} pairs of length
+    log-uniform in [100, 10000] bp and 1-15% mixed edits."""
+    import threading
+    rng = np.random.default_rng(SEED + 5)
+    n_req = args.requests
+    sizes = rng.integers(1, 257, n_req)
+    total = int(sizes.sum())
+    log(f"served: generating {n_req} requests, {total} pairs")
+    ds = tsu.Dataset.generate(total, seed=SEED + 5, len_min=100, len_max=10_000, err_min=0.01, err_max=0.15)
+    firsts = np.concatenate([[0], np.cumsum(sizes)[:-1]])
+    # capacity: batched form over all pairs of the stream
+    b = a.batch(ds)
+    ref, t = a.align_batched(b)
+    ref, t = a.align_batched(b)
+    cap_pairs = total / (t["wall_ms"] / 1e3)
+    b.close()
+    log(f"served: batched capacity {cap_pairs:.0f} pairs/s")
+    out = []
+    for frac in (0.3, 0.6, 0.9):
+        rate_req = frac * cap_pairs / float(sizes.mean())
+        gaps = rng.exponential(1.0 / rate_req, n_req)
+        srv = tsu.Server(a, hbm_budget_bytes=args.serve_budget << 20, max_inflight=args.inflight,
+                         max_batch_pairs=args.batch_pairs)
+        tickets = [None] * n_req
+        ready = [threading.Event() for _ in range(n_req)]
+
+        def submitter():
+            t0 = time.perf_counter()
+            due = 0.0
+            for i in range(n_req):
+                due += gaps[i]
+                d = t0 + due - time.perf_counter()
+                if d > 0:
+                    time.sleep(d)
+                tickets[i] = srv.submit_dataset(ds, int(firsts[i]), int(sizes[i]))
+                ready[i].set()
+
+        th = threading.Thread(target=submitter)
+        th.start()
+        lat, sub, done, exact = [], [], [], True
+        for i in range(n_req):
+            ready[i].wait()
+            st, sc, stats = srv.wait(tickets[i], int(sizes[i]))
+            if st != tsu.TSU_OK:
+                raise RuntimeError(f"request {i}: status {st}")
+            f = int(firsts[i])
+            exact &= bool((sc == ref[f:f + int(sizes[i])]).all())
+            lat.append((stats["done_ns"] - stats["submit_ns"]) / 1e6)
+            sub.append(stats["submit_ns"]); done.append(stats["done_ns"])
+        th.join()
+        srv.close()
+        span = (max(done) - min(sub)) / 1e9
+        res = {"offered_fraction": frac, "offered_requests_per_s": rate_req, "requests": n_req, "pairs": total,
+               "throughput_pairs_per_s": total / span, "p50_ms": float(np.percentile(lat, 50)),
+               "p99_ms": float(np.percentile(lat, 99)), "mean_ms": float(np.mean(lat)),
+               "same_as_batched": exact}
+        log(f"served {frac:.1f}: {res}")
+        out.append(res)
+    return {"capacity_pairs_per_s": cap_pairs, "rates": out}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--form", default="plain", choices=["plain", "batched", "cigar"])
+    ap.add_argument("--form", default="plain", choices=["plain", "batched", "cigar", "served"])
     ap.add_argument("--workload", default="cfg2", choices=sorted(WORKLOADS))
     ap.add_argument("--pairs", type=int, default=0, help="override the pair count (testing only)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--requests", type=int, default=300, help="served: requests per rate")
+    ap.add_argument("--serve-budget", type=int, default=16384, help="served: HBM budget in MiB")
+    ap.add_argument("--inflight", type=int, default=4, help="served: launches in flight")
+    ap.add_argument("--batch-pairs", type=int, default=2048, help="served: pairs merged per launch")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
+                    help="only run the streaming copy kernel over BYTES (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
@@ -99,6 +168,19 @@ def main():
 
     import tsunami_pact_amd as tsu
 
+    if args.calibrate_copy:
+        a = tsu.Aligner(*PEN)
+        ms = a.calibrate_copy(args.calibrate_copy)
+        print(json.dumps({"copy_bytes": args.calibrate_copy, "kernel_ms": ms,
+                          "GB_per_s": 2 * args.calibrate_copy / (ms / 1e3) / 1e9}), flush=True)
+        return
+    if args.form == "served":
+        a = tsu.Aligner(*PEN)
+        res = run_served(a, tsu, args, None)
+        print(json.dumps({"metric": METRIC, "form": "served", "workload": "cfg5: Poisson arrivals, requests of "
+                          "U{1..256} pairs, length log-uniform 100-10000 bp, 1-15% mixed edits, penalties 4/6/2",
+                          "device": a.device_info()["name"], **res}), flush=True)
+        return
     n, lmin, lmax, emin, emax, soff = WORKLOADS[args.workload]
     if args.pairs:
         n = args.pairs

@@ -397,3 +397,79 @@ class Aligner:
             self.close()
         except Exception:
             pass
+
+
+# ---------------------------------------------------------------------------
+# serving
+# ---------------------------------------------------------------------------
+class Server:
+    """Continuous-batching server on one Aligner's device (tsu_serve_*)."""
+
+    def __init__(self, aligner: "Aligner", hbm_budget_bytes: int = 8 << 30, max_inflight: int = 4,
+                 max_batch_pairs: int = 4096):
+        self._lib = library()
+        self._a = aligner
+        cfg = ServeCfg(hbm_budget_bytes, max_inflight, max_batch_pairs)
+        h = ctypes.c_void_p()
+        aligner._check(self._lib.tsu_serve_open(aligner._h, ctypes.byref(cfg), ctypes.byref(h)), "tsu_serve_open")
+        self._h = h
+
+    def submit(self, pairs: Sequence[Tuple[str, str]]) -> int:
+        n = len(pairs)
+        pats = [p.encode() for p, _ in pairs]
+        txts = [t.encode() for _, t in pairs]
+        parr = (ctypes.c_char_p * max(n, 1))(*pats)
+        tarr = (ctypes.c_char_p * max(n, 1))(*txts)
+        pl = np.array([len(p) for p in pats] or [0], dtype=np.int32)
+        tl = np.array([len(t) for t in txts] or [0], dtype=np.int32)
+        ticket = ctypes.c_uint64()
+        r = self._lib.tsu_serve_submit(self._h, parr, _i32p(pl), tarr, _i32p(tl), n, ctypes.byref(ticket))
+        if r != TSU_OK:
+            raise TsuError(r, f"tsu_serve_submit: {strerror(r)}")
+        return ticket.value
+
+    def submit_dataset(self, ds: "Dataset", first: int, count: int) -> int:
+        """Submits pairs [first, first+count) of ds without copying them into Python."""
+        lib = self._lib
+        parr = (ctypes.c_char_p * max(count, 1))()
+        tarr = (ctypes.c_char_p * max(count, 1))()
+        pl = np.zeros(max(count, 1), np.int32)
+        tl = np.zeros(max(count, 1), np.int32)
+        pp, tp = ctypes.c_void_p(), ctypes.c_void_p()
+        a, b = ctypes.c_int32(), ctypes.c_int32()
+        for j in range(count):
+            lib.tsu_dataset_pair(ds._h, first + j, ctypes.byref(pp), ctypes.byref(a), ctypes.byref(tp), ctypes.byref(b))
+            ctypes.cast(parr, ctypes.POINTER(ctypes.c_void_p))[j] = pp.value
+            ctypes.cast(tarr, ctypes.POINTER(ctypes.c_void_p))[j] = tp.value
+            pl[j], tl[j] = a.value, b.value
+        ticket = ctypes.c_uint64()
+        r = lib.tsu_serve_submit(self._h, parr, _i32p(pl), tarr, _i32p(tl), count, ctypes.byref(ticket))
+        if r != TSU_OK:
+            raise TsuError(r, f"tsu_serve_submit: {strerror(r)}")
+        return ticket.value
+
+    def _result(self, fn, ticket: int, n: int):
+        scores = np.zeros(max(n, 1), np.int32)
+        st = ReqStats()
+        r = fn(self._h, ticket, _i32p(scores), ctypes.byref(st))
+        return r, scores[:st.n_pairs if r != TSU_EAGAIN else 0], {f: getattr(st, f) for f, _ in ReqStats._fields_}
+
+    def poll(self, ticket: int, n: int):
+        """(status, scores, stats); status TSU_EAGAIN while not done."""
+        return self._result(self._lib.tsu_serve_poll, ticket, n)
+
+    def wait(self, ticket: int, n: int):
+        return self._result(self._lib.tsu_serve_wait, ticket, n)
+
+    def close(self) -> int:
+        r = TSU_OK
+        if self._h:
+            r = self._lib.tsu_serve_close(self._h)
+            self._h = None
+        return r
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

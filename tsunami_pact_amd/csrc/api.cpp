@@ -306,8 +306,11 @@ static int run_shape(tsu_ctx* ctx, tsu_batch* b, Shape sh, int32_t W, const int3
     return TSU_OK;
 }
 
+bool tsu_host::fixed_242(const tsu_ctx* ctx) { return ctx->xs == 2 && ctx->os == 3 && ctx->es == 1; }
+
 static Shape plain_shape(const tsu_ctx* ctx, int32_t W, int32_t max_len, bool* fits) {
     Shape sh{default_waves(W), max_len <= 32000, true, false, false};
+    sh.fixed_242 = tsu_host::fixed_242(ctx);
     KArgs a = base_args(ctx);
     a.W = W;
     a.seq_words = seq_words_for(max_len);
@@ -420,6 +423,7 @@ int32_t tsu_host::initial_capacity(const tsu_ctx* ctx, int32_t bound) {
 
 Shape tsu_host::batched_shape(const tsu_ctx* ctx, int32_t W, int32_t max_len, int nw_hint) {
     Shape sh{nw_hint > 0 ? nw_hint : default_waves(W), max_len <= 32000, true, false, true};
+    sh.fixed_242 = tsu_host::fixed_242(ctx);
     KArgs a = base_args(ctx);
     a.W = W;
     a.seq_words = seq_words_for(max_len);
@@ -630,6 +634,7 @@ extern "C" int tsu_align_cigar(tsu_ctx* ctx, tsu_batch* b, int32_t* scores, tsu_
             recs += std::min<int64_t>(W, 2 * (std::max<int64_t>(0, s - ctx->os) / ctx->es + 2) + 1);
         recs += 64;
         Shape sh{default_waves(W), true, true, true, true};
+        sh.fixed_242 = tsu_host::fixed_242(ctx);
         KArgs a = base_args(ctx);
         for (end = pos; end < order.size(); ++end) {
             const tsu_pair& pr = b->pairs[(size_t)order[end]];

@@ -67,5 +67,7 @@ int tuned_waves(const tsu_ctx* ctx, int32_t bound);
 // Resident workgroups of a shape on the whole device.
 int64_t resident_grid(const tsu_ctx* ctx, const Shape& sh, const KArgs& a, int* status);
 int fail(tsu_ctx* ctx, int status, const std::string& msg);
+// The context's penalties divide to (2, 3, 1), the case with a compile-time ring.
+bool fixed_242(const tsu_ctx* ctx);
 
 }  // namespace tsu_host

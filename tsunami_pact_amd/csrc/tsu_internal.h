@@ -65,6 +65,7 @@ struct Shape {
     bool lds_ring;   // ring in LDS (else global memory)
     bool bt;         // record backtrace + emit CIGAR
     bool queue;      // persistent launch fed by a work counter
+    bool fixed_242 = false;  // scaled penalties are (2, 3, 1): compile-time ring
 };
 
 // LDS bytes a launch of this shape needs.

@@ -1,3 +1,4 @@
+#pragma once
 // wfa_kernels.hip -- gap-affine WaveFront Alignment kernels for MI355X (gfx950).
 //
 // One workgroup of NW wave64s owns one pair at a time.  Per score step s it
@@ -42,6 +43,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <utility>
+
 #include "tsu_internal.h"
 
 namespace tsu {
@@ -75,37 +78,51 @@ __device__ __forceinline__ int lcp_packed(const uint32_t* __restrict__ P, const 
 
 __device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
-// Entries per ring slot: W diagonals + 2 NULL guards, padded to 16 bytes.
+// Entries per ring slot: W diagonals, 2 NULL guards and one write-only cell
+// (index W+2) that idle lanes store to, padded to 16 bytes.
 template <typename OffT>
 __host__ __device__ __forceinline__ int ring_stride(int W) {
     constexpr int per16 = 16 / (int)sizeof(OffT);
-    return (W + 2 + per16 - 1) / per16 * per16;
+    return (W + 3 + per16 - 1) / per16 * per16;
 }
 
 // LDS bytes before the ring: two sequences, slot limits, reduction area and
 // misc values, rounded up to 16 bytes.
 __host__ __device__ __forceinline__ int64_t lds_head_bytes(int seq_words, int nslot, int nw) {
-    const int64_t b = 4ll * (2 * seq_words + 2 * nslot + nw * 8 + kSmemInts);
+    (void)nw;
+    const int64_t b = 4ll * (2 * seq_words + 2 * (nslot + 1) + kSmemInts);
     return (b + 15) / 16 * 16;
 }
 
-template <int NW, typename OffT, bool LDS_RING, bool BT, bool QUEUE>
+constexpr int cmax(int a, int b) { return a > b ? a : b; }
+
+// Penalty specialisation: XS > 0 fixes the scaled penalties (x, o, e) =
+// (XS, OS, ES) at compile time; XS == 0 reads them from the arguments.
+template <int XS, int OS, int ES>
+struct Pen {
+    static constexpr bool fixed = XS > 0;
+    static constexpr int rm = fixed ? cmax(XS, OS + ES) + 1 : 0;  // M ring depth
+    static constexpr int ri = fixed ? ES + 1 : 0;                  // I/D ring depth
+};
+
+template <int NW, typename OffT, bool LDS_RING, bool BT, bool QUEUE, int XS, int OS, int ES>
 __global__ __launch_bounds__(NW * 64) void wfa_kernel(KArgs a) {
+    using PP = Pen<XS, OS, ES>;
     extern __shared__ __align__(16) uint32_t smem[];
     constexpr int NT = NW * 64;
     constexpr int32_t NUL = Null<OffT>::v;
     const int tid = threadIdx.x;
     const int lane = tid & 63;
-    const int wave = tid >> 6;
+    const int wave = uni(tid >> 6);
 
-    const int rm = a.rm, ri = a.ri;
-    const int nslot = rm + 2 * ri;
+    const int rm = PP::fixed ? PP::rm : a.rm;
+    const int ri = PP::fixed ? PP::ri : a.ri;
+    const int nslot = rm + 2 * ri;      // M slots, I slots, D slots
+    const int null_slot = nslot;        // one more slot that always holds NULL
     uint32_t* sP = smem;
     uint32_t* sT = smem + a.seq_words;
-    int32_t* sLo = reinterpret_cast<int32_t*>(smem + 2 * a.seq_words);
-    int32_t* sHi = sLo + nslot;
-    int32_t* sRed = sHi + nslot;           // NW * 8
-    int32_t* sMisc = sRed + NW * 8;        // kSmemInts
+    int2* sLim = reinterpret_cast<int2*>(smem + 2 * a.seq_words);  // (lo, hi) per slot (runtime penalties)
+    int32_t* sMisc = reinterpret_cast<int32_t*>(sLim + nslot + 1);  // kSmemInts
     const int stride = ring_stride<OffT>(a.W);
     OffT* ring;
     if constexpr (LDS_RING) {
@@ -122,7 +139,10 @@ __global__ __launch_bounds__(NW * 64) void wfa_kernel(KArgs a) {
         stepOff = a.step_off_g + (int64_t)blockIdx.x * a.step_cap;
     }
 
-    const int x = a.x, o = a.o, e = a.e;
+    const int x = PP::fixed ? XS : a.x;
+    const int o = PP::fixed ? OS : a.o;
+    const int e = PP::fixed ? ES : a.e;
+    const int oe = o + e;
     for (int iter = 0;; ++iter) {
         int idx;
         if constexpr (QUEUE) {
@@ -146,16 +166,26 @@ __global__ __launch_bounds__(NW * 64) void wfa_kernel(KArgs a) {
             sT[i] = i < tw ? a.words[pr.t_off + i] : 0u;
         }
         {
-            // ring slots are padded to a multiple of 16 bytes: fill with uint4
-            const int64_t nvec = (int64_t)nslot * stride * (int64_t)sizeof(OffT) / 16;
-            uint32_t nw32 = sizeof(OffT) == 2 ? ((uint32_t)(uint16_t)NUL | ((uint32_t)(uint16_t)NUL << 16))
-                                              : (uint32_t)NUL;
-            uint4 fill = make_uint4(nw32, nw32, nw32, nw32);
+            // slots are padded to a multiple of 16 bytes: fill with uint4
+            const int64_t nvec = (int64_t)(nslot + 1) * stride * (int64_t)sizeof(OffT) / 16;
+            const uint32_t nw32 = sizeof(OffT) == 2 ? ((uint32_t)(uint16_t)NUL | ((uint32_t)(uint16_t)NUL << 16))
+                                                    : (uint32_t)NUL;
+            const uint4 fill = make_uint4(nw32, nw32, nw32, nw32);
             uint4* r4 = reinterpret_cast<uint4*>(ring);
             for (int64_t i = tid; i < nvec; i += NT) r4[i] = fill;
         }
-        for (int i = tid; i < nslot; i += NT) { sLo[i] = 1; sHi[i] = 0; }
+        for (int i = tid; i <= nslot; i += NT) sLim[i] = make_int2(1, 0);
         __syncthreads();
+
+        // slot limits (lo, hi) in LDS, lo > hi = null; the slot of score s is
+        // s mod depth (compile-time depths when the penalties are fixed)
+        auto lim_get = [&](int slot) __attribute__((always_inline)) -> int2 {
+            const int2 v = sLim[slot];
+            return make_int2(uni(v.x), uni(v.y));
+        };
+        auto lim_set = [&](int slot, int2 v) __attribute__((always_inline)) {
+            if (tid == 0) sLim[slot] = v;
+        };
 
         // diagonal window [kmin, kmax] of the capacity; slot index = k - kmin + 1
         int kmin = max(-(a.W / 2), -p - 1);
@@ -174,101 +204,111 @@ __global__ __launch_bounds__(NW * 64) void wfa_kernel(KArgs a) {
         if (tid == 0) {
             const int ext = lcp_packed(sP, sT, 0, 0, min(p, t));
             ring[0 * stride + (0 - kmin + 1)] = (OffT)ext;
-            sLo[0] = 0; sHi[0] = 0;
             sMisc[1] = (p == t && ext == t) ? 1 : 0;
             if constexpr (BT) { rec[0] = 0u; stepLo[0] = 0; stepOff[0] = 0; }
         }
+        lim_set(0, make_int2(0, 0));
         if constexpr (BT) rec_next = 1;
         __syncthreads();
         if (uni(sMisc[1])) score = 0;
 
+        auto clear_range = [&](OffT* sl, int from, int to) __attribute__((always_inline)) {
+            for (int k = from + tid; k <= to; k += NT) sl[k - kmin + 1] = (OffT)NUL;
+        };
+        // NULL every cell of [alo, ahi] outside [klo, khi] (klo > khi: all)
+        auto clear_except = [&](OffT* sl, int alo, int ahi, int klo, int khi) __attribute__((always_inline)) {
+            if (alo > ahi) return;
+            if (klo > khi) { clear_range(sl, alo, ahi); return; }
+            if (alo < klo) clear_range(sl, alo, min(ahi, klo - 1));
+            if (ahi > khi) clear_range(sl, max(alo, khi + 1), ahi);
+        };
+        auto in_bounds = [&](int32_t v, int k) __attribute__((always_inline)) -> bool {
+            return (uint32_t)v <= (uint32_t)t && (uint32_t)(v - k) <= (uint32_t)p;
+        };
+
         int s = 0;
-        while (score < 0) {
+        int rcM = 0, rcI = 0;  // s mod rm, s mod ri
+        // One score step.  Returns false when the pair is finished.
+        auto step = [&](int) __attribute__((always_inline)) -> bool {
             ++s;
-            if (s > s_bound) { status = TSU_EINTERNAL; break; }
-            const int sm = s - x, so = s - o - e, se = s - e;
-            const int slM = s % rm, slI = rm + s % ri, slD = rm + ri + s % ri;
-            int lo_x = 1, hi_x = 0, lo_o = 1, hi_o = 0, lo_i = 1, hi_i = 0, lo_d = 1, hi_d = 0;
-            if (sm >= 0) { lo_x = uni(sLo[sm % rm]); hi_x = uni(sHi[sm % rm]); }
-            if (so >= 0) { lo_o = uni(sLo[so % rm]); hi_o = uni(sHi[so % rm]); }
-            if (se >= 0) {
-                lo_i = uni(sLo[rm + se % ri]); hi_i = uni(sHi[rm + se % ri]);
-                lo_d = uni(sLo[rm + ri + se % ri]); hi_d = uni(sHi[rm + ri + se % ri]);
-            }
-            const bool nx = lo_x > hi_x, no = lo_o > hi_o, ni = lo_i > hi_i, nd = lo_d > hi_d;
-            const int olo_M = uni(sLo[slM]), ohi_M = uni(sHi[slM]);
-            const int olo_I = uni(sLo[slI]), ohi_I = uni(sHi[slI]);
-            const int olo_D = uni(sLo[slD]), ohi_D = uni(sHi[slD]);
+            if (s > s_bound) { status = TSU_EINTERNAL; return false; }
+            rcM = (rcM + 1 == rm) ? 0 : rcM + 1;
+            rcI = (rcI + 1 == ri) ? 0 : rcI + 1;
+            // slots of scores s-x, s-o-e, s-e; a negative score maps to a slot
+            // of a future score, which still holds the null initial range
+            int jx = rcM - x; if (jx < 0) jx += rm;
+            int jo = rcM - oe; if (jo < 0) jo += rm;
+            int je = rcI - e; if (je < 0) je += ri;
+            const int je_i = rm + je, je_d = rm + ri + je;
+            const int slM = rcM, slI = rm + rcI, slD = rm + ri + rcI;
+            const int2 Lx = lim_get(jx), Lo = lim_get(jo), Li = lim_get(je_i), Ld = lim_get(je_d);
+            const int2 Om = lim_get(slM), Oi = lim_get(slI), Od = lim_get(slD);
+            auto push = [&](int2 nm, int2 ni, int2 nd) __attribute__((always_inline)) {
+                lim_set(slM, nm); lim_set(slI, ni); lim_set(slD, nd);
+            };
+            const bool nx = Lx.x > Lx.y, no = Lo.x > Lo.y, ni = Li.x > Li.y, nd = Ld.x > Ld.y;
             OffT* Mt = ring + slM * stride;
             OffT* It = ring + slI * stride;
             OffT* Dt = ring + slD * stride;
-            auto clear_range = [&](OffT* sl, int from, int to) {
-                for (int k = from + tid; k <= to; k += NT) sl[k - kmin + 1] = (OffT)NUL;
-            };
-            auto clear_except = [&](OffT* sl, int alo, int ahi, int klo, int khi) {
-                if (alo > ahi) return;
-                if (klo > khi) { clear_range(sl, alo, ahi); return; }
-                clear_range(sl, alo, min(ahi, klo - 1));
-                clear_range(sl, max(alo, khi + 1), ahi);
-            };
             if (nx && no && ni && nd) {
                 // null step (wfa_gpu.cu:417-422): all three outputs are null
-                clear_except(Mt, olo_M, ohi_M, 1, 0);
-                clear_except(It, olo_I, ohi_I, 1, 0);
-                clear_except(Dt, olo_D, ohi_D, 1, 0);
-                if (tid == 0) {
-                    sLo[slM] = 1; sHi[slM] = 0; sLo[slI] = 1; sHi[slI] = 0; sLo[slD] = 1; sHi[slD] = 0;
-                }
+                clear_except(Mt, Om.x, Om.y, 1, 0);
+                clear_except(It, Oi.x, Oi.y, 1, 0);
+                clear_except(Dt, Od.x, Od.y, 1, 0);
                 if constexpr (BT) {
-                    if (s >= a.step_cap) { status = TSU_ECIGAR; break; }
+                    if (s >= a.step_cap) { status = TSU_ECIGAR; return false; }
                     if (tid == 0) { stepLo[s] = 0; stepOff[s] = rec_next; }
                 }
                 __syncthreads();
-                continue;
+                push(make_int2(1, 0), make_int2(1, 0), make_int2(1, 0));
+                __syncthreads();
+                return true;
             }
-            int lo = 0, hi = 0;
-            {
-                bool have = false;
-                if (!nx) { lo = lo_x; hi = hi_x; have = true; }
-                if (!no) { lo = have ? min(lo, lo_o - 1) : lo_o - 1; hi = have ? max(hi, hi_o + 1) : hi_o + 1; have = true; }
-                if (!ni) { lo = have ? min(lo, lo_i + 1) : lo_i + 1; hi = have ? max(hi, hi_i + 1) : hi_i + 1; have = true; }
-                if (!nd) { lo = have ? min(lo, lo_d - 1) : lo_d - 1; hi = have ? max(hi, hi_d - 1) : hi_d - 1; }
-            }
-            if (lo < kmin || hi > kmax) { status = TSU_EKOVERFLOW; break; }
+            int lo = 0x3fffffff, hi = -0x3fffffff;
+            if (!nx) { lo = Lx.x; hi = Lx.y; }
+            if (!no) { lo = min(lo, Lo.x - 1); hi = max(hi, Lo.y + 1); }
+            if (!ni) { lo = min(lo, Li.x + 1); hi = max(hi, Li.y + 1); }
+            if (!nd) { lo = min(lo, Ld.x - 1); hi = max(hi, Ld.y - 1); }
+            if (lo < kmin || hi > kmax) { status = TSU_EKOVERFLOW; return false; }
             const int width = hi - lo + 1;
             if constexpr (BT) {
-                if (s >= a.step_cap || rec_next + width > a.rec_cap) { status = TSU_ECIGAR; break; }
+                if (s >= a.step_cap || rec_next + width > a.rec_cap) { status = TSU_ECIGAR; return false; }
             }
             const bool hasI = !no || !ni;
             const bool hasD = !no || !nd;
-            const OffT* Mx = ring + (nx ? 0 : (sm % rm)) * stride;
-            const OffT* Mo = ring + (no ? 0 : (so % rm)) * stride;
-            const OffT* Ie = ring + (ni ? rm : (rm + se % ri)) * stride;
-            const OffT* De = ring + (nd ? rm + ri : (rm + ri + se % ri)) * stride;
+            const OffT* Mx = ring + (nx ? null_slot : jx) * stride;
+            const OffT* Mo = ring + (no ? null_slot : jo) * stride;
+            const OffT* Ie = ring + (ni ? null_slot : je_i) * stride;
+            const OffT* De = ring + (nd ? null_slot : je_d) * stride;
             uint32_t* recs = nullptr;
             if constexpr (BT) recs = rec + rec_next;
 
-            int fM = 0x7fffffff, lM = -0x7fffffff, fI = 0x7fffffff, lI = -0x7fffffff, fD = 0x7fffffff, lD = -0x7fffffff;
-            int fin = 0;
-            for (int kb = lo + wave * 64; kb <= hi; kb += NW * 64) {
+            // ---- compute + extend, 64 diagonals per wave per chunk ----------
+            // The pass spans [clo, chi]: [lo, hi] plus whatever the target
+            // slots held before (usually inside [lo, hi]); cells outside
+            // [lo, hi] are written NULL, so no separate clear is needed.
+            int clo = lo, chi = hi;
+            if (Om.x <= Om.y) { clo = min(clo, Om.x); chi = max(chi, Om.y); }
+            if (Oi.x <= Oi.y) { clo = min(clo, Oi.x); chi = max(chi, Oi.y); }
+            if (Od.x <= Od.y) { clo = min(clo, Od.x); chi = max(chi, Od.y); }
+            for (int kb = clo + wave * 64; kb <= chi; kb += NT) {
                 const int k = kb + lane;
-                const bool act = k <= hi;
-                int32_t M = NUL, I = NUL, D = NUL;
-                bool inM = false, inI = false, inD = false, end = false;
-                if (act) {
-                    const int j = k - kmin + 1;
-                    const int32_t mol = no ? NUL : (int32_t)Mo[j - 1];
-                    const int32_t mor = no ? NUL : (int32_t)Mo[j + 1];
-                    const int32_t iel = ni ? NUL : (int32_t)Ie[j - 1];
-                    const int32_t der = nd ? NUL : (int32_t)De[j + 1];
-                    const int32_t mx = nx ? NUL : (int32_t)Mx[j];
-                    const int32_t ipre = max(mol, iel);
-                    I = ipre >= 0 ? min(ipre + 1, t + 1) : NUL;
-                    D = max(mor, der);
-                    const int32_t X = mx >= 0 ? mx + 1 : NUL;
-                    M = max(D, max(X, I));
-                    if ((uint32_t)M > (uint32_t)t || (uint32_t)(M - k) > (uint32_t)p) M = NUL;
-                    if constexpr (BT) {
+                const bool act = k <= chi;
+                const bool inr = k >= lo && k <= hi;
+                const int j = min(k, chi) - kmin + 1;  // idle lanes read a valid cell
+                const int32_t mol = Mo[j - 1];
+                const int32_t mor = Mo[j + 1];
+                const int32_t iel = Ie[j - 1];
+                const int32_t der = De[j + 1];
+                const int32_t mx = Mx[j];
+                const int32_t ipre = max(mol, iel);
+                const int32_t I = ipre >= 0 ? min(ipre + 1, t + 1) : NUL;
+                const int32_t D = max(mor, der);
+                const int32_t X = mx >= 0 ? mx + 1 : NUL;
+                int32_t M = max(D, max(X, I));
+                if ((uint32_t)M > (uint32_t)t || (uint32_t)(M - k) > (uint32_t)p || !inr) M = NUL;
+                if constexpr (BT) {
+                    if (inr) {
                         uint32_t r = 0;
                         if (M >= 0) {
                             const uint32_t mw = (X >= D && X >= I) ? 1u : (D >= I ? 2u : 3u);
@@ -278,60 +318,92 @@ __global__ __launch_bounds__(NW * 64) void wfa_kernel(KArgs a) {
                         if (der >= mor) r |= 8u;
                         recs[k - lo] = r;
                     }
-                    if (M >= 0) {
-                        const int v = M - k;
-                        M += lcp_packed(sP, sT, v, M, min(p - v, t - M));
-                        inM = true;
-                        end = (k == tp) && (M >= t);
+                }
+                // extend in uniform rounds of 32 bases; lanes that are done
+                // keep their count and read a harmless word
+                {
+                    const int v = M - k;
+                    const int lim = (act && M >= 0) ? min(p - v, t - M) : 0;
+                    bool go = lim > 0;
+                    int n = 0;
+                    while (__ballot(go)) {
+                        const int pa = go ? v + n : 0;
+                        const int pb = go ? M + n : 0;
+                        const int ja = pa >> 4, sa = (pa & 15) << 1;
+                        const int jb = pb >> 4, sb = (pb & 15) << 1;
+                        const uint32_t p0 = sP[ja], p1 = sP[ja + 1], p2 = sP[ja + 2];
+                        const uint32_t t0 = sT[jb], t1 = sT[jb + 1], t2 = sT[jb + 2];
+                        const uint32_t xl = __builtin_amdgcn_alignbit(p1, p0, sa) ^ __builtin_amdgcn_alignbit(t1, t0, sb);
+                        const uint32_t xh = __builtin_amdgcn_alignbit(p2, p1, sa) ^ __builtin_amdgcn_alignbit(t2, t1, sb);
+                        // first differing bit of the 64-bit window (64 when equal)
+                        const uint64_t xx = ((uint64_t)xh << 32) | xl;
+                        const int m = (xx ? (int)__builtin_ctzll(xx) : 64) >> 1;
+                        n = go ? n + m : n;
+                        go = go && m == 32 && n < lim;
                     }
-                    inI = (uint32_t)I <= (uint32_t)t && (uint32_t)(I - k) <= (uint32_t)p;
-                    inD = (uint32_t)D <= (uint32_t)t && (uint32_t)(D - k) <= (uint32_t)p;
-                    Mt[j] = (OffT)M;
-                    if (hasI) It[j] = (OffT)I;
-                    if (hasD) Dt[j] = (OffT)D;
+                    M += min(n, lim);
                 }
-                const uint64_t bM = __ballot(inM), bI = __ballot(inI), bD = __ballot(inD);
-                if (bM) { if (fM == 0x7fffffff) fM = kb + __builtin_ctzll(bM); lM = kb + 63 - __builtin_clzll(bM); }
-                if (bI) { if (fI == 0x7fffffff) fI = kb + __builtin_ctzll(bI); lI = kb + 63 - __builtin_clzll(bI); }
-                if (bD) { if (fD == 0x7fffffff) fD = kb + __builtin_ctzll(bD); lD = kb + 63 - __builtin_clzll(bD); }
-                if (__ballot(end)) fin = 1;
+                const int js = act ? j : a.W + 2;  // idle lanes: write-only cell
+                Mt[js] = (OffT)M;
+                It[js] = (OffT)(inr ? I : NUL);
+                Dt[js] = (OffT)(inr ? D : NUL);
             }
-            if constexpr (NW > 1) {
-                if (lane == 0) {
-                    int32_t* r = sRed + wave * 8;
-                    r[0] = fM; r[1] = lM; r[2] = fI; r[3] = lI; r[4] = fD; r[5] = lD; r[6] = fin;
+            __syncthreads();  // every cell of step s is written
+
+            // ---- trim ends (wfa_gpu.cu:633-719) ----------------------------
+            // common case: the six boundary cells are in bounds -> no trim
+            int tloM = lo, thiM = hi, tloI = hasI ? lo : 1, thiI = hasI ? hi : 0, tloD = hasD ? lo : 1, thiD = hasD ? hi : 0;
+            {
+                bool bad = false;
+                if (lane < 6) {
+                    const int comp = lane >> 1;
+                    const int kk = (lane & 1) ? hi : lo;
+                    const OffT* sl = comp == 0 ? Mt : (comp == 1 ? It : Dt);
+                    const int32_t v = sl[kk - kmin + 1];
+                    const bool use = comp == 0 || (comp == 1 ? hasI : hasD);
+                    bad = use && !in_bounds(v, kk);
                 }
-                __syncthreads();
-                fM = 0x7fffffff; lM = -0x7fffffff; fI = 0x7fffffff; lI = -0x7fffffff; fD = 0x7fffffff; lD = -0x7fffffff; fin = 0;
-                for (int w = 0; w < NW; ++w) {
-                    const int32_t* r = sRed + w * 8;
-                    fM = min(fM, uni(r[0])); lM = max(lM, uni(r[1]));
-                    fI = min(fI, uni(r[2])); lI = max(lI, uni(r[3]));
-                    fD = min(fD, uni(r[4])); lD = max(lD, uni(r[5]));
-                    fin |= uni(r[6]);
+                const uint64_t bb = __ballot(bad);
+                if (bb) {
+                    auto scan = [&](const OffT* sl, int& f, int& l) __attribute__((always_inline)) {
+                        f = 0x3fffffff; l = -0x3fffffff;
+                        for (int kb = lo; kb <= hi; kb += 64) {
+                            const int k = kb + lane;
+                            const bool in = k <= hi && in_bounds((int32_t)sl[min(k, hi) - kmin + 1], k);
+                            const uint64_t b = __ballot(in);
+                            if (b) {
+                                if (f == 0x3fffffff) f = kb + (int)__builtin_ctzll(b);
+                                l = kb + 63 - (int)__builtin_clzll(b);
+                            }
+                        }
+                        if (f > l) { f = 1; l = 0; }
+                    };
+                    if (bb & 3ull) scan(Mt, tloM, thiM);
+                    if (bb & 12ull) scan(It, tloI, thiI);
+                    if (bb & 48ull) scan(Dt, tloD, thiD);
                 }
             }
-            // trimmed ranges (empty -> lo=1, hi=0)
-            int tloM = 1, thiM = 0, tloI = 1, thiI = 0, tloD = 1, thiD = 0;
-            if (fM <= lM) { tloM = fM; thiM = lM; }
-            if (hasI && fI <= lI) { tloI = fI; thiI = lI; }
-            if (hasD && fD <= lD) { tloD = fD; thiD = lD; }
-            __syncthreads();  // every cell of step s is written before clearing
-            clear_except(Mt, olo_M, ohi_M, tloM, thiM);
-            if (hasI) clear_except(It, lo, hi, tloI, thiI);
-            clear_except(It, olo_I, ohi_I, tloI, thiI);
-            if (hasD) clear_except(Dt, lo, hi, tloD, thiD);
-            clear_except(Dt, olo_D, ohi_D, tloD, thiD);
-            if (tid == 0) {
-                sLo[slM] = tloM; sHi[slM] = thiM;
-                sLo[slI] = tloI; sHi[slI] = thiI;
-                sLo[slD] = tloD; sHi[slD] = thiD;
-                if constexpr (BT) { stepLo[s] = lo; stepOff[s] = rec_next; }
+            // end test: M[s][t-p] >= t (wfa_gpu.cu:288-331)
+            bool fin = false;
+            if (tp >= tloM && tp <= thiM) fin = uni((int32_t)Mt[tp - kmin + 1]) >= t;
+            // cells outside the kept ranges return to NULL (M cells trimmed
+            // away are NULL already)
+            // (the pass left NULL outside [lo, hi]; I/D cells trimmed away
+            // inside it still hold out-of-bounds values)
+            if (tloI != lo || thiI != hi) clear_except(It, lo, hi, tloI, thiI);
+            if (tloD != lo || thiD != hi) clear_except(Dt, lo, hi, tloD, thiD);
+            push(make_int2(tloM, thiM), make_int2(tloI, thiI), make_int2(tloD, thiD));
+            if constexpr (BT) {
+                if (tid == 0) { stepLo[s] = lo; stepOff[s] = rec_next; }
+                rec_next += width;
             }
-            if constexpr (BT) rec_next += width;
             __syncthreads();
-            if (fin) score = s;
-        }
+            if (fin) { score = s; return false; }
+            return true;
+        };
+
+        if (score < 0)
+            while (step(0)) {}
 
         // ---- outputs --------------------------------------------------------
         if constexpr (BT) {
@@ -350,18 +422,18 @@ __global__ __launch_bounds__(NW * 64) void wfa_kernel(KArgs a) {
                         int64_t nr = 0;
                         uint32_t cur_op = 4, cur_len = 0;
                         bool full = false;
-                        auto put = [&](uint32_t run) {
+                        auto put = [&](uint32_t run) __attribute__((always_inline)) {
                             if (nr >= need) { full = true; return; }
                             out[need - 1 - nr] = run;
                             ++nr;
                         };
-                        auto emit = [&](uint32_t op, int n) {
+                        auto emit = [&](uint32_t op, int n) __attribute__((always_inline)) {
                             if (n <= 0) return;
                             if (op == cur_op) { cur_len += (uint32_t)n; return; }
                             if (cur_len) put((cur_len << 2) | cur_op);
                             cur_op = op; cur_len = (uint32_t)n;
                         };
-                        auto record = [&](int ss, int kk, uint32_t& r) -> bool {
+                        auto record = [&](int ss, int kk, uint32_t& r) __attribute__((always_inline)) -> bool {
                             if (ss < 0 || ss > s) return false;
                             const int l0 = stepLo[ss];
                             const int64_t o0 = stepOff[ss];
@@ -430,105 +502,29 @@ __global__ __launch_bounds__(NW * 64) void wfa_kernel(KArgs a) {
     }
 }
 
-// Plain streaming copy (HBM byte-counter calibration).
-__global__ __launch_bounds__(256) void copy_kernel(const uint4* __restrict__ src, uint4* __restrict__ dst, int64_t n) {
-    for (int64_t i = blockIdx.x * 256ll + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) dst[i] = src[i];
-}
-
-// ---------------------------------------------------------------------------
-// host-side launch plumbing
-// ---------------------------------------------------------------------------
-int32_t ring_stride_entries(const Shape& sh, int32_t W) {
-    return sh.off16 ? ring_stride<int16_t>(W) : ring_stride<int32_t>(W);
-}
-
-int64_t ring_bytes(const Shape& sh, int32_t W, int32_t rm, int32_t ri) {
-    return (int64_t)(rm + 2 * ri) * ring_stride_entries(sh, W) * (sh.off16 ? 2 : 4);
-}
-
-int64_t lds_bytes(const Shape& sh, const KArgs& a) {
-    int64_t b = lds_head_bytes(a.seq_words, a.rm + 2 * a.ri, sh.nw);
-    if (sh.lds_ring) b += ring_bytes(sh, a.W, a.rm, a.ri);
-    return b;
-}
-
 typedef void (*KernelFn)(KArgs);
 
 template <int NW, typename OffT, bool LDS, bool BT, bool Q>
-static KernelFn fn() { return wfa_kernel<NW, OffT, LDS, BT, Q>; }
+inline KernelFn fn(bool fixed_242) {
+    // scaled penalties (2, 3, 1) are the reference's 4/6/2 divided by 2
+    return fixed_242 ? wfa_kernel<NW, OffT, LDS, BT, Q, 2, 3, 1> : wfa_kernel<NW, OffT, LDS, BT, Q, 0, 0, 0>;
+}
 
 template <int NW>
-static KernelFn pick_nw(const Shape& sh) {
+KernelFn pick_nw_impl(const Shape& sh) {
+    const bool f = sh.fixed_242;
     if (!sh.queue) {
         // plain launches: ring in LDS, no backtrace
         if (!sh.lds_ring || sh.bt) return nullptr;
-        return sh.off16 ? fn<NW, int16_t, true, false, false>() : fn<NW, int32_t, true, false, false>();
+        return sh.off16 ? fn<NW, int16_t, true, false, false>(f) : fn<NW, int32_t, true, false, false>(f);
     }
     if (sh.off16) {
-        if (sh.lds_ring) return sh.bt ? fn<NW, int16_t, true, true, true>() : fn<NW, int16_t, true, false, true>();
-        return sh.bt ? fn<NW, int16_t, false, true, true>() : fn<NW, int16_t, false, false, true>();
+        if (sh.lds_ring) return sh.bt ? fn<NW, int16_t, true, true, true>(f) : fn<NW, int16_t, true, false, true>(f);
+        return sh.bt ? fn<NW, int16_t, false, true, true>(f) : fn<NW, int16_t, false, false, true>(f);
     }
-    if (sh.lds_ring) return sh.bt ? fn<NW, int32_t, true, true, true>() : fn<NW, int32_t, true, false, true>();
-    return sh.bt ? fn<NW, int32_t, false, true, true>() : fn<NW, int32_t, false, false, true>();
+    if (sh.lds_ring) return sh.bt ? fn<NW, int32_t, true, true, true>(f) : fn<NW, int32_t, true, false, true>(f);
+    return sh.bt ? fn<NW, int32_t, false, true, true>(f) : fn<NW, int32_t, false, false, true>(f);
 }
 
-static KernelFn pick(const Shape& sh) {
-    switch (sh.nw) {
-    case 1: return pick_nw<1>(sh);
-    case 2: return pick_nw<2>(sh);
-    case 4: return pick_nw<4>(sh);
-    case 8: return pick_nw<8>(sh);
-    case 16: return pick_nw<16>(sh);
-    default: return nullptr;
-    }
-}
-
-int64_t max_lds_for(const Shape& sh) {
-    KernelFn f = pick(sh);
-    if (!f) return 0;
-    int dev = 0;
-    if (hipGetDevice(&dev) != hipSuccess) return 0;
-    int v = 0;
-    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess) return 0;
-    return v;
-}
-
-int occupancy(const Shape& sh, int64_t lds, int* blocks_per_cu) {
-    KernelFn f = pick(sh);
-    if (!f) return (int)hipErrorInvalidValue;
-    if (lds > 65536) {
-        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(f), hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-        if (e != hipSuccess) return (int)e;
-    }
-    int n = 0;
-    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, reinterpret_cast<const void*>(f), sh.nw * 64, (size_t)lds);
-    if (e != hipSuccess) return (int)e;
-    *blocks_per_cu = n;
-    return 0;
-}
-
-int launch_wfa(const Shape& sh, const KArgs& a, int64_t grid, void* stream) {
-    KernelFn f = pick(sh);
-    if (!f) return (int)hipErrorInvalidValue;
-    if (grid <= 0) return 0;
-    if (grid > 0x7fffffff) return (int)hipErrorInvalidValue;
-    const int64_t lds = lds_bytes(sh, a);
-    if (lds > 65536) {
-        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(f), hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-        if (e != hipSuccess) return (int)e;
-    }
-    hipLaunchKernelGGL(f, dim3((unsigned)grid), dim3(sh.nw * 64), (size_t)lds, (hipStream_t)stream, a);
-    return (int)hipGetLastError();
-}
-
-int launch_copy(const void* src, void* dst, int64_t bytes, void* stream) {
-    const int64_t n = bytes / 16;
-    if (n <= 0) return 0;
-    int64_t grid = (n + 255) / 256;
-    if (grid > 65536) grid = 65536;
-    hipLaunchKernelGGL(copy_kernel, dim3((unsigned)grid), dim3(256), 0, (hipStream_t)stream,
-                       reinterpret_cast<const uint4*>(src), reinterpret_cast<uint4*>(dst), n);
-    return (int)hipGetLastError();
-}
 
 }  // namespace tsu
