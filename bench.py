@@ -255,6 +255,8 @@ def main():
                    "pairs": n, "form": args.form, "wf_length": W, "penalties": list(PEN), "seed": SEED + soff},
         "gcups": cells / (wall / args.steps) / 1e9,
         "kernel_ms": kernel_ms,
+        "launches_per_step": t.get("launches"),
+        "pairs_widened_per_step": t.get("retried"),
         "bit_exact_sample": exact,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic},

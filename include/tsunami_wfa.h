@@ -153,6 +153,7 @@ int tsu_batch_create(tsu_ctx* ctx, const uint32_t* words, int64_t n_words, const
 int32_t tsu_batch_size(const tsu_batch* b);
 /* Bytes of HBM the batch holds (packed words + descriptors + score slots). */
 int64_t tsu_batch_bytes(const tsu_batch* b);
+/* May be called before or after tsu_destroy of the batch's context. */
 void tsu_batch_free(tsu_batch* b);
 
 /* ---- alignment ----------------------------------------------------------- */
@@ -161,7 +162,8 @@ void tsu_batch_free(tsu_batch* b);
  * reference's argv[5] (wfa_gpu.cu:63,217).  Replaces the launch of
  * wfa_dispatcher<<<N,32>>> and the score copy (wfa_gpu.cu:113-128,191-194).
  * scores[i] receives pair i's score; a pair whose diagonal range exceeds the
- * capacity gets TSU_EKOVERFLOW and the call returns TSU_EKOVERFLOW. */
+ * capacity gets TSU_EKOVERFLOW and the call returns TSU_EKOVERFLOW.  The
+ * capacity used is at least wf_length (it may be rounded up to 64 diagonals). */
 int tsu_align_plain(tsu_ctx* ctx, tsu_batch* b, int32_t wf_length, int32_t* scores, tsu_timing* t);
 
 /* Reference-shaped convenience: num_couples pairs of seq_len bases in the

@@ -223,3 +223,54 @@ def test_wfa_gpu_cli_check_mode(repo, tsu, tmp_path):
     assert lines[2].startswith("CPU Time: ")
     assert lines[3].startswith("Estimated GCUPS SW: ")
     assert "|ALIGNEMENT FINISHED CORRECTLY|" in r.stdout
+
+
+def test_register_and_lds_kernels_agree(aligner, tsu, oracle, fixtures):
+    """4/6/2 pairs run through the register-resident kernel (capacity <= 512)
+    and the LDS kernel (capacity > 512) give the oracle's scores; narrower
+    register windows either agree or report overflow."""
+    cases = [c for c in fixtures if tuple(c["penalties"]) == (4, 6, 2)]
+    pairs = [(c["pattern"], c["text"]) for c in cases]
+    pairs += tsu.Dataset.generate(400, seed=SEED + 11, len_min=1, len_max=300, err_min=0.0, err_max=0.3).pairs()
+    pairs += [("", ""), ("A", ""), ("", "C"), ("ACGT" * 60, "ACGT" * 60), ("A" * 200, "C" * 200)]
+    want, _, _ = oracle.wfa_batch(pairs, 4, 6, 2)
+    b = aligner.batch(pairs)
+    for W in (512, 520, 2000):
+        say(f"reg/lds W={W}")
+        got, _ = aligner.align_plain(b, W)
+        assert (got == want).all(), W
+    for W in (64, 128, 256):
+        say(f"reg narrow W={W}")
+        got, _ = aligner.align_plain(b, W, raise_on_error=False)
+        ok = got != tsu.TSU_EKOVERFLOW
+        assert ok.sum() > 0
+        assert (got[ok] == want[ok]).all(), W
+        assert (got[~ok] == tsu.TSU_EKOVERFLOW).all()
+
+
+def test_autotune_then_batched(tsu, oracle):
+    """The autotuner fills a per-bucket table; batched results stay exact."""
+    ds = tsu.Dataset.generate(3000, seed=SEED + 12, len_min=100, len_max=3000, err_min=0.01, err_max=0.15)
+    want, _, _ = oracle.wfa_batch(ds.pairs(), 4, 6, 2)
+    a = tsu.Aligner(4, 6, 2)
+    try:
+        b = a.batch(ds)
+        say("autotune 3000 mixed")
+        table = a.autotune(b, 512)
+        assert table and all(e["waves_per_pair"] in (1, 2, 4, 8, 16) for e in table)
+        got, _ = a.align_batched(b)
+        assert (got == want).all()
+    finally:
+        a.close()
+
+
+def test_long_read_cigars(aligner, tsu, oracle):
+    """Config-3 shape with CIGARs: 10,000 bp at 10% (4 pairs, ring in HBM)."""
+    ds = tsu.Dataset.generate(4, seed=SEED + 13, len_min=10000, err_min=0.10)
+    pairs = ds.pairs()
+    want, ops, _ = oracle.wfa_batch(pairs, 4, 6, 2, cigar=True)
+    say("cigar long 4x10000")
+    got, cig, _ = aligner.align_cigar(aligner.batch(ds))
+    assert (got == want).all()
+    for i, (p, t) in enumerate(pairs):
+        assert cig[i] == oracle.rle(ops[i]), i

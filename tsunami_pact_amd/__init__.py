@@ -12,6 +12,7 @@ from __future__ import annotations
 
 import ctypes
 import os
+import weakref
 from typing import Iterable, List, Optional, Sequence, Tuple
 
 import numpy as np
@@ -305,6 +306,7 @@ class Aligner:
             raise TsuError(r, f"tsu_create: {strerror(r)}")
         self._h = h
         self.penalties = (mismatch, gap_opening, gap_extension)
+        self._batches = weakref.WeakSet()
 
     def _check(self, r: int, what: str) -> None:
         if r != TSU_OK:
@@ -322,7 +324,9 @@ class Aligner:
             count = len(ds) - first
         h = ctypes.c_void_p()
         self._check(self._lib.tsu_batch_from_dataset(self._h, ds._h, first, count, ctypes.byref(h)), "tsu_batch_from_dataset")
-        return Batch(self, h)
+        b = Batch(self, h)
+        self._batches.add(b)
+        return b
 
     def align_plain(self, batch: Batch, wf_length: int, raise_on_error: bool = True):
         n = len(batch)
@@ -389,6 +393,8 @@ class Aligner:
 
     def close(self) -> None:
         if self._h:
+            for b in list(self._batches):
+                b.close()
             self._lib.tsu_destroy(self._h)
             self._h = None
 

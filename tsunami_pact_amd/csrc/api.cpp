@@ -165,6 +165,7 @@ static int batch_upload(tsu_ctx* ctx, const uint32_t* words, int64_t n_words, co
     tsu_batch* b = new (std::nothrow) tsu_batch();
     if (!b) return fail(ctx, TSU_ENOMEM, "host allocation");
     b->ctx = ctx;
+    b->device = ctx->device;
     b->n = n;
     b->n_words = n_words;
     try { b->pairs.assign(pairs, pairs + n); } catch (...) { delete b; return fail(ctx, TSU_ENOMEM, "host allocation"); }
@@ -216,7 +217,7 @@ extern "C" int64_t tsu_batch_bytes(const tsu_batch* b) {
 
 extern "C" void tsu_batch_free(tsu_batch* b) {
     if (!b) return;
-    if (b->ctx) (void)hipSetDevice(b->ctx->device);
+    (void)hipSetDevice(b->device);
     if (b->d_words) (void)hipFree(b->d_words);
     if (b->d_pairs) (void)hipFree(b->d_pairs);
     if (b->d_scores) (void)hipFree(b->d_scores);
@@ -308,9 +309,20 @@ static int run_shape(tsu_ctx* ctx, tsu_batch* b, Shape sh, int32_t W, const int3
 
 bool tsu_host::fixed_242(const tsu_ctx* ctx) { return ctx->xs == 2 && ctx->os == 3 && ctx->es == 1; }
 
+// Register-resident kernel window (chunks of 64 diagonals) for capacity W,
+// or 0 when the LDS kernel is used (other penalties, or W above 512).
+int tsu_host::reg_chunks(const tsu_ctx* ctx, int32_t W) {
+    if (!tsu_host::fixed_242(ctx) || W > 512) return 0;
+    int c = 1;
+    while (64 * c < W) c *= 2;
+    return c;
+}
+
 static Shape plain_shape(const tsu_ctx* ctx, int32_t W, int32_t max_len, bool* fits) {
     Shape sh{default_waves(W), max_len <= 32000, true, false, false};
     sh.fixed_242 = tsu_host::fixed_242(ctx);
+    sh.reg_c = tsu_host::reg_chunks(ctx, W);
+    if (sh.reg_c) sh.nw = 1;
     KArgs a = base_args(ctx);
     a.W = W;
     a.seq_words = seq_words_for(max_len);
@@ -424,6 +436,8 @@ int32_t tsu_host::initial_capacity(const tsu_ctx* ctx, int32_t bound) {
 Shape tsu_host::batched_shape(const tsu_ctx* ctx, int32_t W, int32_t max_len, int nw_hint) {
     Shape sh{nw_hint > 0 ? nw_hint : default_waves(W), max_len <= 32000, true, false, true};
     sh.fixed_242 = tsu_host::fixed_242(ctx);
+    sh.reg_c = nw_hint > 1 ? 0 : tsu_host::reg_chunks(ctx, W);
+    if (sh.reg_c) { sh.nw = 1; return sh; }
     KArgs a = base_args(ctx);
     a.W = W;
     a.seq_words = seq_words_for(max_len);

@@ -29,7 +29,8 @@ struct tsu_ctx {
 };
 
 struct tsu_batch {
-    tsu_ctx* ctx = nullptr;
+    tsu_ctx* ctx = nullptr;        // compared, never dereferenced after creation
+    int device = 0;                // freeing does not need the context
     uint32_t* d_words = nullptr;
     KPair* d_pairs = nullptr;
     int32_t* d_scores = nullptr;
@@ -69,5 +70,7 @@ int64_t resident_grid(const tsu_ctx* ctx, const Shape& sh, const KArgs& a, int* 
 int fail(tsu_ctx* ctx, int status, const std::string& msg);
 // The context's penalties divide to (2, 3, 1), the case with a compile-time ring.
 bool fixed_242(const tsu_ctx* ctx);
+// Chunks of 64 diagonals of the register-resident kernel for capacity W (0: LDS kernel).
+int reg_chunks(const tsu_ctx* ctx, int32_t W);
 
 }  // namespace tsu_host

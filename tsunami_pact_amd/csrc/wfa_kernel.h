@@ -347,6 +347,14 @@ __global__ __launch_bounds__(NW * 64) void wfa_kernel(KArgs a) {
                 Mt[js] = (OffT)M;
                 It[js] = (OffT)(inr ? I : NUL);
                 Dt[js] = (OffT)(inr ? D : NUL);
+                // flags read after the barrier: in-bounds state of the six
+                // boundary cells (lo, hi of M, I, D) and the end test at t-p
+                if (k == lo || k == hi) {
+                    const int side = k == hi ? 1 : 0;
+                    if (k == lo) sMisc[2] = M >= 0, sMisc[4] = in_bounds(I, k), sMisc[6] = in_bounds(D, k);
+                    if (side) sMisc[3] = M >= 0, sMisc[5] = in_bounds(I, k), sMisc[7] = in_bounds(D, k);
+                }
+                if (k == tp && inr) sMisc[8] = M >= t;
             }
             __syncthreads();  // every cell of step s is written
 
@@ -354,16 +362,14 @@ __global__ __launch_bounds__(NW * 64) void wfa_kernel(KArgs a) {
             // common case: the six boundary cells are in bounds -> no trim
             int tloM = lo, thiM = hi, tloI = hasI ? lo : 1, thiI = hasI ? hi : 0, tloD = hasD ? lo : 1, thiD = hasD ? hi : 0;
             {
-                bool bad = false;
-                if (lane < 6) {
-                    const int comp = lane >> 1;
-                    const int kk = (lane & 1) ? hi : lo;
-                    const OffT* sl = comp == 0 ? Mt : (comp == 1 ? It : Dt);
-                    const int32_t v = sl[kk - kmin + 1];
-                    const bool use = comp == 0 || (comp == 1 ? hasI : hasD);
-                    bad = use && !in_bounds(v, kk);
-                }
-                const uint64_t bb = __ballot(bad);
+                // bits 0-1: M at lo/hi, 2-3: I, 4-5: D out of bounds
+                uint32_t bb = 0;
+                if (!uni(sMisc[2])) bb |= 1u;
+                if (!uni(sMisc[3])) bb |= 2u;
+                if (hasI && !uni(sMisc[4])) bb |= 4u;
+                if (hasI && !uni(sMisc[5])) bb |= 8u;
+                if (hasD && !uni(sMisc[6])) bb |= 16u;
+                if (hasD && !uni(sMisc[7])) bb |= 32u;
                 if (bb) {
                     auto scan = [&](const OffT* sl, int& f, int& l) __attribute__((always_inline)) {
                         f = 0x3fffffff; l = -0x3fffffff;
@@ -378,14 +384,14 @@ __global__ __launch_bounds__(NW * 64) void wfa_kernel(KArgs a) {
                         }
                         if (f > l) { f = 1; l = 0; }
                     };
-                    if (bb & 3ull) scan(Mt, tloM, thiM);
-                    if (bb & 12ull) scan(It, tloI, thiI);
-                    if (bb & 48ull) scan(Dt, tloD, thiD);
+                    if (bb & 3u) scan(Mt, tloM, thiM);
+                    if (bb & 12u) scan(It, tloI, thiI);
+                    if (bb & 48u) scan(Dt, tloD, thiD);
                 }
             }
             // end test: M[s][t-p] >= t (wfa_gpu.cu:288-331)
             bool fin = false;
-            if (tp >= tloM && tp <= thiM) fin = uni((int32_t)Mt[tp - kmin + 1]) >= t;
+            if (tp >= tloM && tp <= thiM) fin = uni(sMisc[8]) != 0;
             // cells outside the kept ranges return to NULL (M cells trimmed
             // away are NULL already)
             // (the pass left NULL outside [lo, hi]; I/D cells trimmed away

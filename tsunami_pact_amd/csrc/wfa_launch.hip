@@ -12,6 +12,7 @@ KernelFn pick_nw2(const Shape& sh);
 KernelFn pick_nw4(const Shape& sh);
 KernelFn pick_nw8(const Shape& sh);
 KernelFn pick_nw16(const Shape& sh);
+KernelFn pick_reg(const Shape& sh);
 
 // Plain streaming copy (HBM byte-counter calibration).
 __global__ __launch_bounds__(256) void copy_kernel(const uint4* __restrict__ src, uint4* __restrict__ dst, int64_t n) {
@@ -31,12 +32,14 @@ int64_t ring_bytes(const Shape& sh, int32_t W, int32_t rm, int32_t ri) {
 }
 
 int64_t lds_bytes(const Shape& sh, const KArgs& a) {
+    if (sh.reg_c > 0) return 4ll * (2 * a.seq_words + 16);  // sequences + misc only
     int64_t b = lds_head_bytes(a.seq_words, a.rm + 2 * a.ri, sh.nw);
     if (sh.lds_ring) b += ring_bytes(sh, a.W, a.rm, a.ri);
     return b;
 }
 
 static KernelFn pick(const Shape& sh) {
+    if (sh.reg_c > 0) return pick_reg(sh);
     switch (sh.nw) {
     case 1: return pick_nw1(sh);
     case 2: return pick_nw2(sh);
@@ -81,6 +84,7 @@ int launch_wfa(const Shape& sh, const KArgs& a, int64_t grid, void* stream) {
         hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(f), hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         if (e != hipSuccess) return (int)e;
     }
+    (void)hipGetLastError();  // every earlier call was checked where it was made
     hipLaunchKernelGGL(f, dim3((unsigned)grid), dim3(sh.nw * 64), (size_t)lds, (hipStream_t)stream, a);
     return (int)hipGetLastError();
 }
@@ -90,6 +94,7 @@ int launch_copy(const void* src, void* dst, int64_t bytes, void* stream) {
     if (n <= 0) return 0;
     int64_t grid = (n + 255) / 256;
     if (grid > 65536) grid = 65536;
+    (void)hipGetLastError();
     hipLaunchKernelGGL(copy_kernel, dim3((unsigned)grid), dim3(256), 0, (hipStream_t)stream,
                        reinterpret_cast<const uint4*>(src), reinterpret_cast<uint4*>(dst), n);
     return (int)hipGetLastError();
