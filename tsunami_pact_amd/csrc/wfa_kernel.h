@@ -371,18 +371,26 @@ __global__ __launch_bounds__(NW * 64) void wfa_kernel(KArgs a) {
                 if (hasD && !uni(sMisc[6])) bb |= 16u;
                 if (hasD && !uni(sMisc[7])) bb |= 32u;
                 if (bb) {
+                    // first in-bounds cell from lo upwards, then last from hi
+                    // downwards; trimmed ends are short, so each walk usually
+                    // takes one 64-cell round (every wave computes the same)
                     auto scan = [&](const OffT* sl, int& f, int& l) __attribute__((always_inline)) {
-                        f = 0x3fffffff; l = -0x3fffffff;
-                        for (int kb = lo; kb <= hi; kb += 64) {
+                        f = 1; l = 0;
+                        int kb = lo;
+                        for (; kb <= hi; kb += 64) {
                             const int k = kb + lane;
                             const bool in = k <= hi && in_bounds((int32_t)sl[min(k, hi) - kmin + 1], k);
                             const uint64_t b = __ballot(in);
-                            if (b) {
-                                if (f == 0x3fffffff) f = kb + (int)__builtin_ctzll(b);
-                                l = kb + 63 - (int)__builtin_clzll(b);
-                            }
+                            if (b) { f = kb + (int)__builtin_ctzll(b); break; }
                         }
-                        if (f > l) { f = 1; l = 0; }
+                        if (kb > hi) return;  // no cell in bounds: empty range
+                        // cell f is in bounds, so this walk stops at f at the latest
+                        for (int kt = hi;; kt -= 64) {
+                            const int k = kt - lane;
+                            const bool in = k >= f && in_bounds((int32_t)sl[max(k, f) - kmin + 1], k);
+                            const uint64_t b = __ballot(in);
+                            if (b) { l = kt - (int)__builtin_ctzll(b); break; }
+                        }
                     };
                     if (bb & 3u) scan(Mt, tloM, thiM);
                     if (bb & 12u) scan(It, tloI, thiI);
