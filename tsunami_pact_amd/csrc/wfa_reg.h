@@ -31,6 +31,10 @@
 namespace tsu {
 
 constexpr int32_t kRegNull = -1073741824;
+// per-step chunk flag bits (chunk c adds c to the M, I and D positions)
+constexpr uint64_t kBitM = 1ull << 1, kBitI = 1ull << 17, kBitD = 1ull << 33;
+
+__device__ __forceinline__ uint64_t vote(bool b) { return __builtin_amdgcn_ballot_w64(b); }
 
 __device__ __forceinline__ int32_t dpp_shr1(int32_t fill_lane0, int32_t v) {
     // lane l receives lane l-1; lane 0 keeps fill_lane0
@@ -44,7 +48,7 @@ __device__ __forceinline__ int32_t dpp_shl1(int32_t fill_lane63, int32_t v) {
 template <int C, int XS, int OS, int ES, bool QUEUE>
 __global__ __launch_bounds__(64) void wfa_reg_kernel(KArgs a) {
     static_assert(ES == 1, "in-place I/D needs e == 1 after gcd scaling");
-    static_assert(C >= 1 && C <= 16, "window of 64..1024 diagonals");
+    static_assert(C >= 1 && C <= 16, "window of 64..1024 diagonals (16 flag bits per kind)");
     constexpr int RM = (XS > OS + ES ? XS : OS + ES) + 1;
     constexpr int32_t NUL = kRegNull;
     constexpr int W = 64 * C;
@@ -82,6 +86,10 @@ __global__ __launch_bounds__(64) void wfa_reg_kernel(KArgs a) {
         int kmax = kmin + W - 1;
         if (kmax > t + 1) { kmax = t + 1; kmin = max(kmax - W + 1, -p - 1); }
         const int tp = t - p;
+        // chunk and lane of the end diagonal t - p (a chunk index >= C or < 0
+        // never matches: the pair then ends by overflow at the window edge)
+        const int ctp = (tp - kmin) >= 0 ? (tp - kmin) >> 6 : -1;
+        const int ltp = (tp - kmin) & 63;
         const int64_t bound_u = (int64_t)XS * min(p, t) + (p != t ? (int64_t)OS + (int64_t)ES * abs(p - t) : 0);
         const int s_bound = (int)min<int64_t>(bound_u, 0x3fffffff);
 
@@ -120,61 +128,76 @@ __global__ __launch_bounds__(64) void wfa_reg_kernel(KArgs a) {
             const uint32_t srcI = amM[SO] | amI;
             const uint32_t srcD = amM[SO] | amD;
             const uint32_t act = (amM[SX] | srcI | (srcI << 1) | srcD | (srcD >> 1) | amM[PH]) & ALL;
-            uint32_t nnM = 0, nnI = 0, nnD = 0;
-            bool oob = false, fin = false;
+            uint64_t nn = 0;  // chunk flags: M at bit 1 + c, I at 17 + c, D at 33 + c
+            uint64_t oob = 0;
+            int32_t mend = NUL;  // M[s][t-p]
             int32_t carryI = NUL;  // old I of the previous chunk's lane 63
 #pragma unroll
             for (int c = 0; c < C; ++c) {
                 if (act & (1u << c)) {
                     const int k = kmin + 64 * c + lane;
+                    // a value y >= 0 on diagonal k is in bounds iff y <= t and
+                    // y - k <= p, i.e. y < mt1; mt1 >= 0 since k >= -p-1
+                    const int mt1 = min(k + p + 1, t + 1);
                     const int32_t mo = Mr[SO][c];
                     const int32_t mol = dpp_shr1(c > 0 ? __builtin_amdgcn_readlane(Mr[SO][c > 0 ? c - 1 : 0], 63) : NUL, mo);
                     const int32_t mor = dpp_shl1(c < C - 1 ? __builtin_amdgcn_readlane(Mr[SO][c < C - 1 ? c + 1 : c], 0) : NUL, mo);
                     const int32_t iel = dpp_shr1(carryI, Ir[c]);
                     const int32_t der = dpp_shl1(c < C - 1 ? __builtin_amdgcn_readlane(Dr[c < C - 1 ? c + 1 : c], 0) : NUL, Dr[c]);
                     carryI = __builtin_amdgcn_readlane(Ir[c], 63);
-                    const int32_t mx = Mr[SX][c];
-                    const int32_t ipre = max(mol, iel);
-                    const int32_t I = ipre >= 0 ? min(ipre + 1, t + 1) : NUL;
+                    // NULL sources stay negative through the +1s below (at
+                    // most s_bound of them), so I, D and X need no NULL test;
+                    // M is put back to the canonical NULL by the bounds test
+                    const int32_t I = min(max(mol, iel) + 1, t + 1);
                     const int32_t D = max(mor, der);
-                    const int32_t X = mx >= 0 ? mx + 1 : NUL;
-                    int32_t M = max(D, max(X, I));
-                    if ((uint32_t)M > (uint32_t)t || (uint32_t)(M - k) > (uint32_t)p) M = NUL;
-                    {
+                    int32_t M = max(max(D, Mr[SX][c] + 1), I);
+                    const bool valid = (uint32_t)M < (uint32_t)mt1;
+                    M = valid ? M : NUL;
+                    const int lim = mt1 - 1 - M;  // min(p - v, t - M) for a valid M
+                    if (valid && lim > 0) {
+                        // extend: 32 bases per round from the 2-bit packed
+                        // sequences; alignbit takes the shift modulo 32
                         const int v = M - k;
-                        const int lim = M >= 0 ? min(p - v, t - M) : 0;
-                        bool go = lim > 0;
-                        int n = 0;
-                        while (__ballot(go)) {
-                            const int pa = go ? v + n : 0;
-                            const int pb = go ? M + n : 0;
-                            const int ja = pa >> 4, sa = (pa & 15) << 1;
-                            const int jb = pb >> 4, sb = (pb & 15) << 1;
-                            const uint32_t p0 = sP[ja], p1 = sP[ja + 1], p2 = sP[ja + 2];
-                            const uint32_t t0 = sT[jb], t1 = sT[jb + 1], t2 = sT[jb + 2];
+                        const uint32_t* pp = sP + (v >> 4);
+                        const uint32_t* tt = sT + (M >> 4);
+                        const uint32_t sa = (uint32_t)v << 1, sb = (uint32_t)M << 1;
+                        const int lim2 = 2 * lim;
+                        int n2 = 0;  // matched bits, 2 per base
+                        for (;;) {
+                            const uint32_t p0 = pp[0], p1 = pp[1], p2 = pp[2];
+                            const uint32_t t0 = tt[0], t1 = tt[1], t2 = tt[2];
                             const uint32_t xl = __builtin_amdgcn_alignbit(p1, p0, sa) ^ __builtin_amdgcn_alignbit(t1, t0, sb);
                             const uint32_t xh = __builtin_amdgcn_alignbit(p2, p1, sa) ^ __builtin_amdgcn_alignbit(t2, t1, sb);
                             const uint64_t xx = ((uint64_t)xh << 32) | xl;
-                            const int m = (xx ? (int)__builtin_ctzll(xx) : 64) >> 1;
-                            n = go ? n + m : n;
-                            go = go && m == 32 && n < lim;
+                            const int m = xx ? (int)__builtin_ctzll(xx) : 64;
+                            n2 += m;
+                            if (m != 64 || n2 >= lim2) break;
+                            pp += 2;
+                            tt += 2;
                         }
-                        M += min(n, lim);
+                        M += min(n2 >> 1, lim);
                     }
                     Mr[PH][c] = M;
                     Ir[c] = I;
                     Dr[c] = D;
-                    const bool iI = (uint32_t)I <= (uint32_t)t && (uint32_t)(I - k) <= (uint32_t)p;
-                    const bool iD = (uint32_t)D <= (uint32_t)t && (uint32_t)(D - k) <= (uint32_t)p;
-                    if (__ballot(M >= 0)) nnM |= 1u << c;
-                    if (__ballot(I >= 0)) nnI |= 1u << c;
-                    if (__ballot(D >= 0)) nnD |= 1u << c;
-                    oob |= __ballot((I >= 0 && !iI) || (D >= 0 && !iD)) != 0;
-                    fin |= __ballot(k == tp && M >= t) != 0;
+                    // chunk flags as one scalar integer pinned to an SGPR, so
+                    // the compiler carries no lane booleans across the chunk
+                    // branches (two vector ops per flag); no flag uses bit 0,
+                    // whose zero-extension is also lowered through a VGPR
+                    uint64_t fl = (vote(M >= 0) ? kBitM : 0ull) | (vote(I >= 0) ? kBitI : 0ull) |
+                                  (vote(D >= 0) ? kBitD : 0ull);
+                    asm volatile("" : "+s"(fl));
+                    nn |= fl << c;
+                    if (c == ctp) mend = __builtin_amdgcn_readlane(M, ltp);
+                    // I <= t + 1 and D <= t by construction: out of bounds iff >= mt1
+                    oob |= vote(max(I, D) >= mt1);
                 } else {
                     carryI = NUL;  // an idle chunk holds only NULL
                 }
             }
+            uint32_t nnM = (uint32_t)(nn >> 1) & ALL;
+            uint32_t nnI = (uint32_t)(nn >> 17) & ALL;
+            uint32_t nnD = (uint32_t)(nn >> 33) & ALL;
             if (oob) {
                 // exact trim: NULL I/D cells outside [first, last] in-bounds cell
                 auto trim = [&](int32_t (&R)[C], uint32_t& nn) __attribute__((always_inline)) {
@@ -182,7 +205,8 @@ __global__ __launch_bounds__(64) void wfa_reg_kernel(KArgs a) {
 #pragma unroll
                     for (int c = 0; c < C; ++c) {
                         const int k = kmin + 64 * c + lane;
-                        const uint64_t inb = __ballot((uint32_t)R[c] <= (uint32_t)t && (uint32_t)(R[c] - k) <= (uint32_t)p);
+                        const int mt1 = min(k + p + 1, t + 1);
+                        const uint64_t inb = vote((uint32_t)R[c] < (uint32_t)mt1);
                         if (inb) {
                             f = min(f, 64 * c + (int)__builtin_ctzll(inb));
                             l = max(l, 64 * c + 63 - (int)__builtin_clzll(inb));
@@ -193,7 +217,7 @@ __global__ __launch_bounds__(64) void wfa_reg_kernel(KArgs a) {
                     for (int c = 0; c < C; ++c) {
                         const int w = 64 * c + lane;
                         if (w < f || w > l) R[c] = NUL;
-                        if (__ballot(R[c] >= 0)) nn |= 1u << c;
+                        if (vote(R[c] >= 0)) nn |= 1u << c;
                     }
                 };
                 trim(Ir, nnI);
@@ -202,7 +226,7 @@ __global__ __launch_bounds__(64) void wfa_reg_kernel(KArgs a) {
             amM[PH] = nnM;
             amI = nnI;
             amD = nnD;
-            if (fin) return s;
+            if (mend >= t) return s;
             // a non-NULL cell at a window edge could extend the range beyond it
             uint32_t edge = 0;
             {
