@@ -5,19 +5,23 @@
 // and the I / D wavefronts (updated in place, which needs e = 1 after gcd
 // scaling) are arrays of VGPRs indexed by compile-time chunk and slot.  The
 // k-1 / k+1 neighbours come from DPP wave shifts (wave_shr:1 / wave_shl:1)
-// with the boundary lane taken from the adjacent chunk by v_readlane.  The
-// score loop is unrolled over the M ring's depth so every slot index is a
-// constant.
+// whose boundary lane is taken from the adjacent chunk, rotated into place
+// by a DPP wave rotate (or by v_readlane for the in-place I).  The score loop
+// is unrolled over the M ring's depth so every slot index is a constant.
 //
 // All diagonals of the C*64-wide window are computed implicitly: a chunk is
-// computed only when one of its sources can be non-NULL (a scalar bit mask
-// per slot, dilated by one chunk for the k+-1 neighbours), and a diagonal
+// computed only when one of its sources can be non-NULL, and a diagonal
 // outside WFA2-lib's range [lo, hi] has only NULL sources, so it comes out
-// NULL exactly as the range-limited computation leaves it.  Trim-ends
-// (wfa_gpu.cu:633-719) reduce to: NULL every I/D cell below the first and
-// above the last in-bounds cell of that wavefront; M cells outside the
-// bounds are NULL already.  The common case (no out-of-bounds I/D value at
-// all) is detected from one ballot per chunk.
+// NULL exactly as the range-limited computation leaves it.  Activity is
+// tracked per chunk with scalar bit masks: any non-NULL value, and a
+// non-NULL value in lane 0 / lane 63 (only those reach the neighbouring
+// chunk through k-1 / k+1).  Trim-ends (wfa_gpu.cu:633-719) reduce to: NULL
+// every I/D cell below the first and above the last in-bounds cell of that
+// wavefront; M cells outside the bounds are NULL already.  The common case
+// (no out-of-bounds I/D value at all) is detected from one ballot per chunk.
+//
+// Sequences sit in LDS as interleaved 2-bit packed words (pattern word i at
+// 2i, text word i at 2i+1), so both are addressed from LDS offset 0.
 //
 // Recurrences, bounds test, end test and extend are those of the LDS kernel
 // (wfa_gpu.cu:253-331,610-631).
@@ -31,8 +35,13 @@
 namespace tsu {
 
 constexpr int32_t kRegNull = -1073741824;
-// per-step chunk flag bits (chunk c adds c to the M, I and D positions)
-constexpr uint64_t kBitM = 1ull << 1, kBitI = 1ull << 17, kBitD = 1ull << 33;
+
+// Chunk masks: bit c of each 8-bit field is chunk c.  A slot mask (uint32)
+// is  any non-NULL | lane 0 non-NULL << 8 | lane 63 non-NULL << 16.
+// The per-step flags gathered from the chunks (uint64, shifted by c) put
+// these fields at the bit positions below; no field starts at bit 0 (see
+// the note where they are gathered).
+constexpr int kFm = 1, kFmLo = 9, kFmHi = 17, kFi = 25, kFiHi = 33, kFd = 41, kFdLo = 49;
 
 __device__ __forceinline__ uint64_t vote(bool b) { return __builtin_amdgcn_ballot_w64(b); }
 
@@ -44,20 +53,70 @@ __device__ __forceinline__ int32_t dpp_shl1(int32_t fill_lane63, int32_t v) {
     // lane l receives lane l+1; lane 63 keeps fill_lane63
     return __builtin_amdgcn_update_dpp(fill_lane63, v, 0x130, 0xF, 0xF, false);
 }
+__device__ __forceinline__ int32_t dpp_ror1(int32_t v) {
+    // lane l receives lane l-1 mod 64 (lane 0 gets lane 63); every lane has
+    // a source, so no old value is kept
+    return __builtin_amdgcn_mov_dpp(v, 0x13C, 0xF, 0xF, false);
+}
+__device__ __forceinline__ int32_t dpp_rol1(int32_t v) {
+    // lane l receives lane l+1 mod 64 (lane 63 gets lane 0)
+    return __builtin_amdgcn_mov_dpp(v, 0x134, 0xF, 0xF, false);
+}
+
+// Compare 32 bases of pattern and text from the interleaved words at byte
+// offsets bp / bt (word j at 8j, text words 4 bytes further), shifted by sa /
+// sb bits; returns the number of equal bits (64 when all 32 bases match).
+typedef __attribute__((address_space(3))) const uint32_t lds_cu32;
+__device__ __forceinline__ int cmp32(uint32_t bp, uint32_t bt, uint32_t sa, uint32_t sb) {
+    // The interleaved words start the kernel's dynamic LDS, which is LDS
+    // address 0 (the kernel has no static LDS; it checks this and reports
+    // TSU_EINTERNAL otherwise), so no base is added to the byte offsets.
+    lds_cu32* pp = reinterpret_cast<lds_cu32*>((uintptr_t)bp);
+    lds_cu32* tt = reinterpret_cast<lds_cu32*>((uintptr_t)bt);
+    const uint32_t p0 = pp[0], p1 = pp[2], p2 = pp[4];
+    const uint32_t t0 = tt[1], t1 = tt[3], t2 = tt[5];
+    const uint32_t xl = __builtin_amdgcn_alignbit(p1, p0, sa) ^ __builtin_amdgcn_alignbit(t1, t0, sb);
+    const uint32_t xh = __builtin_amdgcn_alignbit(p2, p1, sa) ^ __builtin_amdgcn_alignbit(t2, t1, sb);
+    const uint64_t xx = ((uint64_t)xh << 32) | xl;
+    return xx ? (int)__builtin_ctzll(xx) : 64;
+}
+
+// Longest common prefix of pattern[v..] and text[h..] (at most lim bases)
+// over the interleaved layout; the same 32-bases-per-round compare as the
+// extend in the kernel.
+__device__ __forceinline__ int lcp_interleaved(const uint32_t* S, int v, int h, int lim) {
+    const uint32_t* pp = S + 2 * (v >> 4);
+    const uint32_t* tt = S + 2 * (h >> 4) + 1;
+    const uint32_t sa = (uint32_t)v << 1, sb = (uint32_t)h << 1;
+    const int lim2 = 2 * lim;
+    int n2 = 0;
+    for (;;) {
+        const uint32_t xl = __builtin_amdgcn_alignbit(pp[2], pp[0], sa) ^ __builtin_amdgcn_alignbit(tt[2], tt[0], sb);
+        const uint32_t xh = __builtin_amdgcn_alignbit(pp[4], pp[2], sa) ^ __builtin_amdgcn_alignbit(tt[4], tt[2], sb);
+        const uint64_t xx = ((uint64_t)xh << 32) | xl;
+        const int m = xx ? (int)__builtin_ctzll(xx) : 64;
+        n2 += m;
+        if (m != 64 || n2 >= lim2) break;
+        pp += 4;
+        tt += 4;
+    }
+    return min(n2 >> 1, lim);
+}
 
 template <int C, int XS, int OS, int ES, bool QUEUE>
 __global__ __launch_bounds__(64) void wfa_reg_kernel(KArgs a) {
     static_assert(ES == 1, "in-place I/D needs e == 1 after gcd scaling");
-    static_assert(C >= 1 && C <= 16, "window of 64..1024 diagonals (16 flag bits per kind)");
+    static_assert(C >= 1 && C <= 8, "window of 64..512 diagonals (8-bit chunk fields)");
     constexpr int RM = (XS > OS + ES ? XS : OS + ES) + 1;
     constexpr int32_t NUL = kRegNull;
     constexpr int W = 64 * C;
-    constexpr uint32_t ALL = C == 32 ? 0xffffffffu : ((1u << C) - 1u);
+    constexpr uint32_t ALL = (1u << C) - 1u;
     extern __shared__ __align__(16) uint32_t smem[];
     const int lane = threadIdx.x;
-    uint32_t* sP = smem;
-    uint32_t* sT = smem + a.seq_words;
-    int32_t* sMisc = reinterpret_cast<int32_t*>(sT + a.seq_words);
+    uint32_t* sS = smem;  // interleaved pattern / text words
+    int32_t* sMisc = reinterpret_cast<int32_t*>(sS + 2 * a.seq_words);
+    // the extend addresses the words from LDS address 0 (see cmp32)
+    const bool lds_at_zero = (uint32_t)(uintptr_t)sS == 0u;
 
     for (int iter = 0;; ++iter) {
         int idx;
@@ -77,8 +136,8 @@ __global__ __launch_bounds__(64) void wfa_reg_kernel(KArgs a) {
         const int t = __builtin_amdgcn_readfirstlane(pr.t_len);
         const int pw = (p + 15) >> 4, tw = (t + 15) >> 4;
         for (int i = lane; i < a.seq_words; i += 64) {
-            sP[i] = i < pw ? a.words[pr.p_off + i] : 0u;
-            sT[i] = i < tw ? a.words[pr.t_off + i] : 0u;
+            sS[2 * i] = i < pw ? a.words[pr.p_off + i] : 0u;
+            sS[2 * i + 1] = i < tw ? a.words[pr.t_off + i] : 0u;
         }
         __syncthreads();
 
@@ -101,19 +160,19 @@ __global__ __launch_bounds__(64) void wfa_reg_kernel(KArgs a) {
             Ir[c] = NUL;
             Dr[c] = NUL;
         }
-        uint32_t amM[RM];   // chunks holding a non-NULL M value, per slot
+        uint32_t amM[RM];  // slot masks of the M ring
 #pragma unroll
         for (int r = 0; r < RM; ++r) amM[r] = 0;
-        uint32_t amI = 0, amD = 0;
+        uint32_t amI = 0, amD = 0;  // the I mask needs no lane-0 field, the D mask no lane-63 field
 
         // score 0: M[0][0] = 0, extended (every lane computes it)
-        const int ext0 = lcp_packed(sP, sT, 0, 0, min(p, t));
+        const int ext0 = lcp_interleaved(sS, 0, 0, min(p, t));
         {
             const int c0 = (0 - kmin) >> 6, l0 = (0 - kmin) & 63;
 #pragma unroll
             for (int c = 0; c < C; ++c)
                 if (c == c0) Mr[0][c] = lane == l0 ? ext0 : NUL;
-            amM[0] = 1u << c0;
+            amM[0] = (1u << c0) | (l0 == 0 ? 1u << (8 + c0) : 0u) | (l0 == 63 ? 1u << (16 + c0) : 0u);
         }
         int s = 0;
 
@@ -125,12 +184,16 @@ __global__ __launch_bounds__(64) void wfa_reg_kernel(KArgs a) {
             constexpr int SO = (PH - OS - ES + 4 * RM) % RM;
             ++s;
             if (s > s_bound) return TSU_EINTERNAL;
-            const uint32_t srcI = amM[SO] | amI;
-            const uint32_t srcD = amM[SO] | amD;
-            const uint32_t act = (amM[SX] | srcI | (srcI << 1) | srcD | (srcD >> 1) | amM[PH]) & ALL;
-            uint64_t nn = 0;  // chunk flags: M at bit 1 + c, I at 17 + c, D at 33 + c
-            uint64_t oob = 0;
-            int32_t mend = NUL;  // M[s][t-p]
+            // chunk c computes when it holds a non-NULL source or an old value
+            // to overwrite, or when lane 63 of chunk c-1 feeds its I (k-1) or
+            // lane 0 of chunk c+1 feeds its D (k+1)
+            const uint32_t any = amM[SX] | amM[SO] | amI | amD | amM[PH];
+            const uint32_t spill_r = ((amM[SO] | amI) >> 16) << 1;
+            const uint32_t spill_l = (((amM[SO] | amD) >> 8) & 0xffu) >> 1;
+            const uint32_t act = (any | spill_r | spill_l) & ALL;
+            uint64_t nn = 0;       // gathered chunk flags
+            uint64_t oob = 0;      // some I/D cell out of bounds
+            int32_t mend = NUL;    // M[s][t-p]
             int32_t carryI = NUL;  // old I of the previous chunk's lane 63
 #pragma unroll
             for (int c = 0; c < C; ++c) {
@@ -140,67 +203,71 @@ __global__ __launch_bounds__(64) void wfa_reg_kernel(KArgs a) {
                     // y - k <= p, i.e. y < mt1; mt1 >= 0 since k >= -p-1
                     const int mt1 = min(k + p + 1, t + 1);
                     const int32_t mo = Mr[SO][c];
-                    const int32_t mol = dpp_shr1(c > 0 ? __builtin_amdgcn_readlane(Mr[SO][c > 0 ? c - 1 : 0], 63) : NUL, mo);
-                    const int32_t mor = dpp_shl1(c < C - 1 ? __builtin_amdgcn_readlane(Mr[SO][c < C - 1 ? c + 1 : c], 0) : NUL, mo);
+                    const int32_t mol = dpp_shr1(c > 0 ? dpp_ror1(Mr[SO][c > 0 ? c - 1 : 0]) : NUL, mo);
+                    const int32_t mor = dpp_shl1(c < C - 1 ? dpp_rol1(Mr[SO][c < C - 1 ? c + 1 : c]) : NUL, mo);
                     const int32_t iel = dpp_shr1(carryI, Ir[c]);
-                    const int32_t der = dpp_shl1(c < C - 1 ? __builtin_amdgcn_readlane(Dr[c < C - 1 ? c + 1 : c], 0) : NUL, Dr[c]);
+                    const int32_t der = dpp_shl1(c < C - 1 ? dpp_rol1(Dr[c < C - 1 ? c + 1 : c]) : NUL, Dr[c]);
                     carryI = __builtin_amdgcn_readlane(Ir[c], 63);
                     // NULL sources stay negative through the +1s below (at
                     // most s_bound of them), so I, D and X need no NULL test;
-                    // M is put back to the canonical NULL by the bounds test
-                    const int32_t I = min(max(mol, iel) + 1, t + 1);
+                    // M is put back to the canonical NULL by the bounds test.
+                    // I is not clamped: an I above t is out of bounds and
+                    // only produces out-of-bounds values, as in WFA2-lib.
+                    const int32_t I = max(mol, iel) + 1;
                     const int32_t D = max(mor, der);
                     int32_t M = max(max(D, Mr[SX][c] + 1), I);
                     const bool valid = (uint32_t)M < (uint32_t)mt1;
                     M = valid ? M : NUL;
                     const int lim = mt1 - 1 - M;  // min(p - v, t - M) for a valid M
                     if (valid && lim > 0) {
-                        // extend: 32 bases per round from the 2-bit packed
-                        // sequences; alignbit takes the shift modulo 32
-                        const int v = M - k;
-                        const uint32_t* pp = sP + (v >> 4);
-                        const uint32_t* tt = sT + (M >> 4);
-                        const uint32_t sa = (uint32_t)v << 1, sb = (uint32_t)M << 1;
+                        // extend: 32 bases per round from the packed words
+                        // (v >= 0 for every valid M); alignbit takes the
+                        // shift modulo 32.  The first round is peeled: most
+                        // diagonals stop in it.
+                        const uint32_t v = (uint32_t)(M - k);
+                        uint32_t bp = (v >> 4) << 3, bt = ((uint32_t)M >> 4) << 3;
+                        const uint32_t sa = v << 1, sb = (uint32_t)M << 1;
                         const int lim2 = 2 * lim;
-                        int n2 = 0;  // matched bits, 2 per base
-                        for (;;) {
-                            const uint32_t p0 = pp[0], p1 = pp[1], p2 = pp[2];
-                            const uint32_t t0 = tt[0], t1 = tt[1], t2 = tt[2];
-                            const uint32_t xl = __builtin_amdgcn_alignbit(p1, p0, sa) ^ __builtin_amdgcn_alignbit(t1, t0, sb);
-                            const uint32_t xh = __builtin_amdgcn_alignbit(p2, p1, sa) ^ __builtin_amdgcn_alignbit(t2, t1, sb);
-                            const uint64_t xx = ((uint64_t)xh << 32) | xl;
-                            const int m = xx ? (int)__builtin_ctzll(xx) : 64;
-                            n2 += m;
-                            if (m != 64 || n2 >= lim2) break;
-                            pp += 2;
-                            tt += 2;
+                        int m = cmp32(bp, bt, sa, sb);
+                        int n2 = m;  // matched bits, 2 per base
+                        if (m == 64 && n2 < lim2) {
+                            do {
+                                bp += 16;
+                                bt += 16;
+                                m = cmp32(bp, bt, sa, sb);
+                                n2 += m;
+                            } while (m == 64 && n2 < lim2);
                         }
                         M += min(n2 >> 1, lim);
                     }
                     Mr[PH][c] = M;
                     Ir[c] = I;
                     Dr[c] = D;
-                    // chunk flags as one scalar integer pinned to an SGPR, so
-                    // the compiler carries no lane booleans across the chunk
-                    // branches (two vector ops per flag); no flag uses bit 0,
-                    // whose zero-extension is also lowered through a VGPR
-                    uint64_t fl = (vote(M >= 0) ? kBitM : 0ull) | (vote(I >= 0) ? kBitI : 0ull) |
-                                  (vote(D >= 0) ? kBitD : 0ull);
+                    // Chunk flags as one scalar word pinned to an SGPR, so the
+                    // compiler carries no lane booleans across the chunk
+                    // branches (two vector ops per flag).  No field starts at
+                    // bit 0: a select of 1 : 0 becomes a zero-extension, which
+                    // is also lowered through a VGPR.
+                    const uint64_t bM = vote(M >= 0), bI = vote(I >= 0), bD = vote(D >= 0);
+                    uint64_t fl = (bM ? 1ull << kFm : 0ull) | ((bM & 1ull) << kFmLo) | ((bM >> 63) << kFmHi) |
+                                  (bI ? 1ull << kFi : 0ull) | ((bI >> 63) << kFiHi) |
+                                  (bD ? 1ull << kFd : 0ull) | ((bD & 1ull) << kFdLo);
                     asm volatile("" : "+s"(fl));
                     nn |= fl << c;
                     if (c == ctp) mend = __builtin_amdgcn_readlane(M, ltp);
-                    // I <= t + 1 and D <= t by construction: out of bounds iff >= mt1
+                    // I and D are out of bounds iff >= mt1 (D <= t by construction)
                     oob |= vote(max(I, D) >= mt1);
                 } else {
                     carryI = NUL;  // an idle chunk holds only NULL
                 }
             }
-            uint32_t nnM = (uint32_t)(nn >> 1) & ALL;
-            uint32_t nnI = (uint32_t)(nn >> 17) & ALL;
-            uint32_t nnD = (uint32_t)(nn >> 33) & ALL;
+            uint32_t nnM = (uint32_t)(nn >> kFm) & 0xffffffu;
+            uint32_t nnI = ((uint32_t)(nn >> kFi) & 0xffu) | (((uint32_t)(nn >> kFiHi) & 0xffu) << 16);
+            uint32_t nnD = (uint32_t)(nn >> kFd) & 0xffffu;
             if (oob) {
-                // exact trim: NULL I/D cells outside [first, last] in-bounds cell
-                auto trim = [&](int32_t (&R)[C], uint32_t& nn) __attribute__((always_inline)) {
+                // exact trim: NULL I/D cells outside [first, last] in-bounds
+                // cell, then rebuild the slot mask (all three fields)
+                auto trim = [&](int32_t (&R)[C], uint32_t& msk) __attribute__((always_inline)) {
                     int f = 0x3fffffff, l = -1;
 #pragma unroll
                     for (int c = 0; c < C; ++c) {
@@ -212,13 +279,15 @@ __global__ __launch_bounds__(64) void wfa_reg_kernel(KArgs a) {
                             l = max(l, 64 * c + 63 - (int)__builtin_clzll(inb));
                         }
                     }
-                    nn = 0;
+                    uint32_t m2 = 0;  // fields shifted up by one bit (none at bit 0)
 #pragma unroll
                     for (int c = 0; c < C; ++c) {
                         const int w = 64 * c + lane;
                         if (w < f || w > l) R[c] = NUL;
-                        if (vote(R[c] >= 0)) nn |= 1u << c;
+                        const uint64_t b = vote(R[c] >= 0);
+                        m2 |= ((b ? 2u : 0u) | ((uint32_t)(b & 1ull) << 9) | ((uint32_t)(b >> 63) << 17)) << c;
                     }
+                    msk = m2 >> 1;
                 };
                 trim(Ir, nnI);
                 trim(Dr, nnD);
@@ -227,15 +296,12 @@ __global__ __launch_bounds__(64) void wfa_reg_kernel(KArgs a) {
             amI = nnI;
             amD = nnD;
             if (mend >= t) return s;
-            // a non-NULL cell at a window edge could extend the range beyond it
-            uint32_t edge = 0;
-            {
-                const int32_t lo_any = max(Mr[PH][0], max(Ir[0], Dr[0]));
-                const int32_t hi_any = max(Mr[PH][C - 1], max(Ir[C - 1], Dr[C - 1]));
-                edge = (__builtin_amdgcn_readlane(lo_any, 0) >= 0 ? 1u : 0u) |
-                       (__builtin_amdgcn_readlane(hi_any, 63) >= 0 ? 2u : 0u);
-            }
-            if (edge) return TSU_EKOVERFLOW;
+            // A non-NULL cell at a window edge could extend the range beyond
+            // it.  Lane 0 of chunk 0 can hold M or D (I there reads k-1, which
+            // is outside and NULL); lane 63 of chunk C-1 can hold M or I.
+            const uint32_t edges = nnM | nnI | nnD;
+            if ((edges >> 8) & 1u) return TSU_EKOVERFLOW;
+            if ((edges >> (16 + C - 1)) & 1u) return TSU_EKOVERFLOW;
             return 0;
         };
 
@@ -247,7 +313,7 @@ __global__ __launch_bounds__(64) void wfa_reg_kernel(KArgs a) {
             }(std::make_integer_sequence<int, RM>{});
         }
         const int score = (r > 0 && s == 0) ? 0 : r;  // r > 0 is the score s of the last step
-        if (lane == 0) a.scores[pi] = score < 0 ? score : score * a.g;
+        if (lane == 0) a.scores[pi] = !lds_at_zero ? TSU_EINTERNAL : score < 0 ? score : score * a.g;
         __syncthreads();
     }
 }
