@@ -36,12 +36,10 @@ namespace tsu {
 
 constexpr int32_t kRegNull = -1073741824;
 
-// Chunk masks: bit c of each 8-bit field is chunk c.  A slot mask (uint32)
-// is  any non-NULL | lane 0 non-NULL << 8 | lane 63 non-NULL << 16.
-// The per-step flags gathered from the chunks (uint64, shifted by c) put
-// these fields at the bit positions below; no field starts at bit 0 (see
-// the note where they are gathered).
-constexpr int kFm = 1, kFmLo = 9, kFmHi = 17, kFi = 25, kFiHi = 33, kFd = 41, kFdLo = 49;
+// Step masks: for the M, I and D wavefronts of one score together, bit c of
+//   bits 0-7    chunk c holds a non-NULL value
+//   bits 8-15   lane 0 of chunk c holds one (it feeds D of chunk c-1)
+//   bits 16-23  lane 63 of chunk c holds one (it feeds I of chunk c+1)
 
 __device__ __forceinline__ uint64_t vote(bool b) { return __builtin_amdgcn_ballot_w64(b); }
 
@@ -160,10 +158,9 @@ __global__ __launch_bounds__(64) void wfa_reg_kernel(KArgs a) {
             Ir[c] = NUL;
             Dr[c] = NUL;
         }
-        uint32_t amM[RM];  // slot masks of the M ring
+        uint32_t F[RM];  // step masks of the last RM scores (slot s mod RM)
 #pragma unroll
-        for (int r = 0; r < RM; ++r) amM[r] = 0;
-        uint32_t amI = 0, amD = 0;  // the I mask needs no lane-0 field, the D mask no lane-63 field
+        for (int r = 0; r < RM; ++r) F[r] = 0;
 
         // score 0: M[0][0] = 0, extended (every lane computes it)
         const int ext0 = lcp_interleaved(sS, 0, 0, min(p, t));
@@ -172,7 +169,7 @@ __global__ __launch_bounds__(64) void wfa_reg_kernel(KArgs a) {
 #pragma unroll
             for (int c = 0; c < C; ++c)
                 if (c == c0) Mr[0][c] = lane == l0 ? ext0 : NUL;
-            amM[0] = (1u << c0) | (l0 == 0 ? 1u << (8 + c0) : 0u) | (l0 == 63 ? 1u << (16 + c0) : 0u);
+            F[0] = (1u << c0) | (l0 == 0 ? 1u << (8 + c0) : 0u) | (l0 == 63 ? 1u << (16 + c0) : 0u);
         }
         int s = 0;
 
@@ -184,14 +181,14 @@ __global__ __launch_bounds__(64) void wfa_reg_kernel(KArgs a) {
             constexpr int SO = (PH - OS - ES + 4 * RM) % RM;
             ++s;
             if (s > s_bound) return TSU_EINTERNAL;
-            // chunk c computes when it holds a non-NULL source or an old value
-            // to overwrite, or when lane 63 of chunk c-1 feeds its I (k-1) or
-            // lane 0 of chunk c+1 feeds its D (k+1)
-            const uint32_t any = amM[SX] | amM[SO] | amI | amD | amM[PH];
-            const uint32_t spill_r = ((amM[SO] | amI) >> 16) << 1;
-            const uint32_t spill_l = (((amM[SO] | amD) >> 8) & 0xffu) >> 1;
-            const uint32_t act = (any | spill_r | spill_l) & ALL;
-            uint64_t nn = 0;       // gathered chunk flags
+            constexpr int P1 = (PH - 1 + RM) % RM;  // score s-1 (its I and D are the I_e, D_e sources)
+            // chunk c computes when it may hold a non-NULL source (scores s-1,
+            // s-x, s-o-e) or an old value to overwrite (scores s-1 and s-RM),
+            // or when lane 63 of chunk c-1 feeds its I (k-1) or lane 0 of
+            // chunk c+1 feeds its D (k+1)
+            const uint32_t src = F[P1] | F[SO];
+            const uint32_t act = ((src | F[SX] | F[PH]) | ((src >> 16) << 1) | (((src >> 8) & 0xffu) >> 1)) & ALL;
+            uint32_t acc = 0;      // step mask of score s, fields shifted up by one bit
             uint64_t oob = 0;      // some I/D cell out of bounds
             int32_t mend = NUL;    // M[s][t-p]
             int32_t carryI = NUL;  // old I of the previous chunk's lane 63
@@ -245,15 +242,13 @@ __global__ __launch_bounds__(64) void wfa_reg_kernel(KArgs a) {
                     Dr[c] = D;
                     // Chunk flags as one scalar word pinned to an SGPR, so the
                     // compiler carries no lane booleans across the chunk
-                    // branches (two vector ops per flag).  No field starts at
-                    // bit 0: a select of 1 : 0 becomes a zero-extension, which
+                    // branches (two vector ops per flag).  The fields sit one
+                    // bit up: a select of 1 : 0 becomes a zero-extension, which
                     // is also lowered through a VGPR.
-                    const uint64_t bM = vote(M >= 0), bI = vote(I >= 0), bD = vote(D >= 0);
-                    uint64_t fl = (bM ? 1ull << kFm : 0ull) | ((bM & 1ull) << kFmLo) | ((bM >> 63) << kFmHi) |
-                                  (bI ? 1ull << kFi : 0ull) | ((bI >> 63) << kFiHi) |
-                                  (bD ? 1ull << kFd : 0ull) | ((bD & 1ull) << kFdLo);
+                    const uint64_t b = vote(max(max(M, I), D) >= 0);
+                    uint32_t fl = (b ? 2u : 0u) | ((uint32_t)(b & 1ull) << 9) | ((uint32_t)(b >> 63) << 17);
                     asm volatile("" : "+s"(fl));
-                    nn |= fl << c;
+                    acc |= fl << c;
                     if (c == ctp) mend = __builtin_amdgcn_readlane(M, ltp);
                     // I and D are out of bounds iff >= mt1 (D <= t by construction)
                     oob |= vote(max(I, D) >= mt1);
@@ -261,13 +256,11 @@ __global__ __launch_bounds__(64) void wfa_reg_kernel(KArgs a) {
                     carryI = NUL;  // an idle chunk holds only NULL
                 }
             }
-            uint32_t nnM = (uint32_t)(nn >> kFm) & 0xffffffu;
-            uint32_t nnI = ((uint32_t)(nn >> kFi) & 0xffu) | (((uint32_t)(nn >> kFiHi) & 0xffu) << 16);
-            uint32_t nnD = (uint32_t)(nn >> kFd) & 0xffffu;
+            uint32_t Fs = acc >> 1;
             if (oob) {
                 // exact trim: NULL I/D cells outside [first, last] in-bounds
-                // cell, then rebuild the slot mask (all three fields)
-                auto trim = [&](int32_t (&R)[C], uint32_t& msk) __attribute__((always_inline)) {
+                // cell, then rebuild the step mask
+                auto trim = [&](int32_t (&R)[C]) __attribute__((always_inline)) {
                     int f = 0x3fffffff, l = -1;
 #pragma unroll
                     for (int c = 0; c < C; ++c) {
@@ -279,29 +272,28 @@ __global__ __launch_bounds__(64) void wfa_reg_kernel(KArgs a) {
                             l = max(l, 64 * c + 63 - (int)__builtin_clzll(inb));
                         }
                     }
-                    uint32_t m2 = 0;  // fields shifted up by one bit (none at bit 0)
 #pragma unroll
                     for (int c = 0; c < C; ++c) {
                         const int w = 64 * c + lane;
                         if (w < f || w > l) R[c] = NUL;
-                        const uint64_t b = vote(R[c] >= 0);
-                        m2 |= ((b ? 2u : 0u) | ((uint32_t)(b & 1ull) << 9) | ((uint32_t)(b >> 63) << 17)) << c;
                     }
-                    msk = m2 >> 1;
                 };
-                trim(Ir, nnI);
-                trim(Dr, nnD);
+                trim(Ir);
+                trim(Dr);
+                uint32_t m2 = 0;  // fields shifted up by one bit, as in acc
+#pragma unroll
+                for (int c = 0; c < C; ++c) {
+                    const uint64_t b = vote(max(max(Mr[PH][c], Ir[c]), Dr[c]) >= 0);
+                    m2 |= ((b ? 2u : 0u) | ((uint32_t)(b & 1ull) << 9) | ((uint32_t)(b >> 63) << 17)) << c;
+                }
+                Fs = m2 >> 1;
             }
-            amM[PH] = nnM;
-            amI = nnI;
-            amD = nnD;
+            F[PH] = Fs;
             if (mend >= t) return s;
             // A non-NULL cell at a window edge could extend the range beyond
-            // it.  Lane 0 of chunk 0 can hold M or D (I there reads k-1, which
-            // is outside and NULL); lane 63 of chunk C-1 can hold M or I.
-            const uint32_t edges = nnM | nnI | nnD;
-            if ((edges >> 8) & 1u) return TSU_EKOVERFLOW;
-            if ((edges >> (16 + C - 1)) & 1u) return TSU_EKOVERFLOW;
+            // it (lane 0 of chunk 0, lane 63 of chunk C-1)
+            if ((Fs >> 8) & 1u) return TSU_EKOVERFLOW;
+            if ((Fs >> (16 + C - 1)) & 1u) return TSU_EKOVERFLOW;
             return 0;
         };
 
