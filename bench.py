@@ -12,9 +12,19 @@ recorded on the stream it runs on; the roofline figure divides the
 algorithmic HBM bytes of a launch (2-bit sequences read once + 4-byte score
 written: 8*(ceil(p/32)+ceil(t/32))+4 per pair) by that duration.
 
+The path's binding roof is integer VALU issue, not HBM (the sequences are
+read once, the wavefronts stay on chip).  "roofline_valu" prices the work
+with a fixed op model -- 11 int32 lane-ops per cell-triple (one I, D and M
+update) and 16 per 32-base extend compare, the counts taken from the oracle
+on the CPU sample -- against the chip's int32 VALU peak (256 CUs x 64 lanes
+x 2.4 GHz).
+
 The CPU baseline is the oracle's exact WFA (C, OpenMP over pairs, every host
 thread), timed on the first pairs of the same workload, and its scores double
 as a bit-exactness check of the timed GPU run.
+
+The batched form runs the autotuner on a sample first (untimed); with
+--compare-plain it also times the plain form on the same pairs.
 
     python bench.py [--steps K] [--warmup W] [--form plain|batched|cigar|served]
 """
@@ -31,6 +41,8 @@ sys.path.insert(0, REPO)
 
 SEED = 0x5453554E
 HBM_PEAK_GBS = 8000.0
+VALU_PEAK_LANE_OPS = 256 * 64 * 2.4e9   # int32 lane-ops/s: CUs x lanes per clock x peak engine clock
+OPS_PER_CELL, OPS_PER_EXTEND = 11, 16   # op model: per cell-triple, per 32-base extend compare
 PEN = (4, 6, 2)
 METRIC = "alignments/sec + equiv. GCUPS at 1 GPU, bit-exact; p50/p99 latency when served"
 
@@ -58,7 +70,7 @@ def cpu_baseline(pairs_fn, n_total, target_cpu_s=20.0):
     n = min(n_total, 2000)
     t0 = time.perf_counter()
     sample = pairs_fn(0, n)
-    scores, _, _ = oracle.wfa_batch(sample, *PEN, threads=threads)
+    scores, _, cnt = oracle.wfa_batch(sample, *PEN, threads=threads)
     dt = time.perf_counter() - t0
     # grow the sample to about target_cpu_s of CPU work (threads x wall)
     per_pair = dt * threads / max(n, 1)
@@ -71,7 +83,7 @@ def cpu_baseline(pairs_fn, n_total, target_cpu_s=20.0):
         n = n2
     return {"value": n / dt, "unit": "alignments/s", "cores": threads, "kind": "port",
             "sample": f"first {n} pairs of the workload, oracle exact WFA (C, OpenMP, {threads} threads), "
-                      f"{dt:.2f} s wall"}, scores
+                      f"{dt:.2f} s wall"}, scores, cnt
 
 
 def read_traffic(name):
@@ -154,6 +166,9 @@ def main():
     ap.add_argument("--workload", default="cfg2", choices=sorted(WORKLOADS))
     ap.add_argument("--pairs", type=int, default=0, help="override the pair count (testing only)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-autotune", action="store_true", help="batched: use the default shapes")
+    ap.add_argument("--tune-sample", type=int, default=1024, help="batched: autotuner sample per bucket")
+    ap.add_argument("--compare-plain", action="store_true", help="batched: also time the plain form")
     ap.add_argument("--requests", type=int, default=300, help="served: requests per rate")
     ap.add_argument("--serve-budget", type=int, default=16384, help="served: HBM budget in MiB")
     ap.add_argument("--inflight", type=int, default=4, help="served: launches in flight")
@@ -199,6 +214,12 @@ def main():
     b = a.batch(ds)
     log(f"batch resident: {b.nbytes / 1e6:.1f} MB")
 
+    tune = None
+    if args.form == "batched" and not args.no_autotune:
+        t0 = time.perf_counter()
+        tune = a.autotune(b, args.tune_sample)
+        log(f"autotune ({time.perf_counter() - t0:.1f} s): {tune}")
+
     def step():
         if args.form == "plain":
             return a.align_plain(b, W)
@@ -227,13 +248,34 @@ def main():
     achieved = algo_bytes / (kernel_ms / 1e3) / 1e9
     traffic = read_traffic(f"wfa_{args.form}_{args.workload}")
 
+    plain = None
+    if args.form == "batched" and args.compare_plain:
+        log(f"plain form on the same pairs (W={W})")
+        t0 = time.perf_counter()
+        ps, pt = a.align_plain(b, W)
+        a.sync()
+        pwall = time.perf_counter() - t0
+        plain = {"alignments_per_s": n / pwall, "kernel_ms": pt["kernel_ms"], "wf_length": W,
+                 "batched_speedup": (n * args.steps / wall) / (n / pwall),
+                 "same_scores": bool((ps == scores).all())}
+        log(f"plain: {plain}")
+
     cpu = None
     exact = None
+    valu = None
     if not args.no_cpu:
         log("cpu baseline")
-        cpu, cpu_scores = cpu_baseline(lambda s, m: [ds.pair(i) for i in range(s, s + m)], n)
+        cpu, cpu_scores, cnt = cpu_baseline(lambda s, m: [ds.pair(i) for i in range(s, s + m)], n)
         m = len(cpu_scores)
         exact = bool((scores[:m] == cpu_scores).all())
+        if args.form != "cigar":
+            ops_per_pair = (OPS_PER_CELL * cnt["cells"] + OPS_PER_EXTEND * cnt["extend_words"]) / m
+            ach = ops_per_pair * n / (kernel_ms / 1e3)
+            valu = {"bound": "valu", "achieved": ach / 1e12, "peak": VALU_PEAK_LANE_OPS / 1e12,
+                    "unit": "T int32 lane-ops/s", "frac": ach / VALU_PEAK_LANE_OPS,
+                    "ops_per_pair": ops_per_pair,
+                    "model": f"{OPS_PER_CELL}/cell-triple + {OPS_PER_EXTEND}/extend compare, oracle counts "
+                             f"over the CPU sample ({m} pairs)"}
 
     out = {
         "metric": METRIC,
@@ -260,9 +302,14 @@ def main():
         "bit_exact_sample": exact,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic},
+        "roofline_valu": valu,
         "cpu_baseline": cpu,
         "device": info["name"],
     }
+    if tune is not None:
+        out["tune"] = tune
+    if plain is not None:
+        out["plain_same_pairs"] = plain
     print(json.dumps(out), flush=True)
 
 
