@@ -226,9 +226,9 @@ def test_wfa_gpu_cli_check_mode(repo, tsu, tmp_path):
 
 
 def test_register_and_lds_kernels_agree(aligner, tsu, oracle, fixtures):
-    """4/6/2 pairs run through the register-resident kernel (capacity <= 512)
-    and the LDS kernel (capacity > 512) give the oracle's scores; narrower
-    register windows either agree or report overflow."""
+    """4/6/2 pairs run through the single-wave register kernel (capacity
+    <= 512) and the multi-wave one (capacity > 512) give the oracle's scores;
+    narrower register windows either agree or report overflow."""
     cases = [c for c in fixtures if tuple(c["penalties"]) == (4, 6, 2)]
     pairs = [(c["pattern"], c["text"]) for c in cases]
     pairs += tsu.Dataset.generate(400, seed=SEED + 11, len_min=1, len_max=300, err_min=0.0, err_max=0.3).pairs()
@@ -246,6 +246,31 @@ def test_register_and_lds_kernels_agree(aligner, tsu, oracle, fixtures):
         assert ok.sum() > 0
         assert (got[ok] == want[ok]).all(), W
         assert (got[~ok] == tsu.TSU_EKOVERFLOW).all()
+
+
+def test_multiwave_register_kernel(aligner, tsu, oracle):
+    """Capacities of 513..8192 diagonals run on the multi-wave register kernel
+    (2-16 waves, chunks dealt round-robin, edges exchanged through LDS).
+    Unequal lengths put I/D cells out of bounds at both sequence ends (the
+    trim across waves); narrow windows report overflow, wide ones never."""
+    ds = tsu.Dataset.generate(300, seed=SEED + 14, len_min=200, len_max=4000, err_min=0.0, err_max=0.2)
+    pairs = ds.pairs()
+    rng = np.random.default_rng(14)
+    for _ in range(30):
+        pairs.append(("".join(rng.choice(list("ACGT"), int(rng.integers(50, 1500)))),
+                      "".join(rng.choice(list("ACGT"), int(rng.integers(50, 1500))))))
+    pairs.append(("ACGT" * 900, "ACGT" * 900))
+    pairs.append(("A" * 100, "C" * 3000))
+    want, _, _ = oracle.wfa_batch(pairs, 4, 6, 2)
+    b = aligner.batch(pairs)
+    for W in (600, 1100, 2100, 4100, 8192):
+        say(f"multi-wave W={W}")
+        got, _ = aligner.align_plain(b, W, raise_on_error=False)
+        ok = got != tsu.TSU_EKOVERFLOW
+        assert (got[ok] == want[ok]).all(), W
+        assert ok.sum() > len(pairs) // 3, W
+        if W == 8192:  # the window covers every diagonal of every pair
+            assert ok.all()
 
 
 def test_autotune_then_batched(tsu, oracle):

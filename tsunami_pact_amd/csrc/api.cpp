@@ -318,11 +318,29 @@ int tsu_host::reg_chunks(const tsu_ctx* ctx, int32_t W) {
     return c;
 }
 
+int tsu_host::reg_waves(const tsu_ctx* ctx, int32_t W) {
+    if (!tsu_host::fixed_242(ctx) || W <= 512 || W > 16 * 512) return 0;
+    int nw = 2;
+    while (512 * nw < W) nw *= 2;
+    return nw;
+}
+
+// Register kernels when the penalties and the capacity allow: one wave for
+// up to 512 diagonals, 2-16 waves of 8 chunks for up to 8192.
+static void use_register_kernel(const tsu_ctx* ctx, int32_t W, Shape& sh) {
+    if (int c = tsu_host::reg_chunks(ctx, W)) {
+        sh.reg_c = c;
+        sh.nw = 1;
+    } else if (int nw = tsu_host::reg_waves(ctx, W)) {
+        sh.reg_c = 8;
+        sh.nw = nw;
+    }
+}
+
 static Shape plain_shape(const tsu_ctx* ctx, int32_t W, int32_t max_len, bool* fits) {
     Shape sh{default_waves(W), max_len <= 32000, true, false, false};
     sh.fixed_242 = tsu_host::fixed_242(ctx);
-    sh.reg_c = tsu_host::reg_chunks(ctx, W);
-    if (sh.reg_c) sh.nw = 1;
+    use_register_kernel(ctx, W, sh);
     KArgs a = base_args(ctx);
     a.W = W;
     a.seq_words = seq_words_for(max_len);
@@ -436,8 +454,11 @@ int32_t tsu_host::initial_capacity(const tsu_ctx* ctx, int32_t bound) {
 Shape tsu_host::batched_shape(const tsu_ctx* ctx, int32_t W, int32_t max_len, int nw_hint) {
     Shape sh{nw_hint > 0 ? nw_hint : default_waves(W), max_len <= 32000, true, false, true};
     sh.fixed_242 = tsu_host::fixed_242(ctx);
-    sh.reg_c = nw_hint > 1 ? 0 : tsu_host::reg_chunks(ctx, W);
-    if (sh.reg_c) { sh.nw = 1; return sh; }
+    // a tuned hint above 1 asks for the LDS kernel with that many waves
+    if (nw_hint <= 1) {
+        use_register_kernel(ctx, W, sh);
+        if (sh.reg_c) return sh;
+    }
     KArgs a = base_args(ctx);
     a.W = W;
     a.seq_words = seq_words_for(max_len);

@@ -72,5 +72,7 @@ int fail(tsu_ctx* ctx, int status, const std::string& msg);
 bool fixed_242(const tsu_ctx* ctx);
 // Chunks of 64 diagonals of the register-resident kernel for capacity W (0: LDS kernel).
 int reg_chunks(const tsu_ctx* ctx, int32_t W);
+// Waves of the multi-wave register kernel for capacity W (0: not eligible).
+int reg_waves(const tsu_ctx* ctx, int32_t W);
 
 }  // namespace tsu_host

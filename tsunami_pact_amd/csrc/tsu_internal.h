@@ -66,7 +66,7 @@ struct Shape {
     bool bt;         // record backtrace + emit CIGAR
     bool queue;      // persistent launch fed by a work counter
     bool fixed_242 = false;  // scaled penalties are (2, 3, 1): compile-time ring
-    int reg_c = 0;           // > 0: register-resident kernel over reg_c*64 diagonals
+    int reg_c = 0;           // > 0: register-resident kernel, reg_c chunks of 64 diagonals per wave (nw waves)
 };
 
 // LDS bytes a launch of this shape needs.

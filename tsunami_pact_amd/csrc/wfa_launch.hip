@@ -13,6 +13,8 @@ KernelFn pick_nw4(const Shape& sh);
 KernelFn pick_nw8(const Shape& sh);
 KernelFn pick_nw16(const Shape& sh);
 KernelFn pick_reg(const Shape& sh);
+KernelFn pick_regmw(const Shape& sh);
+int regmw_exchange_words(int nw);
 
 // Plain streaming copy (HBM byte-counter calibration).
 __global__ __launch_bounds__(256) void copy_kernel(const uint4* __restrict__ src, uint4* __restrict__ dst, int64_t n) {
@@ -32,14 +34,17 @@ int64_t ring_bytes(const Shape& sh, int32_t W, int32_t rm, int32_t ri) {
 }
 
 int64_t lds_bytes(const Shape& sh, const KArgs& a) {
-    if (sh.reg_c > 0) return 4ll * (2 * a.seq_words + 16);  // sequences + misc only
+    if (sh.reg_c > 0) {
+        // sequences + misc (one wave) or + the edge exchange area (several)
+        return 4ll * (2 * a.seq_words + (sh.nw > 1 ? regmw_exchange_words(sh.nw) : 16));
+    }
     int64_t b = lds_head_bytes(a.seq_words, a.rm + 2 * a.ri, sh.nw);
     if (sh.lds_ring) b += ring_bytes(sh, a.W, a.rm, a.ri);
     return b;
 }
 
 static KernelFn pick(const Shape& sh) {
-    if (sh.reg_c > 0) return pick_reg(sh);
+    if (sh.reg_c > 0) return sh.nw > 1 ? pick_regmw(sh) : pick_reg(sh);
     switch (sh.nw) {
     case 1: return pick_nw1(sh);
     case 2: return pick_nw2(sh);
