@@ -144,6 +144,11 @@ void tsu_dataset_free(tsu_dataset* ds);
 
 /* ---- device-resident batches (wfa_gpu.cu:96-111: cudaMalloc + H2D) ------- */
 
+/* Both sequences of a pair are staged in LDS while it is aligned, so a
+ * sequence may hold at most about 320,000 bases (the exact bound follows from
+ * the device's LDS size); batches (and server requests) with a longer one
+ * are refused with TSU_EINVAL. */
+
 /* Packs pairs [first, first+count) of ds and copies them to HBM. */
 int tsu_batch_from_dataset(tsu_ctx* ctx, const tsu_dataset* ds, int64_t first, int64_t count,
                            tsu_batch** out);
@@ -223,6 +228,11 @@ int tsu_serve_open(tsu_ctx* ctx, const tsu_serve_cfg* cfg, tsu_server** out);
  * with TSU_EBUDGET if the request alone exceeds the budget. */
 int tsu_serve_submit(tsu_server* s, const char* const* patterns, const int32_t* p_len,
                      const char* const* texts, const int32_t* t_len, int32_t n, uint64_t* ticket);
+/* The same for pairs already packed (tsu_pack layout): pairs[i] index
+ * words[0, n_words), which are copied.  No reference counterpart (the
+ * reference has no server); for clients that keep sequences packed. */
+int tsu_serve_submit_packed(tsu_server* s, const uint32_t* words, int64_t n_words, const tsu_pair* pairs,
+                            int32_t n, uint64_t* ticket);
 /* Non-blocking: TSU_EAGAIN until done; then copies n scores and forgets the
  * ticket. */
 int tsu_serve_poll(tsu_server* s, uint64_t ticket, int32_t* scores, tsu_req_stats* st);

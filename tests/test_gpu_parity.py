@@ -143,6 +143,16 @@ def test_int32_offsets_long_pair(aligner, tsu, oracle):
     assert (got == want).all()
 
 
+def test_sequence_longer_than_lds_is_refused(aligner, tsu):
+    """Both sequences of a pair are staged in LDS; longer ones are refused
+    when the batch is made, with TSU_EINVAL and a message."""
+    huge = tsu.Dataset.generate(1, seed=5, len_min=400_000, err_min=0.0)
+    say("refuse 400 kbp")
+    with pytest.raises(tsu.TsuError) as ei:
+        aligner.batch(huge)
+    assert ei.value.status == tsu.TSU_EINVAL and "LDS" in str(ei.value)
+
+
 def test_fixed_layout_entry_point(aligner, tsu, oracle):
     """tsu_align_fixed takes the reference's fixed-slot char arrays."""
     ds = tsu.Dataset.generate(500, seed=SEED + 8, len_min=128, err_min=0.02, edit_mode=1)

@@ -76,10 +76,16 @@ def test_serve_poll_budget_and_empty(aligner, tsu):
         if st != tsu.TSU_EAGAIN:
             break
     assert st == tsu.TSU_OK and len(sc) == 8
-    big = tsu.Dataset.generate(1, seed=4, len_min=200000, err_min=0.0).pairs()
+    # 600 pairs of 2 x 8 kbp need 2.4 MB packed: more than the whole budget
+    big = tsu.Dataset.generate(600, seed=4, len_min=8000, err_min=0.0).pairs()
     with pytest.raises(tsu.TsuError) as ei:
         srv.submit(big)
     assert ei.value.status == tsu.TSU_EBUDGET
+    # a sequence longer than the kernels stage in LDS is refused as invalid
+    huge = tsu.Dataset.generate(1, seed=4, len_min=400_000, err_min=0.0).pairs()
+    with pytest.raises(tsu.TsuError) as ei:
+        srv.submit(huge)
+    assert ei.value.status == tsu.TSU_EINVAL
     assert srv.close() == tsu.TSU_OK
 
 
@@ -94,4 +100,24 @@ def test_serve_overflow_pairs_are_widened(aligner, tsu, oracle):
     st, sc, _ = srv.wait(t, len(pairs))
     want, _, _ = oracle.wfa_batch(pairs, 4, 6, 2)
     assert st == tsu.TSU_OK and (sc == want).all()
+    assert srv.close() == tsu.TSU_OK
+
+
+def test_serve_submit_packed(aligner, tsu, oracle):
+    """Pre-packed requests give the same scores; pairs outside the given
+    words are refused."""
+    ds = tsu.Dataset.generate(40, seed=21, len_min=100, len_max=3000, err_min=0.01, err_max=0.15)
+    words, pairs = ds.packed()
+    want, _, _ = oracle.wfa_batch(ds.pairs(), 4, 6, 2)
+    say("serve packed")
+    srv = tsu.Server(aligner, hbm_budget_bytes=256 << 20, max_inflight=2, max_batch_pairs=64)
+    t1 = srv.submit_packed(words, pairs[:25])
+    t2 = srv.submit_packed(words, pairs[25:])
+    st1, sc1, _ = srv.wait(t1, 25)
+    st2, sc2, _ = srv.wait(t2, 15)
+    assert st1 == tsu.TSU_OK and st2 == tsu.TSU_OK
+    assert (np.concatenate([sc1, sc2]) == want).all()
+    with pytest.raises(tsu.TsuError) as ei:
+        srv.submit_packed(words[:10], pairs[30:31])
+    assert ei.value.status == tsu.TSU_EINVAL
     assert srv.close() == tsu.TSU_OK

@@ -45,7 +45,8 @@ EXPORTS = (
     "tsu_batch_from_dataset", "tsu_batch_create", "tsu_batch_size", "tsu_batch_bytes", "tsu_batch_free",
     "tsu_align_plain", "tsu_align_fixed", "tsu_align_batched", "tsu_autotune", "tsu_tune_table",
     "tsu_align_cigar", "tsu_cigars_get", "tsu_cigars_string", "tsu_cigars_free",
-    "tsu_serve_open", "tsu_serve_submit", "tsu_serve_poll", "tsu_serve_wait", "tsu_serve_close",
+    "tsu_serve_open", "tsu_serve_submit", "tsu_serve_submit_packed", "tsu_serve_poll", "tsu_serve_wait",
+    "tsu_serve_close",
     "tsu_calibrate_copy", "tsu_sync",
 )
 
@@ -59,6 +60,10 @@ class TsuError(RuntimeError):
 class Pair(ctypes.Structure):
     _fields_ = [("p_off", ctypes.c_int64), ("t_off", ctypes.c_int64),
                 ("p_len", ctypes.c_int32), ("t_len", ctypes.c_int32)]
+
+
+# numpy layout of tsu_pair (24 bytes, naturally aligned)
+PAIR_DTYPE = np.dtype([("p_off", np.int64), ("t_off", np.int64), ("p_len", np.int32), ("t_len", np.int32)])
 
 
 class Timing(ctypes.Structure):
@@ -139,6 +144,7 @@ def library() -> ctypes.CDLL:
         "tsu_cigars_free": (None, [vp]),
         "tsu_serve_open": (ctypes.c_int, [vp, P(ServeCfg), P(vp)]),
         "tsu_serve_submit": (ctypes.c_int, [vp, P(ctypes.c_char_p), P(i32), P(ctypes.c_char_p), P(i32), i32, P(ctypes.c_uint64)]),
+        "tsu_serve_submit_packed": (ctypes.c_int, [vp, P(ctypes.c_uint32), i64, P(Pair), i32, P(ctypes.c_uint64)]),
         "tsu_serve_poll": (ctypes.c_int, [vp, ctypes.c_uint64, P(i32), P(ReqStats)]),
         "tsu_serve_wait": (ctypes.c_int, [vp, ctypes.c_uint64, P(i32), P(ReqStats)]),
         "tsu_serve_close": (ctypes.c_int, [vp]),
@@ -241,6 +247,25 @@ class Dataset:
         if r != TSU_OK:
             raise TsuError(r, "tsu_dataset_lengths")
         return pl[:n], tl[:n]
+
+    def packed(self) -> Tuple[np.ndarray, np.ndarray]:
+        """(words, pairs): every pair packed with tsu_pack into one arena,
+        pattern then text, in order; pairs is a PAIR_DTYPE array."""
+        n = len(self)
+        pl, tl = self.lengths()
+        nwp = (pl.astype(np.int64) + 15) // 16
+        nwt = (tl.astype(np.int64) + 15) // 16
+        pairs = np.zeros(n, PAIR_DTYPE)
+        ends = np.cumsum(nwp + nwt)
+        pairs["p_off"] = ends - (nwp + nwt)
+        pairs["t_off"] = pairs["p_off"] + nwp
+        pairs["p_len"], pairs["t_len"] = pl, tl
+        words = np.zeros(int(ends[-1]) if n else 1, np.uint32)
+        for i in range(n):
+            p, t = self.pair(i)
+            words[pairs["p_off"][i]:pairs["p_off"][i] + nwp[i]] = pack(p)[:nwp[i]]
+            words[pairs["t_off"][i]:pairs["t_off"][i] + nwt[i]] = pack(t)[:nwt[i]]
+        return words, pairs
 
     def close(self) -> None:
         if self._h:
@@ -452,6 +477,19 @@ class Server:
         r = lib.tsu_serve_submit(self._h, parr, _i32p(pl), tarr, _i32p(tl), count, ctypes.byref(ticket))
         if r != TSU_OK:
             raise TsuError(r, f"tsu_serve_submit: {strerror(r)}")
+        return ticket.value
+
+    def submit_packed(self, words: np.ndarray, pairs: np.ndarray) -> int:
+        """Submits pairs already packed: `pairs` is a structured array with the
+        fields of tsu_pair (p_off, t_off, p_len, t_len) indexing `words`."""
+        words = np.ascontiguousarray(words, dtype=np.uint32)
+        pairs = np.ascontiguousarray(pairs, dtype=PAIR_DTYPE)
+        ticket = ctypes.c_uint64()
+        r = self._lib.tsu_serve_submit_packed(self._h, words.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)),
+                                              len(words), pairs.ctypes.data_as(ctypes.POINTER(Pair)), len(pairs),
+                                              ctypes.byref(ticket))
+        if r != TSU_OK:
+            raise TsuError(r, f"tsu_serve_submit_packed: {strerror(r)}")
         return ticket.value
 
     def _result(self, fn, ticket: int, n: int):

@@ -159,8 +159,10 @@ static int batch_upload(tsu_ctx* ctx, const uint32_t* words, int64_t n_words, co
         if (p.p_len < 0 || p.t_len < 0 || p.p_off < 0 || p.t_off < 0 ||
             p.p_off + tsu_packed_words(p.p_len) > n_words || p.t_off + tsu_packed_words(p.t_len) > n_words)
             return fail(ctx, TSU_EINVAL, "pair " + std::to_string(i) + " lies outside the packed arena");
-        if (p.p_len > (1 << 26) || p.t_len > (1 << 26))
-            return fail(ctx, TSU_EINVAL, "pair " + std::to_string(i) + " longer than 2^26 bases");
+        if (p.p_len > tsu_host::max_seq_len(ctx) || p.t_len > tsu_host::max_seq_len(ctx))
+            return fail(ctx, TSU_EINVAL, "pair " + std::to_string(i) + ": a sequence is longer than " +
+                                             std::to_string(tsu_host::max_seq_len(ctx)) +
+                                             " bases (both sequences of a pair are staged in LDS)");
     }
     tsu_batch* b = new (std::nothrow) tsu_batch();
     if (!b) return fail(ctx, TSU_ENOMEM, "host allocation");
@@ -236,6 +238,13 @@ KArgs tsu_host::base_args(const tsu_ctx* ctx) {
 }
 
 int32_t tsu_host::seq_words_for(int32_t max_len) { return (int32_t)((max_len + 15) / 16 + 2); }
+
+// Longest sequence the kernels take: both sequences of a pair are staged in
+// LDS, next to the LDS kernel's slot limits and misc words.
+int32_t tsu_host::max_seq_len(const tsu_ctx* ctx) {
+    const int64_t words = ctx->max_lds / 4 - 2 * (ctx->rm + 2 * ctx->ri + 1) - 16 - 4;
+    return (int32_t)std::max<int64_t>(0, (words / 2 - 2) * 16);
+}
 
 using tsu_host::base_args;
 using tsu_host::seq_words_for;
@@ -326,21 +335,29 @@ int tsu_host::reg_waves(const tsu_ctx* ctx, int32_t W) {
 }
 
 // Register kernels when the penalties and the capacity allow: one wave for
-// up to 512 diagonals, 2-16 waves of 8 chunks for up to 8192.
-static void use_register_kernel(const tsu_ctx* ctx, int32_t W, Shape& sh) {
+// up to 512 diagonals, 2-16 waves of 8 chunks for up to 8192 (and the two
+// sequences fit in LDS).
+static void use_register_kernel(const tsu_ctx* ctx, int32_t W, int32_t max_len, Shape& sh) {
+    Shape r = sh;
     if (int c = tsu_host::reg_chunks(ctx, W)) {
-        sh.reg_c = c;
-        sh.nw = 1;
+        r.reg_c = c;
+        r.nw = 1;
     } else if (int nw = tsu_host::reg_waves(ctx, W)) {
-        sh.reg_c = 8;
-        sh.nw = nw;
+        r.reg_c = 8;
+        r.nw = nw;
+    } else {
+        return;
     }
+    KArgs a = base_args(ctx);
+    a.W = W;
+    a.seq_words = seq_words_for(max_len);
+    if (tsu::lds_bytes(r, a) <= ctx->max_lds) sh = r;
 }
 
 static Shape plain_shape(const tsu_ctx* ctx, int32_t W, int32_t max_len, bool* fits) {
     Shape sh{default_waves(W), max_len <= 32000, true, false, false};
     sh.fixed_242 = tsu_host::fixed_242(ctx);
-    use_register_kernel(ctx, W, sh);
+    use_register_kernel(ctx, W, max_len, sh);
     KArgs a = base_args(ctx);
     a.W = W;
     a.seq_words = seq_words_for(max_len);
@@ -442,12 +459,16 @@ int tsu_host::bucket_of(int32_t len) {
 
 // Initial capacity for a bucket: enough for about 15% divergence at the
 // bucket's length (score ~ 0.15 * L * max(x, o+e)), clamped to what the
-// lengths allow.  Pairs that overflow are re-run wider.
+// lengths allow.  With the register kernels available (4/6/2) it is also
+// capped at their widest window, 8192 diagonals: most pairs fit, and the
+// few that overflow are re-run wider on the LDS kernel, which costs less
+// than running every long pair there.  Pairs that overflow are re-run wider.
 int32_t tsu_host::initial_capacity(const tsu_ctx* ctx, int32_t bound) {
     const double per_base = 0.15 * (double)std::max(ctx->xs, ctx->os + ctx->es);
     const int64_t s_est = (int64_t)std::ceil(per_base * bound) + ctx->os + ctx->es;
     int32_t W = capacity_for_score(ctx, s_est);
     W = std::min<int32_t>(W, capacity_full(bound, bound));
+    if (fixed_242(ctx)) W = std::min<int32_t>(W, 16 * 512);
     return std::max<int32_t>(W, 16);
 }
 
@@ -456,7 +477,7 @@ Shape tsu_host::batched_shape(const tsu_ctx* ctx, int32_t W, int32_t max_len, in
     sh.fixed_242 = tsu_host::fixed_242(ctx);
     // a tuned hint above 1 asks for the LDS kernel with that many waves
     if (nw_hint <= 1) {
-        use_register_kernel(ctx, W, sh);
+        use_register_kernel(ctx, W, max_len, sh);
         if (sh.reg_c) return sh;
     }
     KArgs a = base_args(ctx);

@@ -1,18 +1,24 @@
 // serve.cpp -- continuous-batching alignment server on one GPU.
 //
-// Requests (1..N pairs each) are packed on the submitting thread and queued.
-// One worker thread owns the device side:
-//   - device memory is one pre-allocated region of hbm_budget_bytes, carved
-//     by a first-fit allocator; a request is admitted only when its bytes
-//     (packed sequences, descriptors, scores, order, ring spill) fit;
-//   - admitted requests are merged (FIFO, up to max_batch_pairs pairs) into a
-//     flight: one stream, pairs grouped by length bucket, longest first, one
-//     persistent queue-fed launch per bucket, scores copied back async;
-//   - up to max_inflight flights run at once on separate streams, so requests
+// Requests (1..N pairs each) are packed on the submitting thread; their pairs
+// join one FIFO queue per length bucket.  One worker thread owns the device
+// side:
+//   - a launch slot that frees up takes the bucket whose oldest waiting pair
+//     is the oldest, and merges up to max_batch_pairs waiting pairs of that
+//     bucket from any number of requests into one flight: their sequences
+//     are copied into one arena, one persistent queue-fed launch aligns them,
+//     the scores come back async;
+//   - device memory is one pre-allocated region of hbm_budget_bytes carved
+//     by a first-fit allocator; a flight takes fewer pairs (or waits) when
+//     its bytes (sequences, descriptors, scores, order, ring spill) do not
+//     fit what is free;
+//   - up to max_inflight flights run at once on separate streams, so pairs
 //     that arrive while others are on the GPU are launched as soon as a slot
-//     and memory free up and their work overlaps on the device;
-//   - a finished flight releases its memory; pairs whose diagonal range
-//     overflowed are re-launched wider on the same stream first.
+//     and memory free up, and launches of different buckets overlap;
+//   - a finished flight scatters its scores to the requests (a request is
+//     done when its last pair is) and releases its memory; pairs whose
+//     diagonal range overflowed are re-launched wider on the same stream
+//     first.
 // Each pair's result depends only on that pair, so a request's scores are the
 // same alone and in any mix.
 #include <algorithm>
@@ -77,9 +83,16 @@ struct Req {
     std::vector<tsu_pair> pairs;
     std::vector<int32_t> scores;
     int32_t max_len = 0;
+    int32_t remaining = 0;  // pairs without a score yet
     int64_t submit_ns = 0, admit_ns = 0, done_ns = 0;
     int status = TSU_OK;
     bool done = false;
+};
+
+// One waiting (or in-flight) pair: pair i of request r.
+struct Entry {
+    Req* r;
+    int32_t i;
 };
 
 struct LaunchPlan {
@@ -94,8 +107,7 @@ struct Slot {
     hipStream_t stream = nullptr;
     hipEvent_t ev = nullptr;
     bool active = false;
-    std::vector<Req*> reqs;
-    std::vector<int32_t> req_first;  // first flight pair of each request
+    std::vector<Entry> ents;         // flight pair k is ents[k]
     int32_t n = 0;
     int64_t off = -1, bytes = 0;
     uint32_t* d_words = nullptr;
@@ -134,7 +146,8 @@ struct tsu_server {
     tsu_serve_cfg cfg{};
     std::mutex mu;
     std::condition_variable cv;
-    std::deque<Req*> pending;
+    std::vector<std::deque<Entry>> pending;  // waiting pairs per length bucket, FIFO
+    int64_t n_pending = 0;
     std::unordered_map<uint64_t, Req*> reqs;
     std::vector<Slot> slots;
     Pool pool;
@@ -181,6 +194,19 @@ int64_t plan_flight(tsu_server* s, Slot& sl, int* status) {
         lp.grid = std::min<int64_t>(tsu_host::resident_grid(ctx, lp.sh, a, status), lp.count);
         if (*status != 0 || lp.grid <= 0) { *status = TSU_EINTERNAL; return -1; }
         if (!lp.sh.lds_ring) ring = std::max(ring, tsu::ring_bytes(lp.sh, lp.W, ctx->rm, ctx->ri) * lp.grid);
+        // the first re-launch of pairs that overflow (retire: twice the
+        // capacity) may need a ring in HBM: reserve it with the flight
+        const int32_t W2 = std::min<int32_t>(2 * lp.W, tsu_host::capacity_full(lp.max_len, lp.max_len));
+        if (W2 > lp.W) {
+            const Shape sh2 = tsu_host::batched_shape(ctx, W2, lp.max_len, 0);
+            if (!sh2.lds_ring) {
+                KArgs a2 = a;
+                a2.W = W2;
+                const int64_t g2 = std::min<int64_t>(tsu_host::resident_grid(ctx, sh2, a2, status), lp.count);
+                if (*status != 0 || g2 <= 0) { *status = TSU_EINTERNAL; return -1; }
+                ring = std::max(ring, tsu::ring_bytes(sh2, W2, ctx->rm, ctx->ri) * g2);
+            }
+        }
         sl.order.insert(sl.order.end(), v.begin(), v.end());
         sl.plan.push_back(lp);
     }
@@ -215,68 +241,80 @@ int launch_plan(tsu_server* s, Slot& sl) {
     return TSU_OK;
 }
 
-// Builds and launches a flight in slot sl from the head of the queue.
-// Returns true when something was admitted.
+// Records the result (score or negative status) of pair i of request r; the
+// request is done when its last pair has one.
+void finish_entry(Req* r, int32_t i, int32_t v, int64_t now) {
+    r->scores[(size_t)i] = v;
+    if (v < 0 && r->status == TSU_OK) r->status = v;
+    if (--r->remaining == 0) {
+        r->done = true;
+        r->done_ns = now;
+    }
+}
+
+// A flight that could not be launched or did not complete: its pairs get
+// `status` and its memory is released.
+void fail_flight(tsu_server* s, Slot& sl, int status) {
+    const int64_t now = mono_ns();
+    for (const Entry& en : sl.ents) finish_entry(en.r, en.i, status, now);
+    sl.ents.clear();
+    if (sl.off >= 0) s->pool.release(sl.off, sl.bytes);
+    sl.off = -1;
+    sl.active = false;
+    s->cv.notify_all();
+}
+
+// Builds and launches a flight in slot sl: the bucket whose oldest waiting
+// pair is the oldest, up to max_batch_pairs of its waiting pairs, fewer when
+// the flight would not fit the free memory.  Returns true when something
+// was launched.
 bool admit(tsu_server* s, Slot& sl) {
-    sl.reqs.clear();
-    sl.req_first.clear();
+    sl.ents.clear();
     sl.fpairs.clear();
-    int64_t words = 0;
-    int32_t n = 0;
-    int64_t bytes = 0;
-    // FIFO: take requests while they fit the pair cap and the free memory
-    while (!s->pending.empty()) {
-        Req* r = s->pending.front();
-        const int32_t rn = (int32_t)r->pairs.size();
-        if (n > 0 && n + rn > s->cfg.max_batch_pairs) break;
-        // tentative flight with r added
-        const size_t keep_pairs = sl.fpairs.size();
-        for (const tsu_pair& pr : r->pairs) {
-            tsu_pair q = pr;
-            q.p_off += words;
-            q.t_off += words;
-            sl.fpairs.push_back(q);
+    int bsel = -1;
+    for (int b = 0; b < (int)s->pending.size(); ++b) {
+        if (s->pending[(size_t)b].empty()) continue;
+        if (bsel < 0 || s->pending[(size_t)b].front().r->submit_ns <= s->pending[(size_t)bsel].front().r->submit_ns) bsel = b;
+    }
+    if (bsel < 0) return false;
+    std::deque<Entry>& q = s->pending[(size_t)bsel];
+    int32_t take = (int32_t)std::min<int64_t>((int64_t)q.size(), s->cfg.max_batch_pairs);
+    int64_t words = 0, bytes = 0;
+    for (;;) {
+        // flight arena: each pair's pattern and text words, in order
+        sl.fpairs.resize((size_t)take);
+        words = 0;
+        for (int32_t k = 0; k < take; ++k) {
+            const Entry& en = q[(size_t)k];
+            const tsu_pair& pr = en.r->pairs[(size_t)en.i];
+            tsu_pair f = pr;
+            f.p_off = words;
+            words += tsu_packed_words(pr.p_len);
+            f.t_off = words;
+            words += tsu_packed_words(pr.t_len);
+            sl.fpairs[(size_t)k] = f;
         }
-        sl.n = n + rn;
+        sl.n = take;
         int st = 0;
-        const int64_t need = plan_flight(s, sl, &st);
-        if (st != 0 || need < 0 || need > s->pool.size) {
-            sl.fpairs.resize(keep_pairs);
-            if (n == 0) {  // cannot ever run: fail it
-                s->pending.pop_front();
-                r->status = st != 0 ? TSU_EINTERNAL : TSU_EBUDGET;
-                r->done = true;
-                r->done_ns = mono_ns();
-                s->cv.notify_all();
-                continue;
-            }
+        bytes = plan_flight(s, sl, &st);
+        if (st != 0 || bytes < 0) {
+            s->fatal = TSU_EINTERNAL;
+            return false;
+        }
+        const int64_t off = s->pool.alloc(bytes);
+        if (off >= 0) {
+            sl.off = off;
             break;
         }
-        // does it fit the memory free right now?
-        const int64_t off = s->pool.alloc(need);
-        if (off < 0) {
-            sl.fpairs.resize(keep_pairs);
-            break;
-        }
-        s->pool.release(off, need);
-        s->pending.pop_front();
-        sl.reqs.push_back(r);
-        sl.req_first.push_back(n);
-        words += (int64_t)r->words.size();
-        n += rn;
-        bytes = need;
+        if (take == 1) return false;  // waits for memory
+        take /= 2;
     }
-    if (sl.reqs.empty()) return false;
-    sl.n = n;
-    int st = 0;
-    bytes = plan_flight(s, sl, &st);
-    sl.off = s->pool.alloc(bytes);
-    if (st != 0 || sl.off < 0) {  // cannot happen: checked above; put the requests back
-        for (auto it = sl.reqs.rbegin(); it != sl.reqs.rend(); ++it) s->pending.push_front(*it);
-        sl.reqs.clear();
-        return false;
-    }
+    sl.ents.assign(q.begin(), q.begin() + take);
+    q.erase(q.begin(), q.begin() + take);
+    s->n_pending -= take;
+    const int32_t n = take;
     sl.bytes = bytes;
+    sl.active = false;
     uint8_t* p = s->pool.base + sl.off;
     auto carve = [&](int64_t b) { uint8_t* q = p; p += align256(b); return q; };
     sl.d_words = reinterpret_cast<uint32_t*>(carve(words * 4));
@@ -290,24 +328,34 @@ bool admit(tsu_server* s, Slot& sl) {
         !grow_pinned(sl.h_pairs, sl.h_pairs_cap, n) || !grow_pinned(sl.h_order, sl.h_order_cap, n) ||
         !grow_pinned(sl.h_scores, sl.h_scores_cap, n)) {
         s->fatal = TSU_ENOMEM;
+        fail_flight(s, sl, TSU_ENOMEM);
         return false;
     }
-    int64_t w = 0;
-    for (Req* r : sl.reqs) {
-        std::memcpy(sl.h_words + w, r->words.data(), r->words.size() * 4);
-        w += (int64_t)r->words.size();
+    for (int32_t k = 0; k < n; ++k) {
+        const Entry& en = sl.ents[(size_t)k];
+        const tsu_pair& pr = en.r->pairs[(size_t)en.i];
+        const tsu_pair& f = sl.fpairs[(size_t)k];
+        std::memcpy(sl.h_words + f.p_off, en.r->words.data() + pr.p_off, (size_t)tsu_packed_words(pr.p_len) * 4);
+        std::memcpy(sl.h_words + f.t_off, en.r->words.data() + pr.t_off, (size_t)tsu_packed_words(pr.t_len) * 4);
     }
     std::memcpy(sl.h_pairs, sl.fpairs.data(), (size_t)n * sizeof(tsu_pair));
     std::memcpy(sl.h_order, sl.order.data(), (size_t)n * 4);
     const int64_t now = mono_ns();
-    for (Req* r : sl.reqs) r->admit_ns = now;
+    for (const Entry& en : sl.ents)
+        if (en.r->admit_ns == 0) en.r->admit_ns = now;
     bool ok = hipMemcpyAsync(sl.d_words, sl.h_words, (size_t)words * 4, hipMemcpyHostToDevice, sl.stream) == hipSuccess &&
               hipMemcpyAsync(sl.d_pairs, sl.h_pairs, (size_t)n * sizeof(KPair), hipMemcpyHostToDevice, sl.stream) == hipSuccess &&
               hipMemcpyAsync(sl.d_order, sl.h_order, (size_t)n * 4, hipMemcpyHostToDevice, sl.stream) == hipSuccess;
     ok = ok && launch_plan(s, sl) == TSU_OK;
     ok = ok && hipMemcpyAsync(sl.h_scores, sl.d_scores, (size_t)n * 4, hipMemcpyDeviceToHost, sl.stream) == hipSuccess &&
          hipEventRecord(sl.ev, sl.stream) == hipSuccess;
-    if (!ok) { s->fatal = TSU_EHIP; return false; }
+    if (!ok) {
+        // nothing of this flight may still run when its memory is reused
+        (void)hipStreamSynchronize(sl.stream);
+        s->fatal = TSU_EHIP;
+        fail_flight(s, sl, TSU_EHIP);
+        return false;
+    }
     sl.active = true;
     return true;
 }
@@ -349,23 +397,15 @@ void retire(tsu_server* s, Slot& sl) {
             ok = ok && hipMemcpyAsync(sl.h_scores, sl.d_scores, (size_t)sl.n * 4, hipMemcpyDeviceToHost, sl.stream) == hipSuccess &&
                  hipEventRecord(sl.ev, sl.stream) == hipSuccess;
             if (ok) return;  // still in flight
+            (void)hipStreamSynchronize(sl.stream);  // before the memory is reused
             s->fatal = TSU_EHIP;
         }
     }
     const int64_t now = mono_ns();
-    for (size_t ri = 0; ri < sl.reqs.size(); ++ri) {
-        Req* r = sl.reqs[ri];
-        const int32_t f = sl.req_first[ri];
-        r->scores.assign(sl.h_scores + f, sl.h_scores + f + (int64_t)r->pairs.size());
-        r->status = TSU_OK;
-        for (int32_t v : r->scores)
-            if (v < 0) { r->status = v; break; }
-        r->done = true;
-        r->done_ns = now;
-    }
+    for (int32_t k = 0; k < sl.n; ++k) finish_entry(sl.ents[(size_t)k].r, sl.ents[(size_t)k].i, sl.h_scores[k], now);
     s->pool.release(sl.off, sl.bytes);
     sl.off = -1;
-    sl.reqs.clear();
+    sl.ents.clear();
     sl.active = false;
     s->cv.notify_all();
 }
@@ -380,23 +420,32 @@ void worker_loop(tsu_server* s) {
             if (!sl.active) continue;
             const hipError_t q = hipEventQuery(sl.ev);
             if (q == hipSuccess) { retire(s, sl); progressed = true; }
-            else if (q != hipErrorNotReady) { s->fatal = TSU_EHIP; }
+            else if (q != hipErrorNotReady) {
+                (void)hipStreamSynchronize(sl.stream);
+                s->fatal = TSU_EHIP;
+                fail_flight(s, sl, TSU_EHIP);
+                progressed = true;
+            }
             if (sl.active) any_active = true;
         }
         if (s->fatal == TSU_OK) {
             for (Slot& sl : s->slots) {
-                if (sl.active || s->pending.empty()) continue;
+                if (sl.active || s->n_pending == 0) continue;
                 if (admit(s, sl)) { progressed = true; any_active = true; }
                 else break;
             }
         }
         if (s->fatal != TSU_OK) {
             // fail everything still queued; in-flight work is waited for below
-            for (Req* r : s->pending) { r->status = s->fatal; r->done = true; r->done_ns = mono_ns(); }
-            s->pending.clear();
+            const int64_t now = mono_ns();
+            for (std::deque<Entry>& q : s->pending) {
+                for (const Entry& en : q) finish_entry(en.r, en.i, s->fatal, now);
+                q.clear();
+            }
+            s->n_pending = 0;
             s->cv.notify_all();
         }
-        if (s->stop && s->pending.empty() && !any_active) break;
+        if (s->stop && s->n_pending == 0 && !any_active) break;
         if (!progressed) {
             if (any_active) s->cv.wait_for(lk, std::chrono::microseconds(50));
             else s->cv.wait(lk);
@@ -421,6 +470,7 @@ extern "C" int tsu_serve_open(tsu_ctx* ctx, const tsu_serve_cfg* cfg, tsu_server
         return tsu_host::fail(ctx, TSU_ENOMEM, "tsu_serve_open: cannot reserve the HBM budget");
     }
     s->pool.init(s->d_base, cfg->hbm_budget_bytes);
+    s->pending.resize((size_t)tsu_host::bucket_count());
     s->slots.resize((size_t)cfg->max_inflight);
     for (Slot& sl : s->slots) {
         if (hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking) != hipSuccess ||
@@ -434,6 +484,8 @@ extern "C" int tsu_serve_open(tsu_ctx* ctx, const tsu_serve_cfg* cfg, tsu_server
     return TSU_OK;
 }
 
+static int enqueue(tsu_server* s, Req* r, uint64_t* ticket);
+
 extern "C" int tsu_serve_submit(tsu_server* s, const char* const* patterns, const int32_t* p_len,
                                 const char* const* texts, const int32_t* t_len, int32_t n, uint64_t* ticket) {
     if (!s || !ticket || n < 0 || (n > 0 && (!patterns || !p_len || !texts || !t_len))) return TSU_EINVAL;
@@ -444,7 +496,8 @@ extern "C" int tsu_serve_submit(tsu_server* s, const char* const* patterns, cons
         r->pairs.resize((size_t)n);
         int64_t w = 0;
         for (int32_t i = 0; i < n; ++i) {
-            if (p_len[i] < 0 || t_len[i] < 0 || p_len[i] > (1 << 26) || t_len[i] > (1 << 26)) { delete r; return TSU_EINVAL; }
+            const int32_t lmax = tsu_host::max_seq_len(s->ctx);  // both sequences are staged in LDS
+            if (p_len[i] < 0 || t_len[i] < 0 || p_len[i] > lmax || t_len[i] > lmax) { delete r; return TSU_EINVAL; }
             r->pairs[(size_t)i] = tsu_pair{w, w + tsu_packed_words(p_len[i]), p_len[i], t_len[i]};
             w += tsu_packed_words(p_len[i]) + tsu_packed_words(t_len[i]);
             r->max_len = std::max(r->max_len, std::max(p_len[i], t_len[i]));
@@ -461,6 +514,36 @@ extern "C" int tsu_serve_submit(tsu_server* s, const char* const* patterns, cons
         delete r;
         return TSU_ENOMEM;
     }
+    return enqueue(s, r, ticket);
+}
+
+extern "C" int tsu_serve_submit_packed(tsu_server* s, const uint32_t* words, int64_t n_words, const tsu_pair* pairs,
+                                       int32_t n, uint64_t* ticket) {
+    if (!s || !ticket || n < 0 || n_words < 0 || (n > 0 && (!words || !pairs))) return TSU_EINVAL;
+    const int32_t lmax = tsu_host::max_seq_len(s->ctx);  // both sequences are staged in LDS
+    for (int32_t i = 0; i < n; ++i) {
+        const tsu_pair& pr = pairs[i];
+        if (pr.p_len < 0 || pr.t_len < 0 || pr.p_len > lmax || pr.t_len > lmax || pr.p_off < 0 || pr.t_off < 0 ||
+            pr.p_off + tsu_packed_words(pr.p_len) > n_words || pr.t_off + tsu_packed_words(pr.t_len) > n_words)
+            return TSU_EINVAL;
+    }
+    Req* r = new (std::nothrow) Req();
+    if (!r) return TSU_ENOMEM;
+    r->submit_ns = mono_ns();
+    try {
+        r->words.assign(words, words + n_words);
+        r->pairs.assign(pairs, pairs + n);
+    } catch (...) {
+        delete r;
+        return TSU_ENOMEM;
+    }
+    for (int32_t i = 0; i < n; ++i) r->max_len = std::max(r->max_len, std::max(pairs[i].p_len, pairs[i].t_len));
+    return enqueue(s, r, ticket);
+}
+
+// Queues a packed request (or completes an empty one) under the server lock.
+static int enqueue(tsu_server* s, Req* r, uint64_t* ticket) {
+    const int32_t n = (int32_t)r->pairs.size();
     (void)hipSetDevice(s->ctx->device);
     std::lock_guard<std::mutex> lk(s->mu);
     if (s->stop) { delete r; return TSU_EINVAL; }
@@ -483,7 +566,13 @@ extern "C" int tsu_serve_submit(tsu_server* s, const char* const* patterns, cons
             delete r;
             return st != 0 ? TSU_EINTERNAL : TSU_EBUDGET;
         }
-        s->pending.push_back(r);
+        r->scores.assign((size_t)n, 0);
+        r->remaining = n;
+        for (int32_t i = 0; i < n; ++i) {
+            const tsu_pair& pr = r->pairs[(size_t)i];
+            s->pending[(size_t)tsu_host::bucket_of(std::max(pr.p_len, pr.t_len))].push_back(Entry{r, i});
+        }
+        s->n_pending += n;
     }
     s->cv.notify_all();
     return TSU_OK;

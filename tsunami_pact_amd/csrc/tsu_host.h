@@ -72,6 +72,8 @@ int fail(tsu_ctx* ctx, int status, const std::string& msg);
 bool fixed_242(const tsu_ctx* ctx);
 // Chunks of 64 diagonals of the register-resident kernel for capacity W (0: LDS kernel).
 int reg_chunks(const tsu_ctx* ctx, int32_t W);
+// Longest sequence a pair may hold (both sequences are staged in LDS).
+int32_t max_seq_len(const tsu_ctx* ctx);
 // Waves of the multi-wave register kernel for capacity W (0: not eligible).
 int reg_waves(const tsu_ctx* ctx, int32_t W);
 

@@ -3,6 +3,9 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <map>
+#include <mutex>
+
 #include "wfa_kernel.h"
 
 namespace tsu {
@@ -65,9 +68,34 @@ int64_t max_lds_for(const Shape& sh) {
     return v;
 }
 
+// Occupancy answers are cached per (device, kernel, LDS bytes): the server
+// plans a launch for every flight, and the runtime query is not free.
+namespace {
+struct OccKey {
+    int dev;
+    KernelFn f;
+    int64_t lds;
+    bool operator<(const OccKey& o) const {
+        if (dev != o.dev) return dev < o.dev;
+        if (f != o.f) return reinterpret_cast<uintptr_t>(f) < reinterpret_cast<uintptr_t>(o.f);
+        return lds < o.lds;
+    }
+};
+std::mutex g_occ_mu;
+std::map<OccKey, int> g_occ;
+}  // namespace
+
 int occupancy(const Shape& sh, int64_t lds, int* blocks_per_cu) {
     KernelFn f = pick(sh);
     if (!f) return (int)hipErrorInvalidValue;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return (int)hipErrorInvalidDevice;
+    const OccKey key{dev, f, lds};
+    {
+        std::lock_guard<std::mutex> g(g_occ_mu);
+        auto it = g_occ.find(key);
+        if (it != g_occ.end()) { *blocks_per_cu = it->second; return 0; }
+    }
     if (lds > 65536) {
         hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(f), hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         if (e != hipSuccess) return (int)e;
@@ -76,6 +104,8 @@ int occupancy(const Shape& sh, int64_t lds, int* blocks_per_cu) {
     hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, reinterpret_cast<const void*>(f), sh.nw * 64, (size_t)lds);
     if (e != hipSuccess) return (int)e;
     *blocks_per_cu = n;
+    std::lock_guard<std::mutex> g(g_occ_mu);
+    g_occ[key] = n;
     return 0;
 }
 
