@@ -390,18 +390,21 @@ class Aligner:
         return [{"waves_per_pair": arr[i].waves_per_pair, "capacity": arr[i].capacity,
                  "bucket_bound": arr[i].bucket_bound} for i in range(min(n.value, 64))]
 
-    def align_cigar(self, batch: Batch):
+    def align_cigar(self, batch: Batch, strings: bool = True):
+        """(scores, CIGAR strings, timing).  With strings=False the CIGARs are
+        computed and copied to the host by the library as usual but not
+        turned into Python strings (None is returned in their place)."""
         n = len(batch)
         scores = np.zeros(max(n, 1), np.int32)
         t = Timing()
         h = ctypes.c_void_p()
         self._check(self._lib.tsu_align_cigar(self._h, batch._h, _i32p(scores), ctypes.byref(h), ctypes.byref(t)),
                     "tsu_align_cigar")
-        cigars = []
+        cigars = [] if strings else None
         try:
             rp = ctypes.POINTER(ctypes.c_uint32)()
             nr = ctypes.c_int32()
-            for i in range(n):
+            for i in range(n if strings else 0):
                 self._check(self._lib.tsu_cigars_get(h, i, ctypes.byref(rp), ctypes.byref(nr)), "tsu_cigars_get")
                 cigars.append(cigar_string(rp[j] for j in range(nr.value)))
         finally:

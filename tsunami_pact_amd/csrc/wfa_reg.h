@@ -79,6 +79,36 @@ __device__ __forceinline__ int cmp32(uint32_t bp, uint32_t bt, uint32_t sa, uint
     return xx ? (int)__builtin_ctzll(xx) : 64;
 }
 
+// Bases pattern[v..] and text[h..] agree on, at most lim (>= 1): 16 bases
+// first (most diagonals stop there), then the rest of the 32-base window,
+// then rounds of 32.  Reads stay below word (v+lim-1)/16 + 2 of each
+// sequence, inside the two padding words.
+__device__ __forceinline__ int extend_len(uint32_t v, uint32_t h, int lim) {
+    const uint32_t bp = (v >> 4) << 3, bt = (h >> 4) << 3;
+    const uint32_t sa = v << 1, sb = h << 1;
+    lds_cu32* pp = reinterpret_cast<lds_cu32*>((uintptr_t)bp);
+    lds_cu32* tt = reinterpret_cast<lds_cu32*>((uintptr_t)bt);
+    const uint32_t p0 = pp[0], p1 = pp[2], t0 = tt[1], t1 = tt[3];
+    const uint32_t xl = __builtin_amdgcn_alignbit(p1, p0, sa) ^ __builtin_amdgcn_alignbit(t1, t0, sb);
+    const int lim2 = 2 * lim;
+    int n2 = xl ? (int)__builtin_ctz(xl) : 32;  // matched bits, 2 per base
+    if (n2 == 32 && n2 < lim2) {
+        const uint32_t xh = __builtin_amdgcn_alignbit(pp[4], p1, sa) ^ __builtin_amdgcn_alignbit(tt[5], t1, sb);
+        int m = xh ? (int)__builtin_ctz(xh) : 32;
+        n2 += m;
+        if (m == 32 && n2 < lim2) {
+            uint32_t bp2 = bp + 16, bt2 = bt + 16;
+            do {
+                m = cmp32(bp2, bt2, sa, sb);
+                n2 += m;
+                bp2 += 16;
+                bt2 += 16;
+            } while (m == 64 && n2 < lim2);
+        }
+    }
+    return min(n2 >> 1, lim);
+}
+
 // Longest common prefix of pattern[v..] and text[h..] (at most lim bases)
 // over the interleaved layout; the same 32-bases-per-round compare as the
 // extend in the kernel.
@@ -216,27 +246,8 @@ __global__ __launch_bounds__(64) void wfa_reg_kernel(KArgs a) {
                     const bool valid = (uint32_t)M < (uint32_t)mt1;
                     M = valid ? M : NUL;
                     const int lim = mt1 - 1 - M;  // min(p - v, t - M) for a valid M
-                    if (valid && lim > 0) {
-                        // extend: 32 bases per round from the packed words
-                        // (v >= 0 for every valid M); alignbit takes the
-                        // shift modulo 32.  The first round is peeled: most
-                        // diagonals stop in it.
-                        const uint32_t v = (uint32_t)(M - k);
-                        uint32_t bp = (v >> 4) << 3, bt = ((uint32_t)M >> 4) << 3;
-                        const uint32_t sa = v << 1, sb = (uint32_t)M << 1;
-                        const int lim2 = 2 * lim;
-                        int m = cmp32(bp, bt, sa, sb);
-                        int n2 = m;  // matched bits, 2 per base
-                        if (m == 64 && n2 < lim2) {
-                            do {
-                                bp += 16;
-                                bt += 16;
-                                m = cmp32(bp, bt, sa, sb);
-                                n2 += m;
-                            } while (m == 64 && n2 < lim2);
-                        }
-                        M += min(n2 >> 1, lim);
-                    }
+                    // extend from the packed words (v >= 0 for every valid M)
+                    if (valid && lim > 0) M += extend_len((uint32_t)(M - k), (uint32_t)M, lim);
                     Mr[PH][c] = M;
                     Ir[c] = I;
                     Dr[c] = D;

@@ -217,23 +217,7 @@ __global__ __launch_bounds__(NW * 64) void wfa_regmw_kernel(KArgs a) {
                     const bool valid = (uint32_t)M < (uint32_t)mt1;
                     M = valid ? M : NUL;
                     const int lim = mt1 - 1 - M;
-                    if (valid && lim > 0) {
-                        const uint32_t v = (uint32_t)(M - k);
-                        uint32_t bp = (v >> 4) << 3, bt = ((uint32_t)M >> 4) << 3;
-                        const uint32_t sa = v << 1, sb = (uint32_t)M << 1;
-                        const int lim2 = 2 * lim;
-                        int m = cmp32(bp, bt, sa, sb);
-                        int n2 = m;
-                        if (m == 64 && n2 < lim2) {
-                            do {
-                                bp += 16;
-                                bt += 16;
-                                m = cmp32(bp, bt, sa, sb);
-                                n2 += m;
-                            } while (m == 64 && n2 < lim2);
-                        }
-                        M += min(n2 >> 1, lim);
-                    }
+                    if (valid && lim > 0) M += extend_len((uint32_t)(M - k), (uint32_t)M, lim);
                     Mr[PH][c] = M;
                     Ir[c] = I;
                     Dr[c] = D;
