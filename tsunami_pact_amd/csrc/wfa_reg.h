@@ -211,6 +211,11 @@ __global__ __launch_bounds__(64) void wfa_reg_kernel(KArgs a) {
             constexpr int SO = (PH - OS - ES + 4 * RM) % RM;
             ++s;
             if (s > s_bound) return TSU_EINTERNAL;
+            // an opaque zero per step keeps the per-chunk bounds below from
+            // being hoisted out of the score loop (16 VGPRs at C = 8): they
+            // are recomputed, which costs less than the occupancy
+            int zero_s;
+            asm volatile("s_mov_b32 %0, 0" : "=s"(zero_s));
             constexpr int P1 = (PH - 1 + RM) % RM;  // score s-1 (its I and D are the I_e, D_e sources)
             // chunk c computes when it may hold a non-NULL source (scores s-1,
             // s-x, s-o-e) or an old value to overwrite (scores s-1 and s-RM),
@@ -225,7 +230,7 @@ __global__ __launch_bounds__(64) void wfa_reg_kernel(KArgs a) {
 #pragma unroll
             for (int c = 0; c < C; ++c) {
                 if (act & (1u << c)) {
-                    const int k = kmin + 64 * c + lane;
+                    const int k = kmin + zero_s + 64 * c + lane;
                     // a value y >= 0 on diagonal k is in bounds iff y <= t and
                     // y - k <= p, i.e. y < mt1; mt1 >= 0 since k >= -p-1
                     const int mt1 = min(k + p + 1, t + 1);
