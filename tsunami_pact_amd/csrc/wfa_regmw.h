@@ -179,6 +179,10 @@ __global__ __launch_bounds__(NW * 64) void wfa_regmw_kernel(KArgs a) {
             ++s;
             if (s > s_bound) return TSU_EINTERNAL;
             fidx = fidx + 1 == 3 ? 0 : fidx + 1;
+            // an opaque zero per step: the per-chunk bounds are recomputed
+            // rather than held in VGPRs across the score loop
+            int zero_s;
+            asm volatile("s_mov_b32 %0, 0" : "=s"(zero_s));
             const int par = s & 1, par1 = par ^ 1;
             // neighbour waves' sources (scores s-1 and s-o-e) for the spills
             const uint32_t srcL = (uint32_t)__builtin_amdgcn_readfirstlane(X[L::FW + P1 * NW + wl]) |
@@ -199,7 +203,7 @@ __global__ __launch_bounds__(NW * 64) void wfa_regmw_kernel(KArgs a) {
 #pragma unroll
             for (int c = 0; c < C; ++c) {
                 if (act & (1u << c)) {
-                    const int k = kmin + 64 * (c * NW + w) + lane;
+                    const int k = kmin + zero_s + 64 * (c * NW + w) + lane;
                     const int mt1 = min(k + p + 1, t + 1);
                     const int32_t mo = Mr[SO][c];
                     // neighbour chunks' edge lanes from LDS (broadcast reads)
