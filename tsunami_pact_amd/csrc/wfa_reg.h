@@ -79,34 +79,43 @@ __device__ __forceinline__ int cmp32(uint32_t bp, uint32_t bt, uint32_t sa, uint
     return xx ? (int)__builtin_ctzll(xx) : 64;
 }
 
-// Bases pattern[v..] and text[h..] agree on, at most lim (>= 1): 16 bases
-// first (most diagonals stop there), then the rest of the 32-base window,
-// then rounds of 32.  Reads stay below word (v+lim-1)/16 + 2 of each
-// sequence, inside the two padding words.
-__device__ __forceinline__ int extend_len(uint32_t v, uint32_t h, int lim) {
-    const uint32_t bp = (v >> 4) << 3, bt = (h >> 4) << 3;
-    const uint32_t sa = v << 1, sb = h << 1;
+// Extension on doubled offsets (2 x bases, the unit the register kernels
+// keep offsets in): v2 = 2v, h2 = 2h, lim2 = 2 x the most bases allowed
+// (>= 2).  Returns 2 x the bases pattern[v..] and text[h..] agree on, at
+// most lim2.  16 bases first (most diagonals stop there), then the rest of
+// the 32-base window, then rounds of 32; a doubled offset is also the bit
+// position of its base in the packed words, so it is the funnel shift
+// (alignbit takes it modulo 32).  Reads stay below word (v+lim-1)/16 + 2 of
+// each sequence, inside the two padding words.
+__device__ __forceinline__ int extend2(uint32_t v2, uint32_t h2, int lim2) {
+    const uint32_t bp = (v2 >> 5) << 3, bt = (h2 >> 5) << 3;
     lds_cu32* pp = reinterpret_cast<lds_cu32*>((uintptr_t)bp);
     lds_cu32* tt = reinterpret_cast<lds_cu32*>((uintptr_t)bt);
     const uint32_t p0 = pp[0], p1 = pp[2], t0 = tt[1], t1 = tt[3];
-    const uint32_t xl = __builtin_amdgcn_alignbit(p1, p0, sa) ^ __builtin_amdgcn_alignbit(t1, t0, sb);
-    const int lim2 = 2 * lim;
-    int n2 = xl ? (int)__builtin_ctz(xl) : 32;  // matched bits, 2 per base
-    if (n2 == 32 && n2 < lim2) {
-        const uint32_t xh = __builtin_amdgcn_alignbit(pp[4], p1, sa) ^ __builtin_amdgcn_alignbit(tt[5], t1, sb);
+    const uint32_t ap = __builtin_amdgcn_alignbit(p1, p0, v2), at = __builtin_amdgcn_alignbit(t1, t0, h2);
+    const uint32_t x = ap ^ at;
+    int n2 = x ? (int)__builtin_ctz(x) : 32;  // matched bits (2 per base)
+    if (n2 == 32 && lim2 > 32) {
+        const uint32_t xh = __builtin_amdgcn_alignbit(pp[4], p1, v2) ^ __builtin_amdgcn_alignbit(tt[5], t1, h2);
         int m = xh ? (int)__builtin_ctz(xh) : 32;
         n2 += m;
         if (m == 32 && n2 < lim2) {
             uint32_t bp2 = bp + 16, bt2 = bt + 16;
             do {
-                m = cmp32(bp2, bt2, sa, sb);
+                m = cmp32(bp2, bt2, v2, h2);
                 n2 += m;
                 bp2 += 16;
                 bt2 += 16;
             } while (m == 64 && n2 < lim2);
         }
     }
-    return min(n2 >> 1, lim);
+    return min(n2 & ~1, lim2);
+}
+
+// The same on plain offsets: bases pattern[v..] and text[h..] agree on, at
+// most lim (>= 1).
+__device__ __forceinline__ int extend_len(uint32_t v, uint32_t h, int lim) {
+    return extend2(v << 1, h << 1, lim << 1) >> 1;
 }
 
 // Longest common prefix of pattern[v..] and text[h..] (at most lim bases)
@@ -141,6 +150,7 @@ __global__ __launch_bounds__(64) void wfa_reg_kernel(KArgs a) {
     constexpr uint32_t ALL = (1u << C) - 1u;
     extern __shared__ __align__(16) uint32_t smem[];
     const int lane = threadIdx.x;
+    const int lane2 = 2 * lane;
     uint32_t* sS = smem;  // interleaved pattern / text words
     int32_t* sMisc = reinterpret_cast<int32_t*>(sS + 2 * a.seq_words);
     // the extend addresses the words from LDS address 0 (see cmp32)
@@ -198,7 +208,7 @@ __global__ __launch_bounds__(64) void wfa_reg_kernel(KArgs a) {
             const int c0 = (0 - kmin) >> 6, l0 = (0 - kmin) & 63;
 #pragma unroll
             for (int c = 0; c < C; ++c)
-                if (c == c0) Mr[0][c] = lane == l0 ? ext0 : NUL;
+                if (c == c0) Mr[0][c] = lane == l0 ? 2 * ext0 : NUL;
             F[0] = (1u << c0) | (l0 == 0 ? 1u << (8 + c0) : 0u) | (l0 == 63 ? 1u << (16 + c0) : 0u);
         }
         int s = 0;
@@ -230,10 +240,11 @@ __global__ __launch_bounds__(64) void wfa_reg_kernel(KArgs a) {
 #pragma unroll
             for (int c = 0; c < C; ++c) {
                 if (act & (1u << c)) {
-                    const int k = kmin + zero_s + 64 * c + lane;
-                    // a value y >= 0 on diagonal k is in bounds iff y <= t and
-                    // y - k <= p, i.e. y < mt1; mt1 >= 0 since k >= -p-1
-                    const int mt1 = min(k + p + 1, t + 1);
+                    // offsets are kept doubled: y2 = 2y.  A value y >= 0 on
+                    // diagonal k is in bounds iff y <= t and y - k <= p, i.e.
+                    // y2 < mt2 = 2 min(k+p+1, t+1) (>= 0 since k >= -p-1)
+                    const int k2 = 2 * (kmin + zero_s + 64 * c) + lane2;
+                    const int mt2 = min(k2 + 2 * (p + 1), 2 * (t + 1));
                     const int32_t mo = Mr[SO][c];
                     const int32_t mol = dpp_shr1(c > 0 ? dpp_ror1(Mr[SO][c > 0 ? c - 1 : 0]) : NUL, mo);
                     const int32_t mor = dpp_shl1(c < C - 1 ? dpp_rol1(Mr[SO][c < C - 1 ? c + 1 : c]) : NUL, mo);
@@ -245,14 +256,14 @@ __global__ __launch_bounds__(64) void wfa_reg_kernel(KArgs a) {
                     // M is put back to the canonical NULL by the bounds test.
                     // I is not clamped: an I above t is out of bounds and
                     // only produces out-of-bounds values, as in WFA2-lib.
-                    const int32_t I = max(mol, iel) + 1;
+                    const int32_t I = max(mol, iel) + 2;
                     const int32_t D = max(mor, der);
-                    int32_t M = max(max(D, Mr[SX][c] + 1), I);
-                    const bool valid = (uint32_t)M < (uint32_t)mt1;
+                    int32_t M = max(max(D, Mr[SX][c] + 2), I);
+                    const bool valid = (uint32_t)M < (uint32_t)mt2;
                     M = valid ? M : NUL;
-                    const int lim = mt1 - 1 - M;  // min(p - v, t - M) for a valid M
+                    const int lim2 = mt2 - 2 - M;  // 2 min(p - v, t - M) for a valid M
                     // extend from the packed words (v >= 0 for every valid M)
-                    if (valid && lim > 0) M += extend_len((uint32_t)(M - k), (uint32_t)M, lim);
+                    if (valid && lim2 > 0) M += extend2((uint32_t)(M - k2), (uint32_t)M, lim2);
                     Mr[PH][c] = M;
                     Ir[c] = I;
                     Dr[c] = D;
@@ -266,8 +277,8 @@ __global__ __launch_bounds__(64) void wfa_reg_kernel(KArgs a) {
                     asm volatile("" : "+s"(fl));
                     acc |= fl << c;
                     if (c == ctp) mend = __builtin_amdgcn_readlane(M, ltp);
-                    // I and D are out of bounds iff >= mt1 (D <= t by construction)
-                    oob |= vote(max(I, D) >= mt1);
+                    // I and D are out of bounds iff >= mt2 (D <= t by construction)
+                    oob |= vote(max(I, D) >= mt2);
                 } else {
                     carryI = NUL;  // an idle chunk holds only NULL
                 }
@@ -280,9 +291,9 @@ __global__ __launch_bounds__(64) void wfa_reg_kernel(KArgs a) {
                     int f = 0x3fffffff, l = -1;
 #pragma unroll
                     for (int c = 0; c < C; ++c) {
-                        const int k = kmin + 64 * c + lane;
-                        const int mt1 = min(k + p + 1, t + 1);
-                        const uint64_t inb = vote((uint32_t)R[c] < (uint32_t)mt1);
+                        const int k2 = 2 * (kmin + 64 * c) + lane2;
+                        const int mt2 = min(k2 + 2 * (p + 1), 2 * (t + 1));
+                        const uint64_t inb = vote((uint32_t)R[c] < (uint32_t)mt2);
                         if (inb) {
                             f = min(f, 64 * c + (int)__builtin_ctzll(inb));
                             l = max(l, 64 * c + 63 - (int)__builtin_clzll(inb));
@@ -305,7 +316,7 @@ __global__ __launch_bounds__(64) void wfa_reg_kernel(KArgs a) {
                 Fs = m2 >> 1;
             }
             F[PH] = Fs;
-            if (mend >= t) return s;
+            if (mend >= 2 * t) return s;
             // A non-NULL cell at a window edge could extend the range beyond
             // it (lane 0 of chunk 0, lane 63 of chunk C-1)
             if ((Fs >> 8) & 1u) return TSU_EKOVERFLOW;
