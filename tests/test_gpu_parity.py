@@ -283,6 +283,34 @@ def test_multiwave_register_kernel(aligner, tsu, oracle):
             assert ok.all()
 
 
+def test_wide_windows_lds_ring_kernel(aligner, tsu, oracle):
+    """Capacities above 2048 diagonals (up to 16384) run on waves of 16
+    chunks whose older M scores sit in an int16 LDS ring (4, 8 or 16 waves).
+    Long divergent pairs (12-20 kbp at 12-15%) need 9.5k-14.8k diagonals;
+    random and very unequal pairs put I/D cells out of bounds at both ends
+    (the trim across waves).  Narrow windows report overflow; 16384 covers
+    every pair here.  The batched form (initial capacity 16384) agrees."""
+    ds = tsu.Dataset.generate(16, seed=SEED + 16, len_min=12000, len_max=20000, err_min=0.12, err_max=0.15)
+    pairs = ds.pairs()
+    rng = np.random.default_rng(16)
+    for _ in range(4):
+        pairs.append(("".join(rng.choice(list("ACGT"), int(rng.integers(2000, 6000)))),
+                      "".join(rng.choice(list("ACGT"), int(rng.integers(2000, 6000))))))
+    pairs.append(("A" * 300, "C" * 7000))
+    want, _, _ = oracle.wfa_batch(pairs, 4, 6, 2)
+    b = aligner.batch(pairs)
+    for W in (3000, 6000, 9000, 16384):
+        say(f"wide W={W}")
+        got, _ = aligner.align_plain(b, W, raise_on_error=False)
+        ok = got != tsu.TSU_EKOVERFLOW
+        assert (got[ok] == want[ok]).all(), W
+        if W == 16384:
+            assert ok.all()
+    say("wide batched")
+    got, _ = aligner.align_batched(b)
+    assert (got == want).all()
+
+
 def test_autotune_then_batched(tsu, oracle):
     """The autotuner fills a per-bucket table; batched results stay exact."""
     ds = tsu.Dataset.generate(3000, seed=SEED + 12, len_min=100, len_max=3000, err_min=0.01, err_max=0.15)

@@ -12,6 +12,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 #include <new>
@@ -334,14 +335,30 @@ int tsu_host::reg_waves(const tsu_ctx* ctx, int32_t W) {
     return nw;
 }
 
+// Capacities above this use the 16-chunk waves with the int16 LDS M ring
+// rather than 8-chunk waves with the whole ring in registers (measured
+// faster on 2-20 kbp pairs: fewer waves per pair, more pairs per CU).
+constexpr int32_t kMlMinCapacity = 2048;
+
+int tsu_host::ml_waves(const tsu_ctx* ctx, int32_t W, int32_t max_len) {
+    if (!tsu_host::fixed_242(ctx) || W <= kMlMinCapacity || W > 16 * 1024 || max_len > 32000) return 0;
+    int nw = 4;
+    while (1024 * nw < W) nw *= 2;
+    return nw;
+}
+
 // Register kernels when the penalties and the capacity allow: one wave for
-// up to 512 diagonals, 2-16 waves of 8 chunks for up to 8192 (and the two
-// sequences fit in LDS).
+// up to 512 diagonals, 2-16 waves of 8 chunks for up to 8192, 4-16 waves of
+// 16 chunks (older M scores in LDS) for up to 16384 (and the two sequences
+// fit in LDS).
 static void use_register_kernel(const tsu_ctx* ctx, int32_t W, int32_t max_len, Shape& sh) {
     Shape r = sh;
     if (int c = tsu_host::reg_chunks(ctx, W)) {
         r.reg_c = c;
         r.nw = 1;
+    } else if (int nwl = tsu_host::ml_waves(ctx, W, max_len)) {
+        r.reg_c = 16;
+        r.nw = nwl;
     } else if (int nw = tsu_host::reg_waves(ctx, W)) {
         r.reg_c = 8;
         r.nw = nw;
@@ -379,6 +396,30 @@ extern "C" int tsu_align_plain(tsu_ctx* ctx, tsu_batch* b, int32_t wf_length, in
     int r;
     if (fits) {
         r = run_shape(ctx, b, sh, W, nullptr, b->n, b->max_len, 0, nullptr, &kms);
+        if (r == TSU_OK && sh.reg_c == 16) {
+            // the 16-chunk kernel stops past its score limit (int16 LDS ring)
+            // with an overflow status: such pairs get the LDS kernel at the
+            // same capacity, so the result depends on the capacity only
+            std::vector<int32_t> tmp((size_t)b->n);
+            HIPCHK(hipMemcpy(tmp.data(), b->d_scores, (size_t)b->n * sizeof(int32_t), hipMemcpyDeviceToHost), "hipMemcpy D2H (scores)");
+            std::vector<int32_t> again;
+            for (int32_t i = 0; i < b->n; ++i)
+                if (tmp[(size_t)i] == TSU_EKOVERFLOW) again.push_back(i);
+            if (!again.empty()) {
+                Shape s2{default_waves(W), b->max_len <= 32000, true, false, true};
+                s2.fixed_242 = sh.fixed_242;
+                KArgs a2 = base_args(ctx);
+                a2.W = W;
+                a2.seq_words = seq_words_for(b->max_len);
+                if (tsu::lds_bytes(s2, a2) > ctx->max_lds) { s2.lds_ring = false; s2.nw = 16; }
+                DevBuf<int32_t> d_order;
+                HIPCHK(d_order.alloc(again.size()), "hipMalloc (order)");
+                HIPCHK(hipMemcpy(d_order.p, again.data(), again.size() * sizeof(int32_t), hipMemcpyHostToDevice), "hipMemcpy (order)");
+                double kms2 = 0;
+                r = run_shape(ctx, b, s2, W, d_order.p, (int32_t)again.size(), b->max_len, 0, nullptr, &kms2);
+                kms += kms2;
+            }
+        }
     } else {
         // ring too wide for LDS: same one-workgroup-per-pair work, ring in HBM,
         // resident workgroups fed from the queue
@@ -459,16 +500,20 @@ int tsu_host::bucket_of(int32_t len) {
 
 // Initial capacity for a bucket: enough for about 15% divergence at the
 // bucket's length (score ~ 0.15 * L * max(x, o+e)), clamped to what the
-// lengths allow.  With the register kernels available (4/6/2) it is also
-// capped at their widest window, 8192 diagonals: most pairs fit, and the
-// few that overflow are re-run wider on the LDS kernel, which costs less
-// than running every long pair there.  Pairs that overflow are re-run wider.
+// lengths allow and to the bucket's length itself (pairs up to 15%
+// divergence need at most ~0.85 L diagonals; a window no wider than the
+// length keeps more pairs resident per CU).  With the register kernels
+// available (4/6/2) it is also capped at their widest window (16384
+// diagonals, 8192 for sequences over 32000 bases): most pairs fit, and the
+// few that overflow are re-run wider, which costs less than running every
+// pair at the widest capacity.
 int32_t tsu_host::initial_capacity(const tsu_ctx* ctx, int32_t bound) {
     const double per_base = 0.15 * (double)std::max(ctx->xs, ctx->os + ctx->es);
     const int64_t s_est = (int64_t)std::ceil(per_base * bound) + ctx->os + ctx->es;
     int32_t W = capacity_for_score(ctx, s_est);
     W = std::min<int32_t>(W, capacity_full(bound, bound));
-    if (fixed_242(ctx)) W = std::min<int32_t>(W, 16 * 512);
+    W = std::min<int32_t>(W, std::max<int32_t>(bound, 64));
+    if (fixed_242(ctx)) W = std::min<int32_t>(W, bound <= 32000 ? 16 * 1024 : 16 * 512);
     return std::max<int32_t>(W, 16);
 }
 

@@ -76,5 +76,8 @@ int reg_chunks(const tsu_ctx* ctx, int32_t W);
 int32_t max_seq_len(const tsu_ctx* ctx);
 // Waves of the multi-wave register kernel for capacity W (0: not eligible).
 int reg_waves(const tsu_ctx* ctx, int32_t W);
+// Waves of the 16-chunk multi-wave kernel (older M scores in an int16 LDS
+// ring) for capacity W and sequences up to max_len (0: not eligible).
+int ml_waves(const tsu_ctx* ctx, int32_t W, int32_t max_len);
 
 }  // namespace tsu_host

@@ -18,6 +18,8 @@ KernelFn pick_nw16(const Shape& sh);
 KernelFn pick_reg(const Shape& sh);
 KernelFn pick_regmw(const Shape& sh);
 int regmw_exchange_words(int nw);
+KernelFn pick_regml(const Shape& sh);
+int regml_exchange_words(int nw);
 
 // Plain streaming copy (HBM byte-counter calibration).
 __global__ __launch_bounds__(256) void copy_kernel(const uint4* __restrict__ src, uint4* __restrict__ dst, int64_t n) {
@@ -38,8 +40,10 @@ int64_t ring_bytes(const Shape& sh, int32_t W, int32_t rm, int32_t ri) {
 
 int64_t lds_bytes(const Shape& sh, const KArgs& a) {
     if (sh.reg_c > 0) {
-        // sequences + misc (one wave) or + the edge exchange area (several)
-        return 4ll * (2 * a.seq_words + (sh.nw > 1 ? regmw_exchange_words(sh.nw) : 16));
+        // sequences + misc (one wave), + the edge exchange area (several waves
+        // of 8 chunks), + the exchange area and the int16 M ring (16 chunks)
+        const int ex = sh.nw == 1 ? 16 : sh.reg_c == 16 ? regml_exchange_words(sh.nw) : regmw_exchange_words(sh.nw);
+        return 4ll * (2 * a.seq_words + ex);
     }
     int64_t b = lds_head_bytes(a.seq_words, a.rm + 2 * a.ri, sh.nw);
     if (sh.lds_ring) b += ring_bytes(sh, a.W, a.rm, a.ri);
@@ -47,7 +51,7 @@ int64_t lds_bytes(const Shape& sh, const KArgs& a) {
 }
 
 static KernelFn pick(const Shape& sh) {
-    if (sh.reg_c > 0) return sh.nw > 1 ? pick_regmw(sh) : pick_reg(sh);
+    if (sh.reg_c > 0) return sh.nw == 1 ? pick_reg(sh) : sh.reg_c == 16 ? pick_regml(sh) : pick_regmw(sh);
     switch (sh.nw) {
     case 1: return pick_nw1(sh);
     case 2: return pick_nw2(sh);

@@ -1,0 +1,339 @@
+// wfa_regml.h -- multi-wave WFA with the two newest M scores, I and D in
+// registers and the two older M scores in LDS.
+//
+// With the scaled penalties 2/3/1 the score-s step reads M[s-2] (mismatch,
+// same diagonal), M[s-4] (gap open, diagonals k-1 and k+1) and I/D[s-1].
+// Here a lane keeps, per chunk of 64 diagonals, M of the two newest scores
+// (a ring of two VGPRs by the parity of s) and I and D (updated in place):
+// four VGPRs per chunk instead of the seven of a full register ring, so a
+// wave holds 16 chunks (1024 diagonals) and 16 waves hold 16384.  M[s-2],
+// once read for the mismatch, is written to an int16 LDS ring of three
+// slots (the scores s-2, s-3 and s-4 of the current step), from which the
+// gap-open values M[s-4][k-1] and M[s-4][k+1] are read straight at their
+// diagonals, across chunk and wave boundaries alike.  The slot written at
+// step s (score s-2) is never the one read (score s-4), and one workgroup
+// barrier per step orders the writes before the reads two steps later.
+//
+// Global chunk j belongs to wave j mod NW, register slot j / NW, as in the
+// multi-wave register kernel (wfa_regmw.h).  I and D cross chunk edges
+// through LDS: each computed chunk publishes lane 63 of I and lane 0 of D
+// for its parity of s, and each wave publishes its step mask every step.
+// Chunks whose sources are all NULL are skipped; the argument that the
+// skipped registers, edge words and LDS slots then hold NULL values is the
+// same as for the other register kernels: a chunk with a non-NULL value at
+// score s is computed again at s+1, s+2, s+4 and s+5.
+//
+// NULL is -2^30 in registers and -32768 in the int16 ring.  An I or D value
+// fed only by the ring's NULL grows by one per score, so the kernel gives up
+// (TSU_EKOVERFLOW, the host re-runs the pair on another kernel) beyond
+// kMlMaxScore scaled scores, where such a value would still be negative;
+// offsets must fit int16, so sequences are at most 32000 bases.
+//
+// Recurrences, bounds test, extend, trim and end test are those of the
+// other kernels (wfa_gpu.cu:253-331,610-631,633-719).
+#pragma once
+
+#include "wfa_regmw.h"
+
+namespace tsu {
+
+constexpr int kMlMaxScore = 30000;
+
+// LDS layout (int32 words) after the interleaved sequence words.
+template <int NW, int C>
+struct MlLayout {
+    static constexpr int NC = NW * C;                       // chunks in the window
+    static constexpr int W = 64 * NC;                       // diagonals in the window
+    static constexpr int EI = 0;                            // [2][NC+2] I lane 63 per chunk, by parity of s
+    static constexpr int ED = EI + 2 * (NC + 2);            // [2][NC+2] D lane 0 per chunk
+    static constexpr int FW = ED + 2 * (NC + 2);            // [5][NW] step masks, by s mod 5
+    static constexpr int FL = FW + 5 * NW;                  // [3] step flags, rotating
+    static constexpr int TR = FL + 3;                       // [4] trim bounds: I first, I last, D first, D last
+    static constexpr int MISC = TR + 4;                     // [4] queue index
+    static constexpr int SK = MISC + 4;                     // [64+NC] write sink of the lanes that publish nothing
+    static constexpr int MR = (SK + 64 + NC + 3) & ~3;      // int16 [3][W+2] M ring, NULL guards at both ends
+    static constexpr int MR_STRIDE = W + 2;                 // int16 entries per slot
+    static constexpr int WORDS = MR + (3 * MR_STRIDE + 1) / 2;
+};
+
+// Step masks (32 bits): bit c (c < 16) is set when chunk c holds a
+// non-NULL M, I or D value; bit 16 when the window's edge lane does (lane 0
+// of chunk 0 in wave 0, lane 63 of the last chunk in the last wave).  A
+// neighbour chunk counts as a source whenever it holds any non-NULL value,
+// which computes a few chunks more at the wavefront's ends than edge-lane
+// masks would, for a third of the scalar work per chunk.
+constexpr uint32_t kMlEdgeBit = 1u << 16;
+
+template <int NW, int C, bool QUEUE>
+__global__ __launch_bounds__(NW * 64) void wfa_regml_kernel(KArgs a) {
+    static_assert(NW >= 2 && NW <= 16, "2..16 waves");
+    static_assert(C >= 1 && C <= 16, "16 chunk bits per wave");
+    constexpr int32_t NUL = kRegNull;
+    constexpr int NC = NW * C;
+    constexpr int W = 64 * NC;
+    constexpr int NT = NW * 64;
+    constexpr uint32_t ALL = (1u << C) - 1u;
+    using L = MlLayout<NW, C>;
+    extern __shared__ __align__(16) uint32_t smem[];
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
+    uint32_t* sS = smem;  // interleaved pattern / text words
+    int32_t* X = reinterpret_cast<int32_t*>(smem + 2 * a.seq_words);
+    int16_t* MR = reinterpret_cast<int16_t*>(X + L::MR);
+    uint32_t* FWp = reinterpret_cast<uint32_t*>(X + L::FW);
+    const bool lds_at_zero = (uint32_t)(uintptr_t)sS == 0u;
+    const int wl = w == 0 ? NW - 1 : w - 1;  // owner of chunk j-1 (slot r-1 when w == 0)
+    const int wr = w == NW - 1 ? 0 : w + 1;  // owner of chunk j+1 (slot r+1 when w == NW-1)
+    // per computed chunk every lane stores once: lane 63 its I edge, lane 0
+    // its D edge, the others into a sink (cheaper than masking the store)
+    const int pub_off = lane == 0 ? L::ED - L::EI : 0;
+
+    for (int iter = 0;; ++iter) {
+        int idx;
+        if constexpr (QUEUE) {
+            if (tid == 0) X[L::MISC] = atomicAdd(a.queue, 1);
+            __syncthreads();
+            idx = __builtin_amdgcn_readfirstlane(X[L::MISC]);
+            __syncthreads();
+        } else {
+            if (iter > 0) break;
+            idx = blockIdx.x;
+        }
+        if (idx >= a.n) break;
+        const int pi = a.order ? __builtin_amdgcn_readfirstlane(a.order[idx]) : idx;
+        const KPair pr = a.pairs[pi];
+        const int p = __builtin_amdgcn_readfirstlane(pr.p_len);
+        const int t = __builtin_amdgcn_readfirstlane(pr.t_len);
+        const int pw = (p + 15) >> 4, tw = (t + 15) >> 4;
+        for (int i = tid; i < a.seq_words; i += NT) {
+            sS[2 * i] = i < pw ? a.words[pr.p_off + i] : 0u;
+            sS[2 * i + 1] = i < tw ? a.words[pr.t_off + i] : 0u;
+        }
+        for (int i = tid; i < L::WORDS; i += NT) {
+            int32_t v = NUL;                                          // I / D edges
+            if (i >= L::FW) v = 0;                                    // masks, flags
+            if (i >= L::TR && i < L::MISC) v = ((i - L::TR) & 1) ? -kBig : kBig;  // trim bounds
+            if (i >= L::MR) v = (int32_t)0x80008000u;                 // two int16 NULLs
+            if (i < L::MISC || i >= L::MR) X[i] = v;
+        }
+        __syncthreads();
+
+        int kmin = max(-(W / 2), -p - 1);
+        int kmax = kmin + W - 1;
+        if (kmax > t + 1) { kmax = t + 1; kmin = max(kmax - W + 1, -p - 1); }
+        const int tp = t - p;
+        const int jtp = tp - kmin >= 0 && tp - kmin < W ? (tp - kmin) >> 6 : -1;
+        // chunk bit of the end diagonal in its owner wave, and its lane
+        const uint32_t endm = jtp >= 0 && jtp % NW == w ? 1u << (jtp / NW) : 0u;
+        const int ltp = (tp - kmin) & 63;
+        const int64_t bound_u = 2ll * min(p, t) + (p != t ? 3ll + (int64_t)abs(p - t) : 0);
+        const int s_bound = (int)min<int64_t>(bound_u, 0x3fffffff);
+
+        int32_t Mr[2][C], Ir[C], Dr[C];
+#pragma unroll
+        for (int c = 0; c < C; ++c) {
+            Mr[0][c] = NUL;
+            Mr[1][c] = NUL;
+            Ir[c] = NUL;
+            Dr[c] = NUL;
+        }
+        // this wave's step masks of scores s-1 .. s-5
+        uint32_t F1 = 0, F2 = 0, F3 = 0, F4 = 0, F5 = 0;
+
+        // score 0: M[0][0] = 0, extended (every lane computes it)
+        const int ext0 = lcp_interleaved(sS, 0, 0, min(p, t));
+        {
+            const int j0 = (0 - kmin) >> 6, l0 = (0 - kmin) & 63;
+            if (j0 % NW == w) {
+                const int r0 = j0 / NW;
+#pragma unroll
+                for (int r = 0; r < C; ++r)
+                    if (r == r0) Mr[0][r] = lane == l0 ? ext0 : NUL;
+                F1 = 1u << r0;
+            }
+            if (lane == 0) FWp[0 * NW + w] = F1;
+        }
+        __syncthreads();
+        int s = 0;
+        int fidx = 0;  // flag word of score s+1 (s+1 mod 3)
+        int l5 = 0;    // s mod 5
+        int l3 = 0;    // s mod 3
+        // neighbour waves' masks of scores s and s-3 (the sources of the
+        // next step), read together with the flag word after each barrier
+        uint32_t nbL = (uint32_t)__builtin_amdgcn_readfirstlane(FWp[wl]);
+        uint32_t nbR = (uint32_t)__builtin_amdgcn_readfirstlane(FWp[wr]);
+
+        // One score step; PAR = s & 1 selects the register slot holding
+        // M[s-2], which receives M[s].  Returns 0 to go on, the score when
+        // the end cell is reached, or a negative status.  Every wave takes
+        // the same branches: they depend only on values read from LDS after
+        // a barrier.
+        auto step = [&](auto PC) __attribute__((always_inline)) -> int {
+            constexpr int PAR = decltype(PC)::value;
+            ++s;
+            if (s > s_bound) return TSU_EINTERNAL;
+            if (s > kMlMaxScore) return TSU_EKOVERFLOW;
+            fidx = fidx + 1 == 3 ? 0 : fidx + 1;
+            l5 = l5 == 4 ? 0 : l5 + 1;
+            l3 = l3 == 2 ? 0 : l3 + 1;
+            const int rd = l3 == 0 ? 2 : l3 - 1;   // ring slot of s-4
+            const int wrs = l3 == 2 ? 0 : l3 + 1;  // ring slot of s-2 (formerly s-5)
+            // opaque per-step copies of the window base and the lane: the
+            // per-chunk diagonals are recomputed each step rather than held
+            // in VGPRs across the score loop
+            int pb = 64 * w;
+            int ln = lane;
+            asm volatile("" : "+s"(pb));
+            asm volatile("" : "+v"(ln));
+            const int kb = kmin + pb;
+            uint32_t hiL = nbL & ALL;
+            uint32_t loR = nbR & ALL;
+            if (w == 0) hiL <<= 1;
+            if (w == NW - 1) loR >>= 1;
+            const uint32_t act = (F1 | F2 | F4 | F5 | hiL | loR) & ALL;
+            const int32_t* ei1 = X + L::EI + (PAR ^ 1) * (NC + 2) + w;
+            const int32_t* ed1 = X + L::ED + (PAR ^ 1) * (NC + 2) + w;
+            int32_t* epub = (lane == 0 || lane == 63) ? X + L::EI + PAR * (NC + 2) + w + 1 + pub_off : X + L::SK + lane;
+            uint32_t em = endm;  // opaque per step: no per-chunk predicates held across the loop
+            asm volatile("" : "+s"(em));
+            const int16_t* mrd = MR + rd * L::MR_STRIDE + pb + ln;        // entry of k-1
+            int16_t* mwr = MR + wrs * L::MR_STRIDE + pb + ln + 1;         // entry of k
+            uint32_t acc = 0;  // step mask, one bit up
+            uint32_t edge = 0;
+            uint64_t oob = 0;
+            int32_t mend = NUL;
+#pragma unroll
+            for (int c = 0; c < C; ++c) {
+                if (act & (1u << c)) {
+                    const int k = kb + 64 * NW * c + ln;
+                    const int mt1 = min(k + p + 1, t + 1);
+                    const int32_t mx = Mr[PAR][c];
+                    mwr[64 * NW * c] = (int16_t)max(mx, -32768);
+                    const int32_t mol = mrd[64 * NW * c];
+                    const int32_t mor = mrd[64 * NW * c + 2];
+                    const int32_t fil = ei1[c * NW];
+                    const int32_t fdr = ed1[c * NW + 2];
+                    const int32_t iel = dpp_shr1(fil, Ir[c]);
+                    const int32_t der = dpp_shl1(fdr, Dr[c]);
+                    const int32_t I = max(mol, iel) + 1;
+                    const int32_t D = max(mor, der);
+                    int32_t M = max(max(D, mx + 1), I);
+                    const bool valid = (uint32_t)M < (uint32_t)mt1;
+                    M = valid ? M : NUL;
+                    const int lim = mt1 - 1 - M;
+                    if (valid && lim > 0) M += extend_len((uint32_t)(M - k), (uint32_t)M, lim);
+                    Mr[PAR][c] = M;
+                    Ir[c] = I;
+                    Dr[c] = D;
+                    epub[c * NW] = lane == 0 ? D : I;
+                    const uint64_t b = vote(max(max(M, I), D) >= 0);
+                    uint32_t fl = b ? 2u << c : 0u;
+                    asm volatile("" : "+s"(fl));
+                    acc |= fl;
+                    if (c == 0 && w == 0) edge |= (uint32_t)(b & 1ull);
+                    if (c == C - 1 && w == NW - 1) edge |= (uint32_t)(b >> 63);
+                    if (em & (1u << c)) mend = __builtin_amdgcn_readlane(M, ltp);
+                    oob |= vote(max(I, D) >= mt1);
+                }
+            }
+            uint32_t Fs = acc >> 1;
+            const uint32_t flags = (oob ? kFlOob : 0u) | (mend >= t ? kFlEnd : 0u) | (edge ? kFlEdge : 0u);
+            if (lane == 0) {
+                if (flags) atomicOr(reinterpret_cast<uint32_t*>(X + L::FL + fidx), flags);
+                if (w == 0) X[L::FL + (fidx + 1 == 3 ? 0 : fidx + 1)] = 0;  // the next score's word
+                FWp[l5 * NW + w] = Fs | (edge ? kMlEdgeBit : 0u);
+            }
+            __syncthreads();
+            const int n4 = l5 >= 3 ? l5 - 3 : l5 + 2;  // slot of s-3
+            const uint32_t all = (uint32_t)__builtin_amdgcn_readfirstlane(X[L::FL + fidx]);
+            nbL = (uint32_t)__builtin_amdgcn_readfirstlane(FWp[l5 * NW + wl] | FWp[n4 * NW + wl]);
+            nbR = (uint32_t)__builtin_amdgcn_readfirstlane(FWp[l5 * NW + wr] | FWp[n4 * NW + wr]);
+            if (all & kFlEnd) return s;
+            if (all & kFlOob) {
+                // exact trim (wfa_gpu.cu:633-719): first / last in-bounds I and
+                // D over the whole window, cells outside them NULL
+                int b4[4] = {kBig, -kBig, kBig, -kBig};
+#pragma unroll
+                for (int c = 0; c < C; ++c) {
+                    // one chunk at a time (the opaque lane keeps the compiler
+                    // from issuing all 2C ballots first, out of scalar registers)
+                    int lc = ln;
+                    asm volatile("" : "+v"(lc));
+                    const int k = kb + 64 * NW * c + lc;
+                    const int mt1 = min(k + p + 1, t + 1);
+                    const int base = pb + 64 * NW * c;
+                    const uint64_t bi = vote((uint32_t)Ir[c] < (uint32_t)mt1);
+                    const uint64_t bd = vote((uint32_t)Dr[c] < (uint32_t)mt1);
+                    if (bi) {
+                        b4[0] = min(b4[0], base + (int)__builtin_ctzll(bi));
+                        b4[1] = max(b4[1], base + 63 - (int)__builtin_clzll(bi));
+                    }
+                    if (bd) {
+                        b4[2] = min(b4[2], base + (int)__builtin_ctzll(bd));
+                        b4[3] = max(b4[3], base + 63 - (int)__builtin_clzll(bd));
+                    }
+                    asm volatile("" : "+s"(b4[0]), "+s"(b4[1]), "+s"(b4[2]), "+s"(b4[3]));
+                }
+                if (lane == 0) {
+                    atomicMin(X + L::TR + 0, b4[0]);
+                    atomicMax(X + L::TR + 1, b4[1]);
+                    atomicMin(X + L::TR + 2, b4[2]);
+                    atomicMax(X + L::TR + 3, b4[3]);
+                }
+                __syncthreads();
+                const int fI = __builtin_amdgcn_readfirstlane(X[L::TR + 0]);
+                const int lI = __builtin_amdgcn_readfirstlane(X[L::TR + 1]);
+                const int fD = __builtin_amdgcn_readfirstlane(X[L::TR + 2]);
+                const int lD = __builtin_amdgcn_readfirstlane(X[L::TR + 3]);
+                uint32_t m2 = 0, e2 = 0;
+                int32_t* ep = (lane == 0 || lane == 63) ? X + L::EI + PAR * (NC + 2) + w + 1 + pub_off : X + L::SK + lane;
+#pragma unroll
+                for (int c = 0; c < C; ++c) {
+                    int lc = ln;
+                    asm volatile("" : "+v"(lc));
+                    const int pos = pb + 64 * NW * c + lc;
+                    if (pos < fI || pos > lI) Ir[c] = NUL;
+                    if (pos < fD || pos > lD) Dr[c] = NUL;
+                    ep[c * NW] = lane == 0 ? Dr[c] : Ir[c];
+                    const uint64_t b = vote(max(max(Mr[PAR][c], Ir[c]), Dr[c]) >= 0);
+                    m2 |= b ? 2u << c : 0u;
+                    asm volatile("" : "+s"(m2));
+                    if (c == 0 && w == 0) e2 |= (uint32_t)(b & 1ull);
+                    if (c == C - 1 && w == NW - 1) e2 |= (uint32_t)(b >> 63);
+                }
+                Fs = m2 >> 1;
+                if (lane == 0) FWp[l5 * NW + w] = Fs | (e2 ? kMlEdgeBit : 0u);
+                __syncthreads();
+                if (tid == 0) {
+                    X[L::TR + 0] = kBig;
+                    X[L::TR + 1] = -kBig;
+                    X[L::TR + 2] = kBig;
+                    X[L::TR + 3] = -kBig;
+                }
+                F5 = F4; F4 = F3; F3 = F2; F2 = F1; F1 = Fs;
+                nbL = (uint32_t)__builtin_amdgcn_readfirstlane(FWp[l5 * NW + wl] | FWp[n4 * NW + wl]);
+                nbR = (uint32_t)__builtin_amdgcn_readfirstlane(FWp[l5 * NW + wr] | FWp[n4 * NW + wr]);
+                // the window edges after the trim
+                const uint32_t e0 = (uint32_t)__builtin_amdgcn_readfirstlane(FWp[l5 * NW + 0]);
+                const uint32_t e1 = (uint32_t)__builtin_amdgcn_readfirstlane(FWp[l5 * NW + NW - 1]);
+                if ((e0 | e1) & kMlEdgeBit) return TSU_EKOVERFLOW;
+                return 0;
+            }
+            F5 = F4; F4 = F3; F3 = F2; F2 = F1; F1 = Fs;
+            if (all & kFlEdge) return TSU_EKOVERFLOW;
+            return 0;
+        };
+
+        int r = (p == t && ext0 == t) ? 1 : 0;  // 1: score 0 (identical sequences)
+        while (r == 0) {
+            r = step(std::integral_constant<int, 1>{});
+            if (r == 0) r = step(std::integral_constant<int, 0>{});
+        }
+        const int score = (r > 0 && s == 0) ? 0 : r;
+        if (tid == 0) a.scores[pi] = !lds_at_zero ? TSU_EINTERNAL : score < 0 ? score : score * a.g;
+        __syncthreads();  // LDS reused by the next pair of this workgroup
+    }
+}
+
+}  // namespace tsu

@@ -1,0 +1,29 @@
+// Instances of the multi-wave kernel with the older M scores in LDS (scaled
+// penalties 2/3/1, i.e. 4/6/2): 16 chunks of 64 diagonals per wave, 4 to 16
+// waves per pair (windows of 4096 to 16384 diagonals).
+#include "wfa_regml.h"
+
+namespace tsu {
+
+constexpr int kMlChunks = 16;
+
+KernelFn pick_regml(const Shape& sh) {
+    if (!sh.fixed_242 || sh.bt || sh.reg_c != kMlChunks) return nullptr;
+    switch (sh.nw) {
+    case 4: return sh.queue ? wfa_regml_kernel<4, 16, true> : wfa_regml_kernel<4, 16, false>;
+    case 8: return sh.queue ? wfa_regml_kernel<8, 16, true> : wfa_regml_kernel<8, 16, false>;
+    case 16: return sh.queue ? wfa_regml_kernel<16, 16, true> : wfa_regml_kernel<16, 16, false>;
+    default: return nullptr;
+    }
+}
+
+int regml_exchange_words(int nw) {
+    switch (nw) {
+    case 4: return MlLayout<4, kMlChunks>::WORDS;
+    case 8: return MlLayout<8, kMlChunks>::WORDS;
+    case 16: return MlLayout<16, kMlChunks>::WORDS;
+    default: return 0;
+    }
+}
+
+}  // namespace tsu
