@@ -181,7 +181,7 @@ def main():
     ap.add_argument("--pairs", type=int, default=0, help="override the pair count (testing only)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-autotune", action="store_true", help="batched: use the default shapes")
-    ap.add_argument("--tune-sample", type=int, default=1024, help="batched: autotuner sample per bucket")
+    ap.add_argument("--tune-sample", type=int, default=4096, help="batched: autotuner sample per bucket")
     ap.add_argument("--compare-plain", action="store_true", help="batched: also time the plain form")
     ap.add_argument("--requests", type=int, default=300, help="served: requests per rate")
     ap.add_argument("--serve-budget", type=int, default=16384, help="served: HBM budget in MiB")

@@ -535,6 +535,12 @@ Shape tsu_host::batched_shape(const tsu_ctx* ctx, int32_t W, int32_t max_len, in
     return sh;
 }
 
+int32_t tsu_host::tuned_capacity(const tsu_ctx* ctx, int32_t bound) {
+    for (const tsu_tune_entry& te : ctx->tune)
+        if (te.bucket_bound == bound) return te.capacity;
+    return 0;
+}
+
 int tsu_host::tuned_waves(const tsu_ctx* ctx, int32_t bound) {
     for (const tsu_tune_entry& te : ctx->tune)
         if (te.bucket_bound == bound) return te.waves_per_pair;
@@ -586,7 +592,9 @@ static int batched_core(tsu_ctx* ctx, tsu_batch* b, const std::vector<int32_t>& 
         std::vector<int32_t> todo = bk.idx;
         for (int attempt = 0; !todo.empty(); ++attempt) {
             W = std::min<int32_t>(W, capacity_full(max_len, max_len));
-            Shape sh = batched_shape(ctx, W, max_len, nw_hint);
+            // the tuned shape is for the first attempt; wider re-runs take
+            // the default shape of their capacity
+            Shape sh = batched_shape(ctx, W, max_len, attempt == 0 ? nw_hint : 0);
             HIPCHK(hipMemcpyAsync(d_order.p, todo.data(), todo.size() * sizeof(int32_t), hipMemcpyHostToDevice, ctx->stream),
                    "hipMemcpyAsync (order)");
             double kms = 0;
@@ -630,8 +638,9 @@ extern "C" int tsu_align_batched(tsu_ctx* ctx, tsu_batch* b, int32_t* scores, ts
     return TSU_OK;
 }
 
-// Autotuner: for each populated bucket, time the candidate waves-per-pair on
-// a sample of the bucket and keep the fastest.
+// Autotuner: for each populated bucket, time the candidate shapes (initial
+// capacity, or waves per pair of the LDS kernel) on a sample of the bucket
+// and keep the fastest.
 extern "C" int tsu_autotune(tsu_ctx* ctx, tsu_batch* b, int32_t sample_pairs) {
     if (!ctx || !b || b->ctx != ctx) return fail(ctx, TSU_EINVAL, "tsu_autotune: bad arguments");
     HIPCHK(hipSetDevice(ctx->device), "hipSetDevice");
@@ -653,19 +662,32 @@ extern "C" int tsu_autotune(tsu_ctx* ctx, tsu_batch* b, int32_t sample_pairs) {
         std::vector<int32_t> sample;
         const size_t step = std::max<size_t>(1, m.size() / (size_t)sample_pairs);
         for (size_t j = 0; j < m.size() && sample.size() < (size_t)sample_pairs; j += step) sample.push_back(m[j]);
+        // Candidates: where the register kernels take the bucket's initial
+        // capacity (4/6/2 penalties), that capacity and half of it (more
+        // pairs per CU; pairs that overflow are re-run at the full one, and
+        // that time is counted); otherwise the LDS kernel with 1-16 waves.
+        int32_t max_len = 0;
+        for (int32_t i : sample) max_len = std::max(max_len, std::max(b->pairs[(size_t)i].p_len, b->pairs[(size_t)i].t_len));
+        const int32_t W0 = initial_capacity(ctx, std::min(kBucketBounds[bi], max_len));
+        std::vector<tsu_tune_entry> cands;
+        if (batched_shape(ctx, W0, max_len, 1).reg_c > 0) {
+            cands.push_back(tsu_tune_entry{1, W0, kBucketBounds[bi], 0});
+            if (W0 / 2 >= 256) cands.push_back(tsu_tune_entry{1, W0 / 2, kBucketBounds[bi], 0});
+        } else {
+            for (int nw : {1, 2, 4, 8, 16}) cands.push_back(tsu_tune_entry{nw, 0, kBucketBounds[bi], 0});
+        }
         double best = 1e300;
-        int best_nw = 0;
-        for (int nw : {1, 2, 4, 8, 16}) {
+        const tsu_tune_entry* pick = nullptr;
+        for (const tsu_tune_entry& probe : cands) {
             ctx->tune.clear();
-            tsu_tune_entry probe{nw, 0, kBucketBounds[bi], 0};
             ctx->tune.push_back(probe);
             tsu_timing tt;
             std::memset(&tt, 0, sizeof(tt));
             int r = batched_core(ctx, b, sample, scratch.data(), &tt, true);
             if (r != TSU_OK) continue;
-            if (tt.kernel_ms < best) { best = tt.kernel_ms; best_nw = nw; }
+            if (tt.kernel_ms < best) { best = tt.kernel_ms; pick = &probe; }
         }
-        if (best_nw > 0) table.push_back(tsu_tune_entry{best_nw, 0, kBucketBounds[bi], 0});
+        if (pick) table.push_back(*pick);
     }
     ctx->tune = table;
     return TSU_OK;

@@ -185,7 +185,8 @@ int64_t plan_flight(tsu_server* s, Slot& sl, int* status) {
         lp.count = (int32_t)v.size();
         for (int32_t i : v) lp.max_len = std::max(lp.max_len, std::max(sl.fpairs[(size_t)i].p_len, sl.fpairs[(size_t)i].t_len));
         const int32_t bound = tsu_host::bucket_bound(bi);
-        lp.W = std::min(tsu_host::initial_capacity(ctx, std::min(bound, lp.max_len)),
+        const int32_t tcap = tsu_host::tuned_capacity(ctx, bound);
+        lp.W = std::min(tcap > 0 ? tcap : tsu_host::initial_capacity(ctx, std::min(bound, lp.max_len)),
                         tsu_host::capacity_full(lp.max_len, lp.max_len));
         lp.sh = tsu_host::batched_shape(ctx, lp.W, lp.max_len, tsu_host::tuned_waves(ctx, bound));
         KArgs a = tsu_host::base_args(ctx);

@@ -65,6 +65,8 @@ int32_t capacity_full(int32_t p, int32_t t);
 Shape batched_shape(const tsu_ctx* ctx, int32_t W, int32_t max_len, int nw_hint);
 // Tuned waves per pair for a bucket (0 when the autotuner has not run).
 int tuned_waves(const tsu_ctx* ctx, int32_t bound);
+// Tuned initial capacity for a bucket (0 when none was tuned).
+int32_t tuned_capacity(const tsu_ctx* ctx, int32_t bound);
 // Resident workgroups of a shape on the whole device.
 int64_t resident_grid(const tsu_ctx* ctx, const Shape& sh, const KArgs& a, int* status);
 int fail(tsu_ctx* ctx, int status, const std::string& msg);
