@@ -64,6 +64,8 @@ struct MlLayout {
 // masks would, for a third of the scalar work per chunk.
 constexpr uint32_t kMlEdgeBit = 1u << 16;
 
+typedef __attribute__((address_space(3))) const int32_t lds_ci32;
+
 template <int NW, int C, bool QUEUE>
 __global__ __launch_bounds__(NW * 64) void wfa_regml_kernel(KArgs a) {
     static_assert(NW >= 2 && NW <= 16, "2..16 waves");
@@ -186,14 +188,17 @@ __global__ __launch_bounds__(NW * 64) void wfa_regml_kernel(KArgs a) {
             int ln = lane;
             asm volatile("" : "+s"(pb));
             asm volatile("" : "+v"(ln));
-            const int kb = kmin + pb;
+            const int kl = kmin + pb + ln;  // diagonal of this lane in chunk 0
             uint32_t hiL = nbL & ALL;
             uint32_t loR = nbR & ALL;
             if (w == 0) hiL <<= 1;
             if (w == NW - 1) loR >>= 1;
             const uint32_t act = (F1 | F2 | F4 | F5 | hiL | loR) & ALL;
-            const int32_t* ei1 = X + L::EI + (PAR ^ 1) * (NC + 2) + w;
-            const int32_t* ed1 = X + L::ED + (PAR ^ 1) * (NC + 2) + w;
+            // edges of score s-1, read through one VGPR address per step
+            int ea = 4 * (2 * a.seq_words + L::EI + (PAR ^ 1) * (NC + 2) + w);
+            asm volatile("" : "+v"(ea));
+            lds_ci32* ei1 = reinterpret_cast<lds_ci32*>((uintptr_t)(uint32_t)ea);
+            lds_ci32* ed1 = ei1 + (L::ED - L::EI);
             int32_t* epub = (lane == 0 || lane == 63) ? X + L::EI + PAR * (NC + 2) + w + 1 + pub_off : X + L::SK + lane;
             uint32_t em = endm;  // opaque per step: no per-chunk predicates held across the loop
             asm volatile("" : "+s"(em));
@@ -206,8 +211,9 @@ __global__ __launch_bounds__(NW * 64) void wfa_regml_kernel(KArgs a) {
 #pragma unroll
             for (int c = 0; c < C; ++c) {
                 if (act & (1u << c)) {
-                    const int k = kb + 64 * NW * c + ln;
-                    const int mt1 = min(k + p + 1, t + 1);
+                    // one past the last offset in bounds: min(k + p + 1, t + 1)
+                    // (>= 0 on every diagonal of the window, kmin >= -p-1)
+                    const int mt1 = min(kl + (p + 1 + 64 * NW * c), t + 1);
                     const int32_t mx = Mr[PAR][c];
                     mwr[64 * NW * c] = (int16_t)max(mx, -32768);
                     const int32_t mol = mrd[64 * NW * c];
@@ -221,8 +227,8 @@ __global__ __launch_bounds__(NW * 64) void wfa_regml_kernel(KArgs a) {
                     int32_t M = max(max(D, mx + 1), I);
                     const bool valid = (uint32_t)M < (uint32_t)mt1;
                     M = valid ? M : NUL;
-                    const int lim = mt1 - 1 - M;
-                    if (valid && lim > 0) M += extend_len((uint32_t)(M - k), (uint32_t)M, lim);
+                    const int lim1 = mt1 - M;  // bases left + 1
+                    if (valid && lim1 > 1) M += extend_len((uint32_t)(M - kl - 64 * NW * c), (uint32_t)M, lim1 - 1);
                     Mr[PAR][c] = M;
                     Ir[c] = I;
                     Dr[c] = D;
@@ -260,7 +266,7 @@ __global__ __launch_bounds__(NW * 64) void wfa_regml_kernel(KArgs a) {
                     // from issuing all 2C ballots first, out of scalar registers)
                     int lc = ln;
                     asm volatile("" : "+v"(lc));
-                    const int k = kb + 64 * NW * c + lc;
+                    const int k = kmin + pb + 64 * NW * c + lc;
                     const int mt1 = min(k + p + 1, t + 1);
                     const int base = pb + 64 * NW * c;
                     const uint64_t bi = vote((uint32_t)Ir[c] < (uint32_t)mt1);
