@@ -171,12 +171,64 @@ def run_served(a, tsu, args, cpu_sample):
     return {"capacity_pairs_per_s": cap_pairs, "rates": out}
 
 
+# The reference's own benchmark grid (run_h100-a100.sh): generate_dataset N L E
+# (substitution-only edits at E%) then wfa_gpu 4 6 2 sequences.txt W 1.
+REF_GRID = [
+    (500_000, 128, 1, 21), (500_000, 512, 1, 21), (50_000, 1024, 1, 200), (10_000, 10240, 1, 350),
+    (500_000, 128, 2, 42), (500_000, 512, 2, 42), (50_000, 1024, 2, 250), (10_000, 10240, 2, 850),
+    (500_000, 128, 5, 105), (500_000, 512, 5, 105), (50_000, 1024, 5, 250), (10_000, 10240, 5, 2200),
+]
+# Peak equivalent GCUPS the reference reports for its best configuration
+# (README.md:6; the GPU is not named there).
+REF_PEAK_GCUPS = 42604.98
+
+
+def run_refgrid(a, tsu, args):
+    """Plain form on the reference's own grid of configurations: equivalent
+    GCUPS (sum of p*t over the kernel time, wfa_gpu.cu:130-133).  The
+    capacity starts at the reference's wf_length and doubles while a pair
+    overflows (the reference writes past its ring instead)."""
+    out = []
+    for n, L, pct, w_ref in REF_GRID:
+        if args.pairs:
+            n = min(n, args.pairs)
+        ds = tsu.Dataset.generate(n, seed=SEED + 100 + 10 * pct + L, len_min=L, err_min=pct / 100.0, edit_mode=1)
+        b = a.batch(ds)
+        W = w_ref
+        while True:
+            sc, _ = a.align_plain(b, W, raise_on_error=False)
+            if not (sc == tsu.TSU_EKOVERFLOW).any():
+                break
+            W *= 2
+        kms = []
+        for _ in range(max(args.steps, 1)):
+            sc, t = a.align_plain(b, W)
+            kms.append(t["kernel_ms"])
+        kms = float(np.median(kms))
+        res = {"pairs": n, "length": L, "error_pct": pct, "edits": "substitutions", "wf_length_reference": w_ref,
+               "wf_length": W, "kernel_ms": kms, "alignments_per_s": n / (kms / 1e3),
+               "gcups": n * L * L / (kms / 1e3) / 1e9}
+        if not args.no_cpu:
+            m = min(n, 20_000 if L <= 1024 else 400)
+            cpu, want, _ = cpu_baseline(lambda i0, k: [ds.pair(i) for i in range(i0, i0 + k)], m, target_cpu_s=2.0)
+            res["cpu_baseline"] = cpu
+            res["bit_exact_sample"] = bool((want == sc[:len(want)]).all())
+        log(f"refgrid {L} bp {pct}%: {res}")
+        out.append(res)
+        b.close()
+        ds.close()
+    best = max(out, key=lambda r: r["gcups"])
+    return {"configs": out, "best_gcups": best["gcups"], "best_config": f"{best['length']} bp {best['error_pct']}%",
+            "reference_published_peak_gcups": REF_PEAK_GCUPS,
+            "vs_reference_peak": best["gcups"] / REF_PEAK_GCUPS}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--form", default="plain", choices=["plain", "batched", "cigar", "served"])
+    ap.add_argument("--form", default="plain", choices=["plain", "batched", "cigar", "served", "refgrid"])
     ap.add_argument("--workload", default="cfg2", choices=sorted(WORKLOADS))
     ap.add_argument("--pairs", type=int, default=0, help="override the pair count (testing only)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
@@ -202,6 +254,12 @@ def main():
         ms = a.calibrate_copy(args.calibrate_copy)
         print(json.dumps({"copy_bytes": args.calibrate_copy, "kernel_ms": ms,
                           "GB_per_s": 2 * args.calibrate_copy / (ms / 1e3) / 1e9}), flush=True)
+        return
+    if args.form == "refgrid":
+        a = tsu.Aligner(*PEN)
+        res = run_refgrid(a, tsu, args)
+        print(json.dumps({"metric": "equiv. GCUPS on the reference's benchmark grid (plain form, 1 GPU)",
+                          "form": "refgrid", "device": a.device_info()["name"], **res}), flush=True)
         return
     if args.form == "served":
         a = tsu.Aligner(*PEN)
