@@ -123,6 +123,17 @@ def run_served(a, tsu, args, cpu_sample):
     ref, t = a.align_batched(b)
     ref, t = a.align_batched(b)
     cap_pairs = total / (t["wall_ms"] / 1e3)
+    plain = None
+    if args.compare_plain:
+        # the plain form on the same pairs: one launch, capacity for the
+        # score bound of the longest, most divergent request pair
+        x, o, e = PEN
+        W = 2 * ((int(round(0.15 * 10_000)) * (o + e) - o) // e) + 3
+        sc_p, tp = a.align_plain(b, W)
+        sc_p, tp = a.align_plain(b, W)
+        plain = {"alignments_per_s": total / (tp["wall_ms"] / 1e3), "kernel_ms": tp["kernel_ms"], "wf_length": W,
+                 "same_scores": bool((sc_p == ref).all())}
+        log(f"served: plain form on the same pairs {plain}")
     b.close()
     log(f"served: batched capacity {cap_pairs:.0f} pairs/s")
     out = []
@@ -168,7 +179,11 @@ def run_served(a, tsu, args, cpu_sample):
                "same_as_batched": exact}
         log(f"served {frac:.1f}: {res}")
         out.append(res)
-    return {"capacity_pairs_per_s": cap_pairs, "rates": out}
+    res = {"capacity_pairs_per_s": cap_pairs, "rates": out}
+    if plain:
+        plain["served_speedup_at_0.9"] = out[-1]["throughput_pairs_per_s"] / plain["alignments_per_s"]
+        res["plain_same_pairs"] = plain
+    return res
 
 
 # The reference's own benchmark grid (run_h100-a100.sh): generate_dataset N L E
@@ -234,8 +249,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-autotune", action="store_true", help="batched: use the default shapes")
     ap.add_argument("--tune-sample", type=int, default=4096, help="batched: autotuner sample per bucket")
-    ap.add_argument("--compare-plain", action="store_true", help="batched: also time the plain form")
-    ap.add_argument("--requests", type=int, default=300, help="served: requests per rate")
+    ap.add_argument("--compare-plain", action="store_true", help="batched, served: also time the plain form")
+    ap.add_argument("--requests", type=int, default=3000, help="served: requests per rate")
     ap.add_argument("--serve-budget", type=int, default=16384, help="served: HBM budget in MiB")
     ap.add_argument("--inflight", type=int, default=4, help="served: launches in flight")
     ap.add_argument("--batch-pairs", type=int, default=2048, help="served: pairs merged per launch")
