@@ -356,6 +356,7 @@ static void use_register_kernel(const tsu_ctx* ctx, int32_t W, int32_t max_len, 
     if (int c = tsu_host::reg_chunks(ctx, W)) {
         r.reg_c = c;
         r.nw = 1;
+        r.lring = max_len <= 16000;  // doubled offsets fit the int16 ring
     } else if (int nwl = tsu_host::ml_waves(ctx, W, max_len)) {
         r.reg_c = 16;
         r.nw = nwl;

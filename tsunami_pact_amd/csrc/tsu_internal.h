@@ -67,6 +67,7 @@ struct Shape {
     bool queue;      // persistent launch fed by a work counter
     bool fixed_242 = false;  // scaled penalties are (2, 3, 1): compile-time ring
     int reg_c = 0;           // > 0: register-resident kernel, reg_c chunks of 64 diagonals per wave (nw waves)
+    bool lring = false;      // one wave: the older M scores in an int16 LDS ring (sequences up to 16000 bases)
 };
 
 // LDS bytes a launch of this shape needs.

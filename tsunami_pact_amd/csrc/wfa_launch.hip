@@ -42,6 +42,8 @@ int64_t lds_bytes(const Shape& sh, const KArgs& a) {
     if (sh.reg_c > 0) {
         // sequences + misc (one wave), + the edge exchange area (several waves
         // of 8 chunks), + the exchange area and the int16 M ring (16 chunks)
+        // (one wave with the int16 ring: 4 misc words + 3 slots of 64*reg_c + 2 entries)
+        if (sh.nw == 1 && sh.lring) return 4ll * (2 * a.seq_words + 4) + ((3 * (64 * sh.reg_c + 2) * 2 + 3) & ~3);
         const int ex = sh.nw == 1 ? 16 : sh.reg_c == 16 ? regml_exchange_words(sh.nw) : regmw_exchange_words(sh.nw);
         return 4ll * (2 * a.seq_words + ex);
     }

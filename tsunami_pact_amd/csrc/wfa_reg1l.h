@@ -1,0 +1,207 @@
+// wfa_reg1l.h -- single-wave register kernel with the older M scores in an
+// int16 LDS ring.
+//
+// The same one-wave-per-pair kernel as wfa_reg.h (windows of 64..512
+// diagonals, offsets kept doubled, chunk activity masks, fused extend), for
+// the scaled penalties 2/3/1, with the M ring split as in the 16-chunk
+// multi-wave kernel (wfa_regml.h): a lane keeps M of the two newest scores
+// (two VGPRs per chunk, by the parity of s) and I and D; M[s-2], once read
+// for the mismatch, goes to an int16 LDS ring of three slots, from which the
+// gap-open values M[s-4][k-1] and M[s-4][k+1] are read at their diagonals.
+// That replaces the four DPP moves per chunk and step that fetch the k-1 /
+// k+1 neighbours and their chunk-edge fills from registers, and halves the
+// state (4 instead of 7 VGPRs per chunk).  One wave owns the ring, and LDS
+// operations of a wave complete in order, so no barrier is needed.
+//
+// Doubled offsets up to 2t must fit int16 (sequences of at most 16000
+// bases; the host uses wfa_reg.h beyond), and NULL is -32768 in the ring, so
+// the kernel gives up past kMlMaxScore scaled scores with TSU_EKOVERFLOW
+// (the same bound as the multi-wave kernel, for the same reason).
+#pragma once
+
+#include "wfa_regml.h"
+
+namespace tsu {
+
+template <int C, bool QUEUE>
+__global__ __launch_bounds__(64) void wfa_reg1l_kernel(KArgs a) {
+    static_assert(C >= 1 && C <= 8, "window of 64..512 diagonals (8-bit chunk fields)");
+    constexpr int32_t NUL = kRegNull;
+    constexpr int W = 64 * C;
+    constexpr int MRS = W + 2;  // int16 entries per ring slot, NULL guards at both ends
+    constexpr uint32_t ALL = (1u << C) - 1u;
+    extern __shared__ __align__(16) uint32_t smem[];
+    const int lane = threadIdx.x;
+    const int lane2 = 2 * lane;
+    uint32_t* sS = smem;  // interleaved pattern / text words
+    int32_t* sMisc = reinterpret_cast<int32_t*>(sS + 2 * a.seq_words);
+    int16_t* MR = reinterpret_cast<int16_t*>(sMisc + 4);
+    const bool lds_at_zero = (uint32_t)(uintptr_t)sS == 0u;
+
+    for (int iter = 0;; ++iter) {
+        int idx;
+        if constexpr (QUEUE) {
+            if (lane == 0) sMisc[0] = atomicAdd(a.queue, 1);
+            __syncthreads();
+            idx = __builtin_amdgcn_readfirstlane(sMisc[0]);
+            __syncthreads();
+        } else {
+            if (iter > 0) break;
+            idx = blockIdx.x;
+        }
+        if (idx >= a.n) break;
+        const int pi = a.order ? __builtin_amdgcn_readfirstlane(a.order[idx]) : idx;
+        const KPair pr = a.pairs[pi];
+        const int p = __builtin_amdgcn_readfirstlane(pr.p_len);
+        const int t = __builtin_amdgcn_readfirstlane(pr.t_len);
+        const int pw = (p + 15) >> 4, tw = (t + 15) >> 4;
+        for (int i = lane; i < a.seq_words; i += 64) {
+            sS[2 * i] = i < pw ? a.words[pr.p_off + i] : 0u;
+            sS[2 * i + 1] = i < tw ? a.words[pr.t_off + i] : 0u;
+        }
+        uint32_t* mr32 = reinterpret_cast<uint32_t*>(MR);
+        for (int i = lane; i < (3 * MRS + 1) / 2; i += 64) mr32[i] = 0x80008000u;  // int16 NULLs
+        __syncthreads();
+
+        int kmin = max(-(W / 2), -p - 1);
+        int kmax = kmin + W - 1;
+        if (kmax > t + 1) { kmax = t + 1; kmin = max(kmax - W + 1, -p - 1); }
+        const int tp = t - p;
+        const int ctp = (tp - kmin) >= 0 ? (tp - kmin) >> 6 : -1;
+        const int ltp = (tp - kmin) & 63;
+        const int64_t bound_u = 2ll * min(p, t) + (p != t ? 3ll + (int64_t)abs(p - t) : 0);
+        const int s_bound = (int)min<int64_t>(bound_u, 0x3fffffff);
+
+        int32_t Mr[2][C], Ir[C], Dr[C];
+#pragma unroll
+        for (int c = 0; c < C; ++c) {
+            Mr[0][c] = NUL;
+            Mr[1][c] = NUL;
+            Ir[c] = NUL;
+            Dr[c] = NUL;
+        }
+        // step masks of scores s-1 .. s-5 (fields as in wfa_reg.h)
+        uint32_t F1 = 0, F2 = 0, F3 = 0, F4 = 0, F5 = 0;
+
+        // score 0: M[0][0] = 0, extended (every lane computes it)
+        const int ext0 = lcp_interleaved(sS, 0, 0, min(p, t));
+        {
+            const int c0 = (0 - kmin) >> 6, l0 = (0 - kmin) & 63;
+#pragma unroll
+            for (int c = 0; c < C; ++c)
+                if (c == c0) Mr[0][c] = lane == l0 ? 2 * ext0 : NUL;
+            F1 = (1u << c0) | (l0 == 0 ? 1u << (8 + c0) : 0u) | (l0 == 63 ? 1u << (16 + c0) : 0u);
+        }
+        int s = 0;
+        int l3 = 0;  // s mod 3
+
+        // One score step; PAR = s & 1 selects the register slot holding
+        // M[s-2], which receives M[s].  Returns 0 to go on, the score when
+        // the end cell is reached, or a negative status.
+        auto step = [&](auto PC) __attribute__((always_inline)) -> int {
+            constexpr int PAR = decltype(PC)::value;
+            ++s;
+            if (s > s_bound) return TSU_EINTERNAL;
+            if (s > kMlMaxScore) return TSU_EKOVERFLOW;
+            l3 = l3 == 2 ? 0 : l3 + 1;
+            const int rd = l3 == 0 ? 2 : l3 - 1;   // ring slot of s-4
+            const int wrs = l3 == 2 ? 0 : l3 + 1;  // ring slot of s-2 (formerly s-5)
+            // an opaque zero per step keeps the per-chunk bounds from being
+            // hoisted out of the score loop into VGPRs
+            int zero_s;
+            asm volatile("s_mov_b32 %0, 0" : "=s"(zero_s));
+            // chunk c computes when it may hold a non-NULL source (M of s-2
+            // and s-4, I/D of s-1), a ring value to overwrite (s-5), or when
+            // a neighbour chunk's edge lane feeds it
+            const uint32_t src = F1 | F4;
+            const uint32_t act = ((src | F2 | F5) | ((src >> 16) << 1) | (((src >> 8) & 0xffu) >> 1)) & ALL;
+            const int16_t* mrd = MR + rd * MRS + lane;       // entry of k-1
+            int16_t* mwr = MR + wrs * MRS + lane + 1;        // entry of k
+            uint32_t acc = 0;
+            uint64_t oob = 0;
+            int32_t mend = NUL;
+            int32_t carryI = NUL;  // old I of the previous chunk's lane 63
+#pragma unroll
+            for (int c = 0; c < C; ++c) {
+                if (act & (1u << c)) {
+                    const int k2 = 2 * (kmin + zero_s + 64 * c) + lane2;
+                    const int mt2 = min(k2 + 2 * (p + 1), 2 * (t + 1));
+                    const int32_t mx = Mr[PAR][c];
+                    mwr[64 * c] = (int16_t)max(mx, -32768);
+                    const int32_t mol = mrd[64 * c];
+                    const int32_t mor = mrd[64 * c + 2];
+                    const int32_t iel = dpp_shr1(carryI, Ir[c]);
+                    const int32_t der = dpp_shl1(c < C - 1 ? dpp_rol1(Dr[c < C - 1 ? c + 1 : c]) : NUL, Dr[c]);
+                    carryI = __builtin_amdgcn_readlane(Ir[c], 63);
+                    const int32_t I = max(mol, iel) + 2;
+                    const int32_t D = max(mor, der);
+                    int32_t M = max(max(D, mx + 2), I);
+                    const bool valid = (uint32_t)M < (uint32_t)mt2;
+                    M = valid ? M : NUL;
+                    const int lim2 = mt2 - 2 - M;
+                    if (valid && lim2 > 0) M += extend2((uint32_t)(M - k2), (uint32_t)M, lim2);
+                    Mr[PAR][c] = M;
+                    Ir[c] = I;
+                    Dr[c] = D;
+                    const uint64_t b = vote(max(max(M, I), D) >= 0);
+                    uint32_t fl = (b ? 2u : 0u) | ((uint32_t)(b & 1ull) << 9) | ((uint32_t)(b >> 63) << 17);
+                    asm volatile("" : "+s"(fl));
+                    acc |= fl << c;
+                    if (c == ctp) mend = __builtin_amdgcn_readlane(M, ltp);
+                    oob |= vote(max(I, D) >= mt2);
+                } else {
+                    carryI = NUL;  // an idle chunk holds only NULL
+                }
+            }
+            uint32_t Fs = acc >> 1;
+            if (oob) {
+                // exact trim (wfa_gpu.cu:633-719): NULL I/D cells outside
+                // [first, last] in-bounds cell, then rebuild the step mask
+                auto trim = [&](int32_t (&R)[C]) __attribute__((always_inline)) {
+                    int f = 0x3fffffff, l = -1;
+#pragma unroll
+                    for (int c = 0; c < C; ++c) {
+                        const int k2 = 2 * (kmin + 64 * c) + lane2;
+                        const int mt2 = min(k2 + 2 * (p + 1), 2 * (t + 1));
+                        const uint64_t inb = vote((uint32_t)R[c] < (uint32_t)mt2);
+                        if (inb) {
+                            f = min(f, 64 * c + (int)__builtin_ctzll(inb));
+                            l = max(l, 64 * c + 63 - (int)__builtin_clzll(inb));
+                        }
+                    }
+#pragma unroll
+                    for (int c = 0; c < C; ++c) {
+                        const int w = 64 * c + lane;
+                        if (w < f || w > l) R[c] = NUL;
+                    }
+                };
+                trim(Ir);
+                trim(Dr);
+                uint32_t m2 = 0;
+#pragma unroll
+                for (int c = 0; c < C; ++c) {
+                    const uint64_t b = vote(max(max(Mr[PAR][c], Ir[c]), Dr[c]) >= 0);
+                    m2 |= ((b ? 2u : 0u) | ((uint32_t)(b & 1ull) << 9) | ((uint32_t)(b >> 63) << 17)) << c;
+                }
+                Fs = m2 >> 1;
+            }
+            F5 = F4; F4 = F3; F3 = F2; F2 = F1; F1 = Fs;
+            if (mend >= 2 * t) return s;
+            // a non-NULL cell at a window edge could extend the range beyond it
+            if ((Fs >> 8) & 1u) return TSU_EKOVERFLOW;
+            if ((Fs >> (16 + C - 1)) & 1u) return TSU_EKOVERFLOW;
+            return 0;
+        };
+
+        int r = (p == t && ext0 == t) ? 1 : 0;  // 1: score 0 (identical sequences)
+        while (r == 0) {
+            r = step(std::integral_constant<int, 1>{});
+            if (r == 0) r = step(std::integral_constant<int, 0>{});
+        }
+        const int score = (r > 0 && s == 0) ? 0 : r;
+        if (lane == 0) a.scores[pi] = !lds_at_zero ? TSU_EINTERNAL : score < 0 ? score : score * a.g;
+        __syncthreads();
+    }
+}
+
+}  // namespace tsu
