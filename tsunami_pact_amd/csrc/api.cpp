@@ -397,30 +397,6 @@ extern "C" int tsu_align_plain(tsu_ctx* ctx, tsu_batch* b, int32_t wf_length, in
     int r;
     if (fits) {
         r = run_shape(ctx, b, sh, W, nullptr, b->n, b->max_len, 0, nullptr, &kms);
-        if (r == TSU_OK && sh.reg_c == 16) {
-            // the 16-chunk kernel stops past its score limit (int16 LDS ring)
-            // with an overflow status: such pairs get the LDS kernel at the
-            // same capacity, so the result depends on the capacity only
-            std::vector<int32_t> tmp((size_t)b->n);
-            HIPCHK(hipMemcpy(tmp.data(), b->d_scores, (size_t)b->n * sizeof(int32_t), hipMemcpyDeviceToHost), "hipMemcpy D2H (scores)");
-            std::vector<int32_t> again;
-            for (int32_t i = 0; i < b->n; ++i)
-                if (tmp[(size_t)i] == TSU_EKOVERFLOW) again.push_back(i);
-            if (!again.empty()) {
-                Shape s2{default_waves(W), b->max_len <= 32000, true, false, true};
-                s2.fixed_242 = sh.fixed_242;
-                KArgs a2 = base_args(ctx);
-                a2.W = W;
-                a2.seq_words = seq_words_for(b->max_len);
-                if (tsu::lds_bytes(s2, a2) > ctx->max_lds) { s2.lds_ring = false; s2.nw = 16; }
-                DevBuf<int32_t> d_order;
-                HIPCHK(d_order.alloc(again.size()), "hipMalloc (order)");
-                HIPCHK(hipMemcpy(d_order.p, again.data(), again.size() * sizeof(int32_t), hipMemcpyHostToDevice), "hipMemcpy (order)");
-                double kms2 = 0;
-                r = run_shape(ctx, b, s2, W, d_order.p, (int32_t)again.size(), b->max_len, 0, nullptr, &kms2);
-                kms += kms2;
-            }
-        }
     } else {
         // ring too wide for LDS: same one-workgroup-per-pair work, ring in HBM,
         // resident workgroups fed from the queue
@@ -433,10 +409,36 @@ extern "C" int tsu_align_plain(tsu_ctx* ctx, tsu_batch* b, int32_t wf_length, in
     const double w1 = now_ms();
     HIPCHK(hipMemcpy(scores, b->d_scores, (size_t)b->n * sizeof(int32_t), hipMemcpyDeviceToHost), "hipMemcpy D2H (scores)");
     const double w2 = now_ms();
+    int launches = 1;
+    if (sh.reg_c == 16 || sh.lring) {
+        // the kernels with an int16 LDS ring stop past their score limit with
+        // an overflow status: such pairs get the LDS kernel at the same
+        // capacity, so the result depends on the capacity only
+        std::vector<int32_t> again;
+        for (int32_t i = 0; i < b->n; ++i)
+            if (scores[i] == TSU_EKOVERFLOW) again.push_back(i);
+        if (!again.empty()) {
+            Shape s2{default_waves(W), b->max_len <= 32000, true, false, true};
+            s2.fixed_242 = sh.fixed_242;
+            KArgs a2 = base_args(ctx);
+            a2.W = W;
+            a2.seq_words = seq_words_for(b->max_len);
+            if (tsu::lds_bytes(s2, a2) > ctx->max_lds) { s2.lds_ring = false; s2.nw = 16; }
+            DevBuf<int32_t> d_order;
+            HIPCHK(d_order.alloc(again.size()), "hipMalloc (order)");
+            HIPCHK(hipMemcpy(d_order.p, again.data(), again.size() * sizeof(int32_t), hipMemcpyHostToDevice), "hipMemcpy (order)");
+            double kms2 = 0;
+            r = run_shape(ctx, b, s2, W, d_order.p, (int32_t)again.size(), b->max_len, 0, nullptr, &kms2);
+            if (r != TSU_OK) return r;
+            kms += kms2;
+            ++launches;
+            HIPCHK(hipMemcpy(scores, b->d_scores, (size_t)b->n * sizeof(int32_t), hipMemcpyDeviceToHost), "hipMemcpy D2H (scores)");
+        }
+    }
     if (t) {
         std::memset(t, 0, sizeof(*t));
         t->kernel_ms = kms; t->wall_ms = w1 - w0; t->d2h_ms = w2 - w1;
-        t->pairs = b->n; t->cells = b->cells; t->launches = 1;
+        t->pairs = b->n; t->cells = b->cells; t->launches = launches;
     }
     for (int32_t i = 0; i < b->n; ++i) {
         if (scores[i] == TSU_EKOVERFLOW) return fail(ctx, TSU_EKOVERFLOW, "pair " + std::to_string(i) + ": diagonal range exceeded wf_length " + std::to_string(wf_length));
