@@ -54,6 +54,15 @@ def lib() -> ctypes.CDLL:
                                     ctypes.c_int, P(ctypes.c_int32), ctypes.c_int, ctypes.c_char_p,
                                     P(ctypes.c_int64), P(ctypes.c_int32), P(Counters)]
         l.orc_max_threads.restype = ctypes.c_int
+        l.orc_gotoh2p.restype = ctypes.c_int64
+        l.orc_gotoh2p.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_char_p, ctypes.c_int] + [ctypes.c_int] * 5
+        l.orc_wfa2p.restype = ctypes.c_int
+        l.orc_wfa2p.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_char_p, ctypes.c_int] + [ctypes.c_int] * 5 + \
+            [P(ctypes.c_int64)]
+        l.orc_wfa2p_batch.restype = ctypes.c_int
+        l.orc_wfa2p_batch.argtypes = [ctypes.c_char_p, P(ctypes.c_int64), P(ctypes.c_int32), P(ctypes.c_int64),
+                                      P(ctypes.c_int32), ctypes.c_int] + [ctypes.c_int] * 6 + \
+            [P(ctypes.c_int32), P(ctypes.c_int64)]
         _lib = l
     return _lib
 
@@ -74,6 +83,45 @@ def wfa(p: str, t: str, x: int = 4, o: int = 6, e: int = 2, cigar: bool = False)
         raise RuntimeError(f"orc_wfa failed: {r}")
     ops = buf.raw[:n.value].decode() if cigar else None
     return r, ops, c.as_dict()
+
+
+def gotoh2p(p: str, t: str, x: int, o1: int, e1: int, o2: int, e2: int) -> int:
+    """2-piece Gotoh DP score (gap of length L costs min(o1 + L e1, o2 + L e2))."""
+    return int(lib().orc_gotoh2p(p.encode(), len(p), t.encode(), len(t), x, o1, e1, o2, e2))
+
+
+def wfa2p(p: str, t: str, x: int, o1: int, e1: int, o2: int, e2: int) -> Tuple[int, int]:
+    """Gap-affine 2-piece WFA score and the number of computed cells."""
+    c = ctypes.c_int64(0)
+    r = lib().orc_wfa2p(p.encode(), len(p), t.encode(), len(t), x, o1, e1, o2, e2, ctypes.byref(c))
+    if r < 0:
+        raise RuntimeError(f"orc_wfa2p failed: {r}")
+    return r, c.value
+
+
+def wfa2p_batch(pairs: Sequence[Tuple[str, str]], x: int, o1: int, e1: int, o2: int, e2: int,
+                threads: int = 0) -> Tuple[np.ndarray, int]:
+    """Scores (np.int32) of many pairs under 2-piece penalties, and total cells."""
+    n = len(pairs)
+    chunks, p_off, p_len, t_off, t_len = [], [], [], [], []
+    pos = 0
+    for p, t in pairs:
+        pb, tb = p.encode(), t.encode()
+        p_off.append(pos); p_len.append(len(pb)); chunks.append(pb); pos += len(pb)
+        t_off.append(pos); t_len.append(len(tb)); chunks.append(tb); pos += len(tb)
+    blob = b"".join(chunks) or b"\0"
+    arr = lambda v, dt: np.ascontiguousarray(np.array(v or [0], dtype=dt))
+    po, pl, to, tl = arr(p_off, np.int64), arr(p_len, np.int32), arr(t_off, np.int64), arr(t_len, np.int32)
+    scores = np.zeros(max(n, 1), np.int32)
+    cells = ctypes.c_int64(0)
+    P = ctypes.POINTER
+    failed = lib().orc_wfa2p_batch(blob, po.ctypes.data_as(P(ctypes.c_int64)), pl.ctypes.data_as(P(ctypes.c_int32)),
+                                   to.ctypes.data_as(P(ctypes.c_int64)), tl.ctypes.data_as(P(ctypes.c_int32)), n,
+                                   x, o1, e1, o2, e2, threads, scores.ctypes.data_as(P(ctypes.c_int32)),
+                                   ctypes.byref(cells))
+    if failed:
+        raise RuntimeError(f"orc_wfa2p_batch: {failed} pairs failed")
+    return scores[:n], cells.value
 
 
 def rle(ops: str) -> str:

@@ -1,6 +1,8 @@
 """Pins the CPU oracle (oracle/wfa_oracle.c) before it is trusted.
 
-Checks, each chosen so that a plausible mistake fails one of them:
+Checks, each chosen so that a plausible mistake fails one of them (for the
+gap-affine 2-piece oracle too: an explicit-run-length enumeration, closed
+forms, the reduction to one piece, WFA == 2-piece Gotoh):
   - the WFA2-lib README example (published score 24, CIGAR 3M1X4M1D7M1I9M1X6M);
   - Gotoh == exhaustive enumeration of every alignment on tiny inputs;
   - closed forms (identity, one substitution, one gap of length L, empty);
@@ -156,3 +158,76 @@ def test_rescore_catches_a_planted_mistake(oracle):
     swapped = ops.replace("I", "Q", 1).replace("D", "I", 1).replace("Q", "D", 1)
     with pytest.raises(Exception):
         oracle.rescore(swapped, p, t, 4, 6, 2)
+
+
+# ---------------------------------------------------------------------------
+# gap-affine 2-piece (oracle/wfa2p_oracle.c)
+# ---------------------------------------------------------------------------
+PENALTIES_2P = [(4, 6, 2, 24, 1), (4, 6, 2, 12, 1), (2, 3, 2, 8, 1), (5, 1, 3, 10, 1), (3, 2, 2, 2, 2),
+                (1, 0, 1, 4, 1), (6, 4, 3, 9, 1)]
+
+
+def brute2p(p, t, x, o1, e1, o2, e2):
+    """Minimum over alignments written as runs: each gap run of length L costs
+    min(o1 + L e1, o2 + L e2) (explicit run lengths, no gap state machine)."""
+    g = lambda L: min(o1 + L * e1, o2 + L * e2)
+
+    @functools.lru_cache(None)
+    def f(i, j, after_gap):  # after_gap: 0 after a match/mismatch or at start, 1 after an I run, 2 after a D run
+        if i == len(p) and j == len(t):
+            return 0
+        best = 1 << 60
+        if i < len(p) and j < len(t):
+            best = min(best, (0 if p[i] == t[j] else x) + f(i + 1, j + 1, 0))
+        if after_gap != 1:
+            for L in range(1, len(t) - j + 1):
+                best = min(best, g(L) + f(i, j + L, 1))
+        if after_gap != 2:
+            for L in range(1, len(p) - i + 1):
+                best = min(best, g(L) + f(i + L, j, 2))
+        return best
+    return f(0, 0, 0)
+
+
+@pytest.mark.parametrize("pen", PENALTIES_2P)
+def test_2p_gotoh_and_wfa_equal_brute_force(oracle, pen):
+    rng = random.Random(sum(pen) * 7919)
+    for _ in range(250):
+        p = rand_seq(rng, rng.randint(0, 7))
+        t = rand_seq(rng, rng.randint(0, 7))
+        b = brute2p(p, t, *pen)
+        assert oracle.gotoh2p(p, t, *pen) == b, (p, t, pen)
+        assert oracle.wfa2p(p, t, *pen)[0] == b, (p, t, pen)
+
+
+def test_2p_closed_forms_and_reductions(oracle):
+    rng = random.Random(11)
+    for x, o1, e1, o2, e2 in PENALTIES_2P:
+        pen = (x, o1, e1, o2, e2)
+        s = rand_seq(rng, 80)
+        assert oracle.wfa2p(s, s, *pen)[0] == 0
+        assert oracle.wfa2p("", "", *pen)[0] == 0
+        for L in (1, 3, 9, 40):
+            gap = min(o1 + L * e1, o2 + L * e2)
+            assert oracle.wfa2p("", s[:L], *pen)[0] == gap
+            # one extra run of length L inside an otherwise identical pair: any
+            # alignment needs gap runs summing to L, which cost at least g(L)
+            assert oracle.wfa2p(s[:20] + "A" * L + s[20:], s, *pen)[0] == gap
+            assert oracle.wfa2p(s, s[:20] + "C" * L + s[20:], *pen)[0] == gap
+            # a single gap piece (both pieces equal) reduces to gap-affine
+            a = rand_seq(rng, 50)
+            b = a[:10] + rand_seq(rng, L) + a[10 + L // 2:]
+            assert oracle.wfa2p(a, b, x, o1, e1, o1, e1)[0] == oracle.gotoh(a, b, x, o1, e1)
+        # the second piece can only lower the score
+        a, b = rand_seq(rng, 60), rand_seq(rng, 70)
+        assert oracle.wfa2p(a, b, *pen)[0] <= oracle.gotoh(a, b, x, o1, e1)
+        assert oracle.wfa2p(a, b, *pen)[0] == oracle.wfa2p(b, a, *pen)[0]
+
+
+def test_2p_wfa_equals_gotoh_on_longer_pairs(oracle, tsu):
+    ds = tsu.Dataset.generate(120, seed=0x2F, len_min=200, len_max=1500, err_min=0.02, err_max=0.2)
+    pairs = ds.pairs()
+    for pen in ((4, 6, 2, 24, 1), (5, 1, 3, 10, 1)):
+        got, _ = oracle.wfa2p_batch(pairs, *pen)
+        want = [oracle.gotoh2p(p, t, *pen) for p, t in pairs[:40]]
+        assert list(got[:40]) == want, pen
