@@ -83,6 +83,15 @@ typedef struct {
  * Penalties must satisfy mismatch > 0, gap_opening >= 0, gap_extension > 0. */
 int tsu_create(tsu_ctx** out, int device, int32_t mismatch, int32_t gap_opening,
                int32_t gap_extension);
+/* Gap-affine 2-piece penalties: a gap of length L costs
+ * min(gap_opening1 + L*gap_extension1, gap_opening2 + L*gap_extension2).
+ * The reference declares this model (common.h:52-58,80-87: gap_open2,
+ * gap_ext2 and the I2/D2 wavefronts) but never computes it, so this entry
+ * point has no counterpart there.  Such a context runs the plain and batched
+ * forms (one wave per pair, all five wavefronts in LDS); tsu_align_cigar and
+ * tsu_serve_open return TSU_EINVAL for it. */
+int tsu_create_2p(tsu_ctx** out, int device, int32_t mismatch, int32_t gap_opening1,
+                  int32_t gap_extension1, int32_t gap_opening2, int32_t gap_extension2);
 void tsu_destroy(tsu_ctx* ctx);
 const char* tsu_last_error(const tsu_ctx* ctx);
 const char* tsu_strerror(int status);

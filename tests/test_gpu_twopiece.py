@@ -1,0 +1,65 @@
+"""GPU parity for gap-affine 2-piece penalties (a gap of length L costs
+min(o1 + L e1, o2 + L e2)): libtsunami_hip.so against the 2-piece oracle
+(oracle/wfa2p_oracle.c, itself pinned by a 2-piece Gotoh DP), bit-exact."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+SEED = 0x5453554E
+PEN_2P = [(4, 6, 2, 24, 1), (4, 6, 2, 12, 2), (5, 1, 3, 10, 1)]
+
+
+def say(msg):
+    print(f"\n[gpu] {msg}", flush=True)
+
+
+def pairs_for(tsu, seed):
+    ds = tsu.Dataset.generate(500, seed=seed, len_min=20, len_max=400, err_min=0.0, err_max=0.2)
+    pairs = ds.pairs()
+    rng = np.random.default_rng(seed)
+    for _ in range(20):  # unrelated random pairs: long gaps, the second piece wins
+        pairs.append(("".join(rng.choice(list("ACGT"), int(rng.integers(1, 300)))),
+                      "".join(rng.choice(list("ACGT"), int(rng.integers(1, 300))))))
+    pairs += [("", ""), ("A", ""), ("", "ACGT" * 10), ("A" * 200, "C" * 30), ("ACGT" * 50, "ACGT" * 50),
+              ("ACGT" * 30 + "T" * 60 + "ACGT" * 30, "ACGT" * 60)]
+    return pairs
+
+
+@pytest.mark.parametrize("pen", PEN_2P)
+def test_two_piece_plain_and_batched(tsu, oracle, pen):
+    """Plain (capacity 1000 covers every pair here) and batched forms equal the
+    oracle; a narrow capacity reports overflow for the wide pairs only."""
+    a = tsu.Aligner(pen[0], pen[1], pen[2], gap_opening2=pen[3], gap_extension2=pen[4])
+    try:
+        pairs = pairs_for(tsu, SEED + 20 + sum(pen))
+        want, _ = oracle.wfa2p_batch(pairs, *pen)
+        b = a.batch(pairs)
+        say(f"2-piece {pen} plain W=1000")
+        got, _ = a.align_plain(b, 1000)
+        assert (got == want).all(), pen
+        say(f"2-piece {pen} batched")
+        got, _ = a.align_batched(b)
+        assert (got == want).all(), pen
+        say(f"2-piece {pen} plain W=40")
+        got, _ = a.align_plain(b, 40, raise_on_error=False)
+        ok = got != tsu.TSU_EKOVERFLOW
+        assert ok.sum() > 0 and (~ok).sum() > 0
+        assert (got[ok] == want[ok]).all(), pen
+    finally:
+        a.close()
+
+
+def test_two_piece_forms_without_support_refuse(tsu):
+    """The CIGAR and served forms refuse a 2-piece context with TSU_EINVAL."""
+    a = tsu.Aligner(4, 6, 2, gap_opening2=24, gap_extension2=1)
+    try:
+        b = a.batch([("ACGT", "AGGT")])
+        say("2-piece cigar / serve refusal")
+        with pytest.raises(tsu.TsuError) as ei:
+            a.align_cigar(b)
+        assert ei.value.status == tsu.TSU_EINVAL
+        with pytest.raises(tsu.TsuError) as ei:
+            tsu.Server(a)
+        assert ei.value.status == tsu.TSU_EINVAL
+    finally:
+        a.close()

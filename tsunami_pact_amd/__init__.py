@@ -38,7 +38,7 @@ OPS = "MXID"
 
 # Every symbol include/tsunami_wfa.h declares.
 EXPORTS = (
-    "tsu_create", "tsu_destroy", "tsu_last_error", "tsu_strerror", "tsu_device_info",
+    "tsu_create", "tsu_create_2p", "tsu_destroy", "tsu_last_error", "tsu_strerror", "tsu_device_info",
     "tsu_packed_words", "tsu_pack",
     "tsu_dataset_read", "tsu_dataset_write", "tsu_dataset_generate", "tsu_dataset_from_strings",
     "tsu_dataset_size", "tsu_dataset_pair", "tsu_dataset_lengths", "tsu_dataset_header", "tsu_dataset_free",
@@ -113,6 +113,7 @@ def library() -> ctypes.CDLL:
     P = ctypes.POINTER
     sig = {
         "tsu_create": (ctypes.c_int, [P(vp), ctypes.c_int, i32, i32, i32]),
+        "tsu_create_2p": (ctypes.c_int, [P(vp), ctypes.c_int, i32, i32, i32, i32, i32]),
         "tsu_destroy": (None, [vp]),
         "tsu_last_error": (ctypes.c_char_p, [vp]),
         "tsu_strerror": (ctypes.c_char_p, [ctypes.c_int]),
@@ -323,14 +324,22 @@ class Batch:
 class Aligner:
     """One device context with fixed gap-affine penalties (x, o, e)."""
 
-    def __init__(self, mismatch: int = 4, gap_opening: int = 6, gap_extension: int = 2, device: int = 0):
+    def __init__(self, mismatch: int = 4, gap_opening: int = 6, gap_extension: int = 2, device: int = 0,
+                 gap_opening2: Optional[int] = None, gap_extension2: Optional[int] = None):
+        """Gap-affine penalties; with gap_opening2 / gap_extension2 the 2-piece
+        model, where a gap of length L costs min(o1 + L e1, o2 + L e2)."""
         self._lib = library()
         h = ctypes.c_void_p()
-        r = self._lib.tsu_create(ctypes.byref(h), device, mismatch, gap_opening, gap_extension)
+        if gap_opening2 is None and gap_extension2 is None:
+            r = self._lib.tsu_create(ctypes.byref(h), device, mismatch, gap_opening, gap_extension)
+            self.penalties = (mismatch, gap_opening, gap_extension)
+        else:
+            r = self._lib.tsu_create_2p(ctypes.byref(h), device, mismatch, gap_opening, gap_extension,
+                                        gap_opening2, gap_extension2)
+            self.penalties = (mismatch, gap_opening, gap_extension, gap_opening2, gap_extension2)
         if r != TSU_OK:
             raise TsuError(r, f"tsu_create: {strerror(r)}")
         self._h = h
-        self.penalties = (mismatch, gap_opening, gap_extension)
         self._batches = weakref.WeakSet()
 
     def _check(self, r: int, what: str) -> None:

@@ -72,6 +72,8 @@ struct DevBuf {
 // |k| <= (s - o) / e + 1 on each side (+ the union's +-1).
 int32_t capacity_for_score(const tsu_ctx* ctx, int64_t s_scaled) {
     int64_t half = (s_scaled - ctx->os) / ctx->es + 2;
+    // with two gap pieces the range grows with the faster-growing piece
+    if (ctx->two_piece) half = std::max<int64_t>(half, (s_scaled - ctx->o2s) / ctx->e2s + 2);
     if (half < 4) half = 4;
     int64_t w = 2 * half + 1;
     return (int32_t)std::min<int64_t>(w, 1 << 26);
@@ -118,6 +120,26 @@ extern "C" int tsu_create(tsu_ctx** out, int device, int32_t mismatch, int32_t g
         return TSU_EHIP;
     }
     *out = ctx;
+    return TSU_OK;
+}
+
+extern "C" int tsu_create_2p(tsu_ctx** out, int device, int32_t mismatch, int32_t gap_opening1, int32_t gap_extension1,
+                             int32_t gap_opening2, int32_t gap_extension2) {
+    if (!out) return TSU_EINVAL;
+    *out = nullptr;
+    if (gap_opening2 < 0 || gap_extension2 <= 0 || gap_opening2 > (1 << 20) || gap_extension2 > (1 << 20)) return TSU_EINVAL;
+    int r = tsu_create(out, device, mismatch, gap_opening1, gap_extension1);
+    if (r != TSU_OK) return r;
+    tsu_ctx* ctx = *out;
+    ctx->two_piece = true;
+    ctx->o2 = gap_opening2; ctx->e2 = gap_extension2;
+    ctx->g = (int32_t)gcd64(gcd64(gcd64(gcd64(mismatch, gap_opening1), gap_extension1), gap_opening2), gap_extension2);
+    ctx->xs = mismatch / ctx->g; ctx->os = gap_opening1 / ctx->g; ctx->es = gap_extension1 / ctx->g;
+    ctx->o2s = gap_opening2 / ctx->g; ctx->e2s = gap_extension2 / ctx->g;
+    ctx->rm = std::max(std::max(ctx->xs, ctx->os + ctx->es), ctx->o2s + ctx->e2s) + 1;
+    ctx->ri = ctx->es + 1;
+    ctx->r2 = ctx->e2s + 1;
+    if (ctx->rm > 4096) { tsu_destroy(ctx); *out = nullptr; return TSU_EINVAL; }
     return TSU_OK;
 }
 
@@ -235,6 +257,7 @@ KArgs tsu_host::base_args(const tsu_ctx* ctx) {
     std::memset(&a, 0, sizeof(a));
     a.x = ctx->xs; a.o = ctx->os; a.e = ctx->es; a.g = ctx->g;
     a.rm = ctx->rm; a.ri = ctx->ri;
+    a.o2 = ctx->o2s; a.e2 = ctx->e2s; a.r2 = ctx->r2;
     return a;
 }
 
@@ -317,7 +340,7 @@ static int run_shape(tsu_ctx* ctx, tsu_batch* b, Shape sh, int32_t W, const int3
     return TSU_OK;
 }
 
-bool tsu_host::fixed_242(const tsu_ctx* ctx) { return ctx->xs == 2 && ctx->os == 3 && ctx->es == 1; }
+bool tsu_host::fixed_242(const tsu_ctx* ctx) { return !ctx->two_piece && ctx->xs == 2 && ctx->os == 3 && ctx->es == 1; }
 
 // Register-resident kernel window (chunks of 64 diagonals) for capacity W,
 // or 0 when the LDS kernel is used (other penalties, or W above 512).
@@ -373,6 +396,15 @@ static void use_register_kernel(const tsu_ctx* ctx, int32_t W, int32_t max_len, 
 }
 
 static Shape plain_shape(const tsu_ctx* ctx, int32_t W, int32_t max_len, bool* fits) {
+    if (ctx->two_piece) {
+        Shape s2{1, false, true, false, false};
+        s2.two_piece = true;
+        KArgs a = base_args(ctx);
+        a.W = W;
+        a.seq_words = seq_words_for(max_len);
+        *fits = tsu::lds_bytes(s2, a) <= ctx->max_lds;
+        return s2;
+    }
     Shape sh{default_waves(W), max_len <= 32000, true, false, false};
     sh.fixed_242 = tsu_host::fixed_242(ctx);
     use_register_kernel(ctx, W, max_len, sh);
@@ -393,6 +425,8 @@ extern "C" int tsu_align_plain(tsu_ctx* ctx, tsu_batch* b, int32_t wf_length, in
     const double w0 = now_ms();
     bool fits = false;
     Shape sh = plain_shape(ctx, W, b->max_len, &fits);
+    if (sh.two_piece && !fits)
+        return fail(ctx, TSU_EINVAL, "tsu_align_plain: 2-piece rings for wf_length " + std::to_string(W) + " do not fit in LDS");
     double kms = 0;
     int r;
     if (fits) {
@@ -521,6 +555,11 @@ int32_t tsu_host::initial_capacity(const tsu_ctx* ctx, int32_t bound) {
 }
 
 Shape tsu_host::batched_shape(const tsu_ctx* ctx, int32_t W, int32_t max_len, int nw_hint) {
+    if (ctx->two_piece) {  // one shape: the 2-piece kernel, fed from the queue
+        Shape s2{1, false, true, false, true};
+        s2.two_piece = true;
+        return s2;
+    }
     Shape sh{nw_hint > 0 ? nw_hint : default_waves(W), max_len <= 32000, true, false, true};
     sh.fixed_242 = tsu_host::fixed_242(ctx);
     // a tuned hint above 1 asks for the LDS kernel with that many waves
@@ -648,6 +687,7 @@ extern "C" int tsu_autotune(tsu_ctx* ctx, tsu_batch* b, int32_t sample_pairs) {
     if (!ctx || !b || b->ctx != ctx) return fail(ctx, TSU_EINVAL, "tsu_autotune: bad arguments");
     HIPCHK(hipSetDevice(ctx->device), "hipSetDevice");
     if (sample_pairs <= 0) sample_pairs = 4096;
+    if (ctx->two_piece) { ctx->tune.clear(); return TSU_OK; }  // one shape: nothing to tune
     const int nb = (int)(sizeof(kBucketBounds) / sizeof(kBucketBounds[0]));
     std::vector<std::vector<int32_t>> members((size_t)nb);
     for (int32_t i = 0; i < b->n; ++i) {
@@ -709,6 +749,7 @@ extern "C" int tsu_tune_table(const tsu_ctx* ctx, tsu_tune_entry* entries, int32
 extern "C" int tsu_align_cigar(tsu_ctx* ctx, tsu_batch* b, int32_t* scores, tsu_cigars** out, tsu_timing* t) {
     if (!ctx || !b || !scores || !out || b->ctx != ctx) return fail(ctx, TSU_EINVAL, "tsu_align_cigar: bad arguments");
     *out = nullptr;
+    if (ctx->two_piece) return fail(ctx, TSU_EINVAL, "tsu_align_cigar: not available for 2-piece penalties");
     HIPCHK(hipSetDevice(ctx->device), "hipSetDevice");
     if (t) std::memset(t, 0, sizeof(*t));
     const double w0 = now_ms();

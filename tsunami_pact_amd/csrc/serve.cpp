@@ -457,6 +457,7 @@ void worker_loop(tsu_server* s) {
 }  // namespace
 
 extern "C" int tsu_serve_open(tsu_ctx* ctx, const tsu_serve_cfg* cfg, tsu_server** out) {
+    if (ctx && ctx->two_piece) return tsu_host::fail(ctx, TSU_EINVAL, "tsu_serve_open: not available for 2-piece penalties");
     if (!ctx || !cfg || !out) return TSU_EINVAL;
     *out = nullptr;
     if (cfg->hbm_budget_bytes < (1 << 20) || cfg->max_inflight < 1 || cfg->max_inflight > 64 || cfg->max_batch_pairs < 1)

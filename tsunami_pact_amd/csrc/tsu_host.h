@@ -17,6 +17,9 @@ struct tsu_ctx {
     int32_t x = 0, o = 0, e = 0;        // penalties as given
     int32_t xs = 0, os = 0, es = 0, g = 1;  // divided by gcd
     int32_t rm = 0, ri = 0;
+    bool two_piece = false;             // gap-affine 2-piece penalties
+    int32_t o2 = 0, e2 = 0;             // second gap piece as given
+    int32_t o2s = 0, e2s = 0, r2 = 0;   // divided by gcd; I2/D2 ring depth
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     int cu_count = 0;

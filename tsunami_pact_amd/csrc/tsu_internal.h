@@ -42,6 +42,7 @@ struct KArgs {
     int32_t W;                 // diagonal capacity of a ring slot
     int32_t seq_words;         // LDS words reserved per sequence
     int32_t rm, ri;            // ring depths: M keeps max(x, o+e)+1 scores, I/D keep e+1
+    int32_t o2, e2, r2;        // gap-affine 2-piece: second gap piece (divided by g), I2/D2 ring depth e2+1
     int32_t* queue;            // work counter (persistent launches)
     // ring in global memory (when it does not fit in LDS): one region per workgroup
     uint8_t* ring_g;
@@ -68,6 +69,7 @@ struct Shape {
     bool fixed_242 = false;  // scaled penalties are (2, 3, 1): compile-time ring
     int reg_c = 0;           // > 0: register-resident kernel, reg_c chunks of 64 diagonals per wave (nw waves)
     bool lring = false;      // one wave: the older M scores in an int16 LDS ring (sequences up to 16000 bases)
+    bool two_piece = false;  // gap-affine 2-piece kernel (one wave per pair, rings in LDS)
 };
 
 // LDS bytes a launch of this shape needs.

@@ -20,6 +20,8 @@ KernelFn pick_regmw(const Shape& sh);
 int regmw_exchange_words(int nw);
 KernelFn pick_regml(const Shape& sh);
 int regml_exchange_words(int nw);
+KernelFn pick_2p(const Shape& sh);
+int64_t lds_bytes_2p(const KArgs& a);
 
 // Plain streaming copy (HBM byte-counter calibration).
 __global__ __launch_bounds__(256) void copy_kernel(const uint4* __restrict__ src, uint4* __restrict__ dst, int64_t n) {
@@ -39,6 +41,7 @@ int64_t ring_bytes(const Shape& sh, int32_t W, int32_t rm, int32_t ri) {
 }
 
 int64_t lds_bytes(const Shape& sh, const KArgs& a) {
+    if (sh.two_piece) return lds_bytes_2p(a);
     if (sh.reg_c > 0) {
         // sequences + misc (one wave), + the edge exchange area (several waves
         // of 8 chunks), + the exchange area and the int16 M ring (16 chunks)
@@ -53,6 +56,7 @@ int64_t lds_bytes(const Shape& sh, const KArgs& a) {
 }
 
 static KernelFn pick(const Shape& sh) {
+    if (sh.two_piece) return pick_2p(sh);
     if (sh.reg_c > 0) return sh.nw == 1 ? pick_reg(sh) : sh.reg_c == 16 ? pick_regml(sh) : pick_regmw(sh);
     switch (sh.nw) {
     case 1: return pick_nw1(sh);
