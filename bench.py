@@ -26,7 +26,7 @@ as a bit-exactness check of the timed GPU run.
 The batched form runs the autotuner on a sample first (untimed); with
 --compare-plain it also times the plain form on the same pairs.
 
-    python bench.py [--steps K] [--warmup W] [--form plain|batched|cigar|served]
+    python bench.py [--steps K] [--warmup W] [--form plain|batched|cigar|served|refgrid|twopiece]
 """
 import argparse
 import json
@@ -238,12 +238,50 @@ def run_refgrid(a, tsu, args):
             "vs_reference_peak": best["gcups"] / REF_PEAK_GCUPS}
 
 
+PEN_2P = (4, 6, 2, 24, 1)  # gap-affine 2-piece: gap of length L costs min(6 + 2L, 24 + L)
+
+
+def run_twopiece(tsu, args):
+    """Plain form under gap-affine 2-piece penalties on pairs of the config-2
+    shape (1000 bp, 5% mixed edits); capacity from the score bound (each edit
+    costs at most o1 + e1, the range grows by one diagonal per e2)."""
+    n = args.pairs or 100_000
+    x, o1, e1, o2, e2 = PEN_2P
+    ds = tsu.Dataset.generate(n, seed=SEED + 22, len_min=1000, err_min=0.05)
+    s_bound = int(round(0.05 * 1000)) * max(x, o1 + e1)
+    W = 2 * max((s_bound - o1) // e1, (s_bound - o2) // e2) + 3
+    a = tsu.Aligner(x, o1, e1, gap_opening2=o2, gap_extension2=e2)
+    b = a.batch(ds)
+    sc, t = a.align_plain(b, W)
+    kms = []
+    for _ in range(max(args.steps, 1)):
+        sc, t = a.align_plain(b, W)
+        kms.append(t["kernel_ms"])
+    kms = float(np.median(kms))
+    res = {"metric": "alignments/sec, gap-affine 2-piece, plain form, 1 GPU", "form": "twopiece",
+           "penalties": list(PEN_2P), "pairs": n, "length": 1000, "error": 0.05, "wf_length": W,
+           "kernel_ms": kms, "alignments_per_s": n / (kms / 1e3), "gcups": n * 1e6 / (kms / 1e3) / 1e9,
+           "device": a.device_info()["name"]}
+    if not args.no_cpu:
+        from oracle import oracle
+        m = min(n, 4000)
+        sample = [ds.pair(i) for i in range(m)]
+        threads = oracle.max_threads()
+        t0 = time.perf_counter()
+        want, _ = oracle.wfa2p_batch(sample, *PEN_2P, threads=threads)
+        dt = time.perf_counter() - t0
+        res["cpu_baseline"] = {"value": m / dt, "unit": "alignments/s", "cores": threads, "kind": "port",
+                               "sample": f"first {m} pairs, 2-piece oracle WFA (C, OpenMP, {threads} threads), {dt:.2f} s wall"}
+        res["bit_exact_sample"] = bool((want == sc[:m]).all())
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--form", default="plain", choices=["plain", "batched", "cigar", "served", "refgrid"])
+    ap.add_argument("--form", default="plain", choices=["plain", "batched", "cigar", "served", "refgrid", "twopiece"])
     ap.add_argument("--workload", default="cfg2", choices=sorted(WORKLOADS))
     ap.add_argument("--pairs", type=int, default=0, help="override the pair count (testing only)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
@@ -269,6 +307,9 @@ def main():
         ms = a.calibrate_copy(args.calibrate_copy)
         print(json.dumps({"copy_bytes": args.calibrate_copy, "kernel_ms": ms,
                           "GB_per_s": 2 * args.calibrate_copy / (ms / 1e3) / 1e9}), flush=True)
+        return
+    if args.form == "twopiece":
+        print(json.dumps(run_twopiece(tsu, args)), flush=True)
         return
     if args.form == "refgrid":
         a = tsu.Aligner(*PEN)
