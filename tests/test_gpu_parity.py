@@ -144,12 +144,13 @@ def test_int32_offsets_long_pair(aligner, tsu, oracle):
 
 
 def test_single_wave_kernels_short_and_long_sequences(aligner, tsu, oracle):
-    """Windows of at most 512 diagonals run on one wave per pair: sequences
-    up to 16,000 bases keep the older M scores in an int16 LDS ring, longer
-    ones the whole M ring in registers.  Both give the oracle's scores."""
-    short = tsu.Dataset.generate(200, seed=SEED + 17, len_min=2000, len_max=15000, err_min=0.001, err_max=0.004)
+    """Windows of at most 512 diagonals run on one wave per pair: with 8
+    chunks and sequences up to 4,096 bases the older M scores sit in an int16
+    LDS ring, otherwise the whole M ring is in registers.  Both give the
+    oracle's scores."""
+    short = tsu.Dataset.generate(200, seed=SEED + 17, len_min=1000, len_max=4000, err_min=0.005, err_max=0.02)
     long_ = tsu.Dataset.generate(6, seed=SEED + 18, len_min=17000, len_max=30000, err_min=0.001, err_max=0.002)
-    for name, ds in (("<= 16000", short), ("> 16000", long_)):
+    for name, ds in (("<= 4096", short), ("> 16000", long_)):
         want, _, _ = oracle.wfa_batch(ds.pairs(), 4, 6, 2)
         say(f"single wave {name} W=512")
         got, _ = aligner.align_plain(aligner.batch(ds), 512)

@@ -379,7 +379,11 @@ static void use_register_kernel(const tsu_ctx* ctx, int32_t W, int32_t max_len, 
     if (int c = tsu_host::reg_chunks(ctx, W)) {
         r.reg_c = c;
         r.nw = 1;
-        r.lring = max_len <= 16000;  // doubled offsets fit the int16 ring
+        // the int16 LDS ring variant (doubled offsets fit int16 up to 16000
+        // bases) measured faster only with 8 chunks on short sequences; with
+        // narrower windows or long, extend-heavy pairs its ring traffic costs
+        // more than the DPP moves it saves
+        r.lring = c == 8 && max_len <= 4096;
     } else if (int nwl = tsu_host::ml_waves(ctx, W, max_len)) {
         r.reg_c = 16;
         r.nw = nwl;
