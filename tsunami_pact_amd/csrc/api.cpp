@@ -401,7 +401,7 @@ static void use_register_kernel(const tsu_ctx* ctx, int32_t W, int32_t max_len, 
 
 static Shape plain_shape(const tsu_ctx* ctx, int32_t W, int32_t max_len, bool* fits) {
     if (ctx->two_piece) {
-        Shape s2{1, false, true, false, false};
+        Shape s2{1, max_len <= 32000, true, false, false};
         s2.two_piece = true;
         KArgs a = base_args(ctx);
         a.W = W;
@@ -560,7 +560,7 @@ int32_t tsu_host::initial_capacity(const tsu_ctx* ctx, int32_t bound) {
 
 Shape tsu_host::batched_shape(const tsu_ctx* ctx, int32_t W, int32_t max_len, int nw_hint) {
     if (ctx->two_piece) {  // one shape: the 2-piece kernel, fed from the queue
-        Shape s2{1, false, true, false, true};
+        Shape s2{1, max_len <= 32000, true, false, true};
         s2.two_piece = true;
         return s2;
     }

@@ -16,8 +16,12 @@
 // address 0 (as every kernel here, so the extend addresses them directly),
 // 8 misc words, one (lo, hi) pair per ring slot, then the rings: M keeps
 // max(x, o1+e1, o2+e2)+1 scores, I1/D1 e1+1, I2/D2 e2+1, plus one slot that
-// stays NULL (the source of scores below 0); each slot holds W+2 int32
-// entries, diagonal k at k - kmin + 1 with NULL guards at both ends.  Every
+// stays NULL (the source of scores below 0); each slot holds W+2 entries,
+// diagonal k at k - kmin + 1 with NULL guards at both ends.  Entries are
+// int16 for sequences up to 32000 bases, int32 beyond.  Stored values are
+// canonical: a negative value is stored as NULL and an I above t as t+1,
+// which keeps every in-bounds cell and the out-of-bounds classification of
+// every other cell (so int16 NULLs cannot drift towards valid offsets).  Every
 // slot holds NULL outside its current range: a step writes its outputs over
 // the union of the new range and the old contents' ranges, and a trim writes
 // NULL over the cells it cuts.  One wave owns everything, and LDS operations
@@ -30,15 +34,17 @@
 
 namespace tsu {
 
-constexpr int32_t kP2Null = -1073741824;
+template <typename OffT> struct P2Null;
+template <> struct P2Null<int16_t> { static constexpr int32_t v = -16384; };
+template <> struct P2Null<int32_t> { static constexpr int32_t v = -1073741824; };
 
 struct P2Src {
     int lo, hi;  // lo > hi: null
 };
 
-template <bool QUEUE>
+template <typename OffT, bool QUEUE>
 __global__ __launch_bounds__(64) void wfa2p_kernel(KArgs a) {
-    constexpr int32_t NUL = kP2Null;
+    constexpr int32_t NUL = P2Null<OffT>::v;
     extern __shared__ __align__(16) uint32_t smem[];
     const int lane = threadIdx.x;
     uint32_t* sS = smem;
@@ -46,9 +52,9 @@ __global__ __launch_bounds__(64) void wfa2p_kernel(KArgs a) {
     const int RM = a.rm, R1 = a.ri, R2 = a.r2;
     const int NS = RM + 2 * R1 + 2 * R2;  // slot NS is the permanent NULL slot
     int2* lim = reinterpret_cast<int2*>(misc + 8);
-    const int W = a.W;          // multiple of 64
+    const int W = a.W;          // diagonal capacity
     const int STR = W + 2;
-    int32_t* ring = reinterpret_cast<int32_t*>(lim + NS + 1);
+    OffT* ring = reinterpret_cast<OffT*>(lim + NS + 1);
     const bool lds_at_zero = (uint32_t)(uintptr_t)sS == 0u;
     const int x = a.x, oe1 = a.o + a.e, oe2 = a.o2 + a.e2, e1 = a.e, e2 = a.e2;
 
@@ -73,7 +79,7 @@ __global__ __launch_bounds__(64) void wfa2p_kernel(KArgs a) {
             sS[2 * i] = i < pw ? a.words[pr.p_off + i] : 0u;
             sS[2 * i + 1] = i < tw ? a.words[pr.t_off + i] : 0u;
         }
-        for (int i = lane; i < (NS + 1) * STR; i += 64) ring[i] = NUL;
+        for (int i = lane; i < (NS + 1) * STR; i += 64) ring[i] = (OffT)NUL;
         for (int i = lane; i <= NS; i += 64) lim[i] = make_int2(1, 0);
         __syncthreads();
 
@@ -92,7 +98,7 @@ __global__ __launch_bounds__(64) void wfa2p_kernel(KArgs a) {
 
         // score 0: M[0][0], extended
         const int ext0 = lcp_interleaved(sS, 0, 0, min(p, t));
-        if (lane == 0) { R(0)[0] = ext0; lim[0] = make_int2(0, 0); }
+        if (lane == 0) { R(0)[0] = (OffT)ext0; lim[0] = make_int2(0, 0); }
         __syncthreads();
         int status = (p == t && ext0 == t) ? 1 : 0;  // 1: score 0
         int s = 0;
@@ -136,29 +142,32 @@ __global__ __launch_bounds__(64) void wfa2p_kernel(KArgs a) {
             const bool h_i1 = !null_step && (!no1 || !ni1), h_d1 = !null_step && (!no1 || !nd1);
             const bool h_i2 = !null_step && (!no2 || !ni2), h_d2 = !null_step && (!no2 || !nd2);
             if (!null_step) { clo = min(clo, lo); chi = max(chi, hi); }
-            const int32_t* Mx = R(gx);
-            const int32_t* Mo1 = R(go1);
-            const int32_t* Mo2 = R(go2);
-            const int32_t* I1e = R(gi1);
-            const int32_t* D1e = R(gd1);
-            const int32_t* I2e = R(gi2);
-            const int32_t* D2e = R(gd2);
-            int32_t* oM = R(gout[0]);
-            int32_t* oI1 = R(gout[1]);
-            int32_t* oD1 = R(gout[2]);
-            int32_t* oI2 = R(gout[3]);
-            int32_t* oD2 = R(gout[4]);
+            const OffT* Mx = R(gx);
+            const OffT* Mo1 = R(go1);
+            const OffT* Mo2 = R(go2);
+            const OffT* I1e = R(gi1);
+            const OffT* D1e = R(gd1);
+            const OffT* I2e = R(gi2);
+            const OffT* D2e = R(gd2);
+            OffT* oM = R(gout[0]);
+            OffT* oI1 = R(gout[1]);
+            OffT* oD1 = R(gout[2]);
+            OffT* oI2 = R(gout[3]);
+            OffT* oD2 = R(gout[4]);
             int32_t mend = NUL;
             for (int cb = clo; cb <= chi; cb += 64) {
                 const int k = cb + lane;
                 const bool act = k <= chi;
                 const int kk = act ? k : chi;  // idle lanes read a cell of the region
                 const bool in = k >= lo && k <= hi;
-                int32_t i1 = max(Mo1[kk - 1], I1e[kk - 1]) + 1;
-                int32_t i2 = max(Mo2[kk - 1], I2e[kk - 1]) + 1;
-                int32_t d1 = max(Mo1[kk + 1], D1e[kk + 1]);
-                int32_t d2 = max(Mo2[kk + 1], D2e[kk + 1]);
-                int32_t m = max(max(Mx[kk] + 1, max(i1, i2)), max(d1, d2));
+                const int32_t i1p = max((int32_t)Mo1[kk - 1], (int32_t)I1e[kk - 1]);
+                const int32_t i2p = max((int32_t)Mo2[kk - 1], (int32_t)I2e[kk - 1]);
+                int32_t i1 = i1p >= 0 ? min(i1p + 1, t + 1) : NUL;
+                int32_t i2 = i2p >= 0 ? min(i2p + 1, t + 1) : NUL;
+                int32_t d1 = max((int32_t)Mo1[kk + 1], (int32_t)D1e[kk + 1]);
+                int32_t d2 = max((int32_t)Mo2[kk + 1], (int32_t)D2e[kk + 1]);
+                const int32_t mxv = Mx[kk];
+                int32_t m = max(max(mxv >= 0 ? mxv + 1 : NUL, max(i1, i2)), max(d1, d2));
                 const bool valid = in && (uint32_t)m <= (uint32_t)t && (uint32_t)(m - k) <= (uint32_t)p;
                 if (!valid) m = NUL;
                 if (!in || !h_i1) i1 = NUL;
@@ -170,11 +179,11 @@ __global__ __launch_bounds__(64) void wfa2p_kernel(KArgs a) {
                     if (lm > 0) m += extend_len((uint32_t)(m - k), (uint32_t)m, lm);
                 }
                 if (act) {
-                    oM[k] = m;
-                    oI1[k] = i1;
-                    oD1[k] = d1;
-                    oI2[k] = i2;
-                    oD2[k] = d2;
+                    oM[k] = (OffT)m;
+                    oI1[k] = (OffT)i1;
+                    oD1[k] = (OffT)d1;
+                    oI2[k] = (OffT)i2;
+                    oD2[k] = (OffT)d2;
                 }
                 if (tp >= cb && tp < cb + 64) mend = __builtin_amdgcn_readlane(m, tp - cb);
             }
@@ -191,7 +200,7 @@ __global__ __launch_bounds__(64) void wfa2p_kernel(KArgs a) {
             const bool has[5] = {true, h_i1, h_d1, h_i2, h_d2};
 #pragma unroll
             for (int c = 0; c < 5; ++c) {
-                int32_t* Rc = R(gout[c]);
+                OffT* Rc = R(gout[c]);
                 int f = hi + 1, l = lo - 1;
                 if (has[c]) {
                     for (int kb = lo; kb <= hi; kb += 64) {
@@ -210,8 +219,8 @@ __global__ __launch_bounds__(64) void wfa2p_kernel(KArgs a) {
                             if (b) { l = kt - (int)__builtin_ctzll(b); break; }
                         }
                     }
-                    for (int k = lo + lane; k < f; k += 64) Rc[k] = NUL;
-                    for (int k = max(l + 1, lo) + lane; k <= hi; k += 64) Rc[k] = NUL;
+                    for (int k = lo + lane; k < f; k += 64) Rc[k] = (OffT)NUL;
+                    for (int k = max(l + 1, lo) + lane; k <= hi; k += 64) Rc[k] = (OffT)NUL;
                 }
                 if (lane == 0) lim[gout[c]] = has[c] && f <= l ? make_int2(f, l) : make_int2(1, 0);
             }
@@ -223,12 +232,16 @@ __global__ __launch_bounds__(64) void wfa2p_kernel(KArgs a) {
     }
 }
 
-KernelFn pick_2p(const Shape& sh) { return sh.queue ? wfa2p_kernel<true> : wfa2p_kernel<false>; }
+KernelFn pick_2p(const Shape& sh) {
+    if (sh.off16) return sh.queue ? wfa2p_kernel<int16_t, true> : wfa2p_kernel<int16_t, false>;
+    return sh.queue ? wfa2p_kernel<int32_t, true> : wfa2p_kernel<int32_t, false>;
+}
 
 // LDS bytes of the 2-piece kernel: sequences, misc, slot limits, rings.
-int64_t lds_bytes_2p(const KArgs& a) {
+int64_t lds_bytes_2p(const Shape& sh, const KArgs& a) {
     const int64_t ns = (int64_t)a.rm + 2 * a.ri + 2 * a.r2 + 1;
-    return 4 * (2 * (int64_t)a.seq_words + 8) + 8 * ns + 4 * ns * ((int64_t)a.W + 2);
+    const int64_t ring = ns * ((int64_t)a.W + 2) * (sh.off16 ? 2 : 4);
+    return 4 * (2 * (int64_t)a.seq_words + 8) + 8 * ns + ((ring + 3) & ~3ll);
 }
 
 }  // namespace tsu

@@ -21,7 +21,7 @@ int regmw_exchange_words(int nw);
 KernelFn pick_regml(const Shape& sh);
 int regml_exchange_words(int nw);
 KernelFn pick_2p(const Shape& sh);
-int64_t lds_bytes_2p(const KArgs& a);
+int64_t lds_bytes_2p(const Shape& sh, const KArgs& a);
 
 // Plain streaming copy (HBM byte-counter calibration).
 __global__ __launch_bounds__(256) void copy_kernel(const uint4* __restrict__ src, uint4* __restrict__ dst, int64_t n) {
@@ -41,7 +41,7 @@ int64_t ring_bytes(const Shape& sh, int32_t W, int32_t rm, int32_t ri) {
 }
 
 int64_t lds_bytes(const Shape& sh, const KArgs& a) {
-    if (sh.two_piece) return lds_bytes_2p(a);
+    if (sh.two_piece) return lds_bytes_2p(sh, a);
     if (sh.reg_c > 0) {
         // sequences + misc (one wave), + the edge exchange area (several waves
         // of 8 chunks), + the exchange area and the int16 M ring (16 chunks)
