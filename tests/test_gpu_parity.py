@@ -351,3 +351,20 @@ def test_long_read_cigars(aligner, tsu, oracle):
     assert (got == want).all()
     for i, (p, t) in enumerate(pairs):
         assert cig[i] == oracle.rle(ops[i]), i
+
+
+def test_wide_cigars_on_register_kernel(aligner, tsu, oracle):
+    """CIGARs of pairs whose capacity lands above 2048 diagonals come from the
+    16-chunk multi-wave kernel's records (8 and 16 waves here): 12-14 kbp at
+    12% and 6 kbp at 15%; same CIGARs as the oracle, which re-score to the
+    score."""
+    ds = tsu.Dataset.generate(3, seed=SEED + 19, len_min=12000, len_max=14000, err_min=0.12)
+    pairs = ds.pairs() + tsu.Dataset.generate(3, seed=SEED + 23, len_min=6000, err_min=0.15).pairs()
+    want, ops, _ = oracle.wfa_batch(pairs, 4, 6, 2, cigar=True)
+    say("cigar wide 6 pairs")
+    got, cig, _ = aligner.align_cigar(aligner.batch(pairs))
+    assert (got == want).all()
+    for i, (p, t) in enumerate(pairs):
+        assert cig[i] == oracle.rle(ops[i]), i
+        assert oracle.rescore(oracle.expand(cig[i]), p, t, 4, 6, 2) == got[i]
+

@@ -800,22 +800,29 @@ extern "C" int tsu_align_cigar(tsu_ctx* ctx, tsu_batch* b, int32_t* scores, tsu_
         // all pairs with score within this group share W and buffers
         const int32_t W = std::max<int32_t>(capacity_for_score(ctx, s0), 16);
         const int32_t steps = s0 + 2;
-        int64_t recs = 0;
-        for (int64_t s = 0; s <= s0; ++s)
-            recs += std::min<int64_t>(W, 2 * (std::max<int64_t>(0, s - ctx->os) / ctx->es + 2) + 1);
-        recs += 64;
-        Shape sh{default_waves(W), true, true, true, true};
-        sh.fixed_242 = tsu_host::fixed_242(ctx);
         KArgs a = base_args(ctx);
         for (end = pos; end < order.size(); ++end) {
             const tsu_pair& pr = b->pairs[(size_t)order[end]];
             max_len = std::max(max_len, std::max(pr.p_len, pr.t_len));
             if (end - pos >= 65536) break;
         }
-        sh.off16 = max_len <= 32000;
         a.W = W;
         a.seq_words = seq_words_for(max_len);
-        if (tsu::lds_bytes(sh, a) > ctx->max_lds) { sh.lds_ring = false; sh.nw = 16; }
+        Shape sh{default_waves(W), max_len <= 32000, true, true, true};
+        sh.fixed_242 = tsu_host::fixed_242(ctx);
+        int64_t recs = 0;
+        if (int nwl = tsu_host::ml_waves(ctx, W, max_len)) {
+            // 16-chunk multi-wave kernel: records at a fixed place per step and
+            // chunk, (s * chunks + j) * 64 + lane
+            sh.nw = nwl;
+            sh.reg_c = 16;
+            recs = (int64_t)steps * nwl * 16 * 64;
+        } else {
+            for (int64_t s = 0; s <= s0; ++s)
+                recs += std::min<int64_t>(W, 2 * (std::max<int64_t>(0, s - ctx->os) / ctx->es + 2) + 1);
+            recs += 64;
+            if (tsu::lds_bytes(sh, a) > ctx->max_lds) { sh.lds_ring = false; sh.nw = 16; }
+        }
         int per_cu = 0;
         int rr = tsu::occupancy(sh, tsu::lds_bytes(sh, a), &per_cu);
         if (rr != 0 || per_cu <= 0) { tsu_cigars_free(cg); return fail(ctx, TSU_EINTERNAL, "backtrace shape has no occupancy"); }

@@ -31,8 +31,14 @@
 //
 // Recurrences, bounds test, extend, trim and end test are those of the
 // other kernels (wfa_gpu.cu:253-331,610-631,633-719).
+//
+// With BT every computed chunk also writes its 64 cell records (wfa_bt.h)
+// to global memory at ((s * NC + j) * 64 + lane) of the workgroup's region
+// (j the global chunk index), one coalesced 256-byte store per chunk and
+// step; after the end cell is reached, thread 0 walks them into CIGAR runs.
 #pragma once
 
+#include "wfa_bt.h"
 #include "wfa_regmw.h"
 
 namespace tsu {
@@ -66,7 +72,7 @@ constexpr uint32_t kMlEdgeBit = 1u << 16;
 
 typedef __attribute__((address_space(3))) const int32_t lds_ci32;
 
-template <int NW, int C, bool QUEUE>
+template <int NW, int C, bool QUEUE, bool BT = false>
 __global__ __launch_bounds__(NW * 64) void wfa_regml_kernel(KArgs a) {
     static_assert(NW >= 2 && NW <= 16, "2..16 waves");
     static_assert(C >= 1 && C <= 16, "16 chunk bits per wave");
@@ -90,6 +96,8 @@ __global__ __launch_bounds__(NW * 64) void wfa_regml_kernel(KArgs a) {
     // per computed chunk every lane stores once: lane 63 its I edge, lane 0
     // its D edge, the others into a sink (cheaper than masking the store)
     const int pub_off = lane == 0 ? L::ED - L::EI : 0;
+    uint32_t* rec = nullptr;
+    if constexpr (BT) rec = a.rec_g + (int64_t)blockIdx.x * a.rec_cap;
 
     for (int iter = 0;; ++iter) {
         int idx;
@@ -176,6 +184,9 @@ __global__ __launch_bounds__(NW * 64) void wfa_regml_kernel(KArgs a) {
             ++s;
             if (s > s_bound) return TSU_EINTERNAL;
             if (s > kMlMaxScore) return TSU_EKOVERFLOW;
+            if constexpr (BT) {
+                if (s >= a.step_cap) return TSU_ECIGAR;
+            }
             fidx = fidx + 1 == 3 ? 0 : fidx + 1;
             l5 = l5 == 4 ? 0 : l5 + 1;
             l3 = l3 == 2 ? 0 : l3 + 1;
@@ -202,6 +213,11 @@ __global__ __launch_bounds__(NW * 64) void wfa_regml_kernel(KArgs a) {
             int32_t* epub = (lane == 0 || lane == 63) ? X + L::EI + PAR * (NC + 2) + w + 1 + pub_off : X + L::SK + lane;
             uint32_t em = endm;  // opaque per step: no per-chunk predicates held across the loop
             asm volatile("" : "+s"(em));
+            // records of this step: chunk j at (s * NC + j) * 64 (uniform base,
+            // 32-bit lane offset pb + ln)
+            uint32_t* recs = nullptr;
+            const uint32_t rl = (uint32_t)(pb + ln);
+            if constexpr (BT) recs = rec + (int64_t)s * NC * 64;
             const int16_t* mrd = MR + rd * L::MR_STRIDE + pb + ln;        // entry of k-1
             int16_t* mwr = MR + wrs * L::MR_STRIDE + pb + ln + 1;         // entry of k
             uint32_t acc = 0;  // step mask, one bit up
@@ -222,10 +238,17 @@ __global__ __launch_bounds__(NW * 64) void wfa_regml_kernel(KArgs a) {
                     const int32_t fdr = ed1[c * NW + 2];
                     const int32_t iel = dpp_shr1(fil, Ir[c]);
                     const int32_t der = dpp_shl1(fdr, Dr[c]);
+                    uint32_t rb = 0;  // backtrace record bits of the gap sources
+                    if constexpr (BT) rb = (iel >= mol ? 4u : 0u) | (der >= mor ? 8u : 0u);
                     const int32_t I = max(mol, iel) + 1;
                     const int32_t D = max(mor, der);
-                    int32_t M = max(max(D, mx + 1), I);
+                    const int32_t X = mx + 1;
+                    int32_t M = max(max(D, X), I);
                     const bool valid = (uint32_t)M < (uint32_t)mt1;
+                    if constexpr (BT) {
+                        const uint32_t r = rb | (valid ? ((uint32_t)M << 4) | ((X >= D && X >= I) ? 1u : (D >= I ? 2u : 3u)) : 0u);
+                        recs[(uint32_t)(64 * NW * c) + rl] = r;
+                    }
                     M = valid ? M : NUL;
                     const int lim1 = mt1 - M;  // bases left + 1
                     if (valid && lim1 > 1) M += extend_len((uint32_t)(M - kl - 64 * NW * c), (uint32_t)M, lim1 - 1);
@@ -337,7 +360,29 @@ __global__ __launch_bounds__(NW * 64) void wfa_regml_kernel(KArgs a) {
             if (r == 0) r = step(std::integral_constant<int, 0>{});
         }
         const int score = (r > 0 && s == 0) ? 0 : r;
-        if (tid == 0) a.scores[pi] = !lds_at_zero ? TSU_EINTERNAL : score < 0 ? score : score * a.g;
+        if constexpr (BT) {
+            __syncthreads();  // every wave's records written
+            if (tid == 0) {
+                int st = !lds_at_zero ? TSU_EINTERNAL : score < 0 ? score : 0;
+                int64_t out_off = 0;
+                int32_t out_n = 0;
+                if (st == 0) {
+                    auto record = [&](int ss, int kk, uint32_t& rr) __attribute__((always_inline)) -> bool {
+                        if (ss < 0 || ss > s) return false;
+                        const int j = kk - kmin;
+                        if (j < 0 || j >= W) return false;
+                        rr = rec[((int64_t)ss * NC + (j >> 6)) * 64 + (j & 63)];
+                        return true;
+                    };
+                    st = bt_walk(record, score, p, t, 2, 3, 1, a, &out_off, &out_n);
+                }
+                a.run_off[pi] = out_off;
+                a.run_n[pi] = out_n;
+                a.scores[pi] = st ? st : score * a.g;
+            }
+        } else {
+            if (tid == 0) a.scores[pi] = !lds_at_zero ? TSU_EINTERNAL : score < 0 ? score : score * a.g;
+        }
         __syncthreads();  // LDS reused by the next pair of this workgroup
     }
 }

@@ -8,7 +8,16 @@ namespace tsu {
 constexpr int kMlChunks = 16;
 
 KernelFn pick_regml(const Shape& sh) {
-    if (!sh.fixed_242 || sh.bt || sh.reg_c != kMlChunks) return nullptr;
+    if (!sh.fixed_242 || sh.reg_c != kMlChunks) return nullptr;
+    if (sh.bt) {  // CIGAR launches are always queue-fed
+        if (!sh.queue) return nullptr;
+        switch (sh.nw) {
+        case 4: return wfa_regml_kernel<4, 16, true, true>;
+        case 8: return wfa_regml_kernel<8, 16, true, true>;
+        case 16: return wfa_regml_kernel<16, 16, true, true>;
+        default: return nullptr;
+        }
+    }
     switch (sh.nw) {
     case 4: return sh.queue ? wfa_regml_kernel<4, 16, true> : wfa_regml_kernel<4, 16, false>;
     case 8: return sh.queue ? wfa_regml_kernel<8, 16, true> : wfa_regml_kernel<8, 16, false>;
