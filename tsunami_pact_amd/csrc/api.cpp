@@ -14,6 +14,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <memory>
 #include <mutex>
 #include <new>
 #include <numeric>
@@ -114,6 +115,7 @@ extern "C" int tsu_create(tsu_ctx** out, int device, int32_t mismatch, int32_t g
     ctx->max_lds = (int64_t)prop.sharedMemPerBlock;
     std::snprintf(ctx->name, sizeof(ctx->name), "%.180s (%.60s)", prop.name, prop.gcnArchName);
     if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipStreamCreateWithFlags(&ctx->stream2, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreate(&ctx->ev0) != hipSuccess || hipEventCreate(&ctx->ev1) != hipSuccess ||
         hipMalloc(reinterpret_cast<void**>(&ctx->d_queue), 64 * sizeof(int32_t)) != hipSuccess) {
         tsu_destroy(ctx);
@@ -147,10 +149,12 @@ extern "C" void tsu_destroy(tsu_ctx* ctx) {
     if (!ctx) return;
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    if (ctx->stream2) (void)hipStreamSynchronize(ctx->stream2);
     if (ctx->d_queue) (void)hipFree(ctx->d_queue);
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
     if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    if (ctx->stream2) (void)hipStreamDestroy(ctx->stream2);
     delete ctx;
 }
 
@@ -294,10 +298,14 @@ int64_t tsu_host::resident_grid(const tsu_ctx* ctx, const Shape& sh, const KArgs
     return (int64_t)per_cu * ctx->cu_count;
 }
 
-// Runs one launch of `shape` over the queue entries `order` (or pairs [0,n)
-// when order is null and the launch is plain).  Scores land in b->d_scores.
-static int run_shape(tsu_ctx* ctx, tsu_batch* b, Shape sh, int32_t W, const int32_t* d_order, int32_t n_entries,
-                     int32_t max_len, int64_t grid_cap, KArgs* extra, double* kernel_ms) {
+// Launches one shape on `stream` (queue counter `queue`, events e0/e1 around
+// the kernel) without waiting; an HBM ring, when the shape needs one, is left
+// in `ring` and must outlive the kernel.  *launched is false when there was
+// nothing to launch.
+static int launch_shape(tsu_ctx* ctx, tsu_batch* b, Shape sh, int32_t W, const int32_t* d_order, int32_t n_entries,
+                        int32_t max_len, int64_t grid_cap, KArgs* extra, hipStream_t stream, int32_t* queue,
+                        hipEvent_t e0, hipEvent_t e1, DevBuf<uint8_t>& ring, bool* launched) {
+    *launched = false;
     KArgs a = extra ? *extra : base_args(ctx);
     a.x = ctx->xs; a.o = ctx->os; a.e = ctx->es; a.g = ctx->g; a.rm = ctx->rm; a.ri = ctx->ri;
     a.words = b->d_words;
@@ -307,11 +315,10 @@ static int run_shape(tsu_ctx* ctx, tsu_batch* b, Shape sh, int32_t W, const int3
     a.scores = b->d_scores;
     a.W = W;
     a.seq_words = seq_words_for(max_len);
-    a.queue = ctx->d_queue;
+    a.queue = queue;
     const int64_t lds = tsu::lds_bytes(sh, a);
     if (lds > ctx->max_lds) return fail(ctx, TSU_EINVAL, "shape does not fit in LDS");
     int64_t grid = n_entries;
-    DevBuf<uint8_t> ring;
     if (sh.queue) {
         int per_cu = 0;
         int r = tsu::occupancy(sh, lds, &per_cu);
@@ -319,7 +326,7 @@ static int run_shape(tsu_ctx* ctx, tsu_batch* b, Shape sh, int32_t W, const int3
         if (per_cu <= 0) return fail(ctx, TSU_EINVAL, "shape has zero occupancy");
         grid = std::min<int64_t>((int64_t)per_cu * ctx->cu_count, n_entries);
         if (grid_cap > 0) grid = std::min(grid, grid_cap);
-        HIPCHK(hipMemsetAsync(ctx->d_queue, 0, sizeof(int32_t), ctx->stream), "hipMemsetAsync (queue)");
+        HIPCHK(hipMemsetAsync(queue, 0, sizeof(int32_t), stream), "hipMemsetAsync (queue)");
         if (!sh.lds_ring) {
             a.ring_stride = tsu::ring_bytes(sh, W, ctx->rm, ctx->ri);
             HIPCHK(ring.alloc((size_t)(a.ring_stride * std::max<int64_t>(grid, 1))), "hipMalloc (ring)");
@@ -328,11 +335,26 @@ static int run_shape(tsu_ctx* ctx, tsu_batch* b, Shape sh, int32_t W, const int3
     } else if (!sh.lds_ring || sh.bt) {
         return fail(ctx, TSU_EINVAL, "plain launches keep the ring in LDS");
     }
-    if (grid <= 0) { if (kernel_ms) *kernel_ms = 0; return TSU_OK; }
-    HIPCHK(hipEventRecord(ctx->ev0, ctx->stream), "hipEventRecord");
-    int r = tsu::launch_wfa(sh, a, grid, ctx->stream);
+    if (grid <= 0) return TSU_OK;
+    HIPCHK(hipEventRecord(e0, stream), "hipEventRecord");
+    int r = tsu::launch_wfa(sh, a, grid, stream);
     if (r != 0) return hip_fail(ctx, (hipError_t)r, "kernel launch");
-    HIPCHK(hipEventRecord(ctx->ev1, ctx->stream), "hipEventRecord");
+    HIPCHK(hipEventRecord(e1, stream), "hipEventRecord");
+    *launched = true;
+    return TSU_OK;
+}
+
+// Runs one launch of `shape` over the queue entries `order` (or pairs [0,n)
+// when order is null and the launch is plain) on the context's stream and
+// waits for it.  Scores land in b->d_scores.
+static int run_shape(tsu_ctx* ctx, tsu_batch* b, Shape sh, int32_t W, const int32_t* d_order, int32_t n_entries,
+                     int32_t max_len, int64_t grid_cap, KArgs* extra, double* kernel_ms) {
+    DevBuf<uint8_t> ring;
+    bool launched = false;
+    if (kernel_ms) *kernel_ms = 0;
+    int r = launch_shape(ctx, b, sh, W, d_order, n_entries, max_len, grid_cap, extra, ctx->stream, ctx->d_queue,
+                         ctx->ev0, ctx->ev1, ring, &launched);
+    if (r != TSU_OK || !launched) return r;
     HIPCHK(hipEventSynchronize(ctx->ev1), "hipEventSynchronize");
     float ms = 0.f;
     HIPCHK(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1), "hipEventElapsedTime");
@@ -621,21 +643,21 @@ static int batched_core(tsu_ctx* ctx, tsu_batch* b, const std::vector<int32_t>& 
     HIPCHK(d_order.alloc(std::max<size_t>(sel.size(), 1)), "hipMalloc (order)");
     double kms_total = 0;
     int launches = 0, retried = 0;
-    // longest buckets first: their tails overlap nothing, so start them early
-    for (int bi = nb - 1; bi >= 0; --bi) {
+    // every bucket ordered longest first, all sorted up front (one thread per
+    // bucket), so no sort sits between two launches
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int bi = 0; bi < nb; ++bi) {
         Bucket& bk = buckets[(size_t)bi];
-        if (bk.idx.empty()) continue;
         std::stable_sort(bk.idx.begin(), bk.idx.end(), [&](int32_t a1, int32_t a2) {
             const tsu_pair& x1 = b->pairs[(size_t)a1];
             const tsu_pair& x2 = b->pairs[(size_t)a2];
             return (int64_t)x1.p_len * x1.t_len > (int64_t)x2.p_len * x2.t_len;
         });
-        int32_t max_len = 0;
-        for (int32_t i : bk.idx) max_len = std::max(max_len, std::max(b->pairs[(size_t)i].p_len, b->pairs[(size_t)i].t_len));
-        const tsu_tune_entry* te = use_tune ? tune_lookup(ctx, bk.bound) : nullptr;
-        int32_t W = (te && te->capacity > 0) ? te->capacity : initial_capacity(ctx, std::min(bk.bound, max_len));
-        const int nw_hint = te ? te->waves_per_pair : 0;
-        std::vector<int32_t> todo = bk.idx;
+    }
+    std::vector<int32_t> tmp((size_t)b->n);
+    // Launches `todo` at capacity W (widened while pairs overflow) and waits;
+    // scores go to `scores` at their pair index.
+    auto settle = [&](std::vector<int32_t> todo, int32_t W, int32_t max_len, int nw_hint) -> int {
         for (int attempt = 0; !todo.empty(); ++attempt) {
             W = std::min<int32_t>(W, capacity_full(max_len, max_len));
             // the tuned shape is for the first attempt; wider re-runs take
@@ -649,7 +671,6 @@ static int batched_core(tsu_ctx* ctx, tsu_batch* b, const std::vector<int32_t>& 
             kms_total += kms;
             ++launches;
             // scores live at their pair index
-            std::vector<int32_t> tmp((size_t)b->n);
             HIPCHK(hipMemcpy(tmp.data(), b->d_scores, (size_t)b->n * sizeof(int32_t), hipMemcpyDeviceToHost), "hipMemcpy D2H (scores)");
             std::vector<int32_t> again;
             for (int32_t i : todo) {
@@ -663,6 +684,100 @@ static int batched_core(tsu_ctx* ctx, tsu_batch* b, const std::vector<int32_t>& 
             }
             todo.swap(again);
         }
+        return TSU_OK;
+    };
+    // The widened re-run of a bucket's overflowing pairs (a few long pairs,
+    // each on one CU for a long time) runs on the second stream beside the
+    // next bucket's launch instead of alone after it.
+    struct Deferred {
+        std::vector<int32_t> ids;
+        int32_t W, max_len;
+        DevBuf<int32_t> d_ids;
+        DevBuf<uint8_t> ring;
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        bool launched = false;
+        ~Deferred() { if (e0) (void)hipEventDestroy(e0); if (e1) (void)hipEventDestroy(e1); }
+    };
+    std::vector<std::unique_ptr<Deferred>> deferred;
+    // on every return the second stream is drained before the re-runs' buffers are freed
+    struct Drain {
+        hipStream_t s;
+        ~Drain() { (void)hipStreamSynchronize(s); }
+    } drain{ctx->stream2};
+    int last = -1;
+    for (int bi = 0; bi < nb; ++bi)
+        if (!buckets[(size_t)bi].idx.empty()) { last = bi; break; }
+    // longest buckets first: their tails overlap nothing, so start them early
+    for (int bi = nb - 1; bi >= 0; --bi) {
+        Bucket& bk = buckets[(size_t)bi];
+        if (bk.idx.empty()) continue;
+        int32_t max_len = 0;
+        for (int32_t i : bk.idx) max_len = std::max(max_len, std::max(b->pairs[(size_t)i].p_len, b->pairs[(size_t)i].t_len));
+        const tsu_tune_entry* te = use_tune ? tune_lookup(ctx, bk.bound) : nullptr;
+        int32_t W = (te && te->capacity > 0) ? te->capacity : initial_capacity(ctx, std::min(bk.bound, max_len));
+        W = std::min<int32_t>(W, capacity_full(max_len, max_len));
+        const int nw_hint = te ? te->waves_per_pair : 0;
+        if (bi == last) {  // nothing left to overlap with
+            int r = settle(bk.idx, W, max_len, nw_hint);
+            if (r != TSU_OK) return r;
+            continue;
+        }
+        // first attempt, then its overflow re-run deferred to the second stream
+        std::vector<int32_t> first = bk.idx;
+        Shape sh = batched_shape(ctx, W, max_len, nw_hint);
+        HIPCHK(hipMemcpyAsync(d_order.p, first.data(), first.size() * sizeof(int32_t), hipMemcpyHostToDevice, ctx->stream),
+               "hipMemcpyAsync (order)");
+        double kms = 0;
+        int r = run_shape(ctx, b, sh, W, d_order.p, (int32_t)first.size(), max_len, 0, nullptr, &kms);
+        if (r != TSU_OK) return r;
+        kms_total += kms;
+        ++launches;
+        HIPCHK(hipMemcpy(tmp.data(), b->d_scores, (size_t)b->n * sizeof(int32_t), hipMemcpyDeviceToHost), "hipMemcpy D2H (scores)");
+        std::vector<int32_t> again;
+        for (int32_t i : first) {
+            scores[i] = tmp[(size_t)i];
+            if (tmp[(size_t)i] == TSU_EKOVERFLOW) again.push_back(i);
+        }
+        if (again.empty()) continue;
+        if (W >= capacity_full(max_len, max_len)) return fail(ctx, TSU_EINTERNAL, "overflow at full capacity");
+        retried += (int)again.size();
+        auto d = std::make_unique<Deferred>();
+        d->ids = std::move(again);
+        d->W = (int32_t)std::min<int64_t>(2ll * W, capacity_full(max_len, max_len));
+        d->max_len = max_len;
+        HIPCHK(d->d_ids.alloc(d->ids.size()), "hipMalloc (order)");
+        HIPCHK(hipEventCreate(&d->e0), "hipEventCreate");
+        HIPCHK(hipEventCreate(&d->e1), "hipEventCreate");
+        HIPCHK(hipMemcpyAsync(d->d_ids.p, d->ids.data(), d->ids.size() * sizeof(int32_t), hipMemcpyHostToDevice, ctx->stream2),
+               "hipMemcpyAsync (order)");
+        r = launch_shape(ctx, b, batched_shape(ctx, d->W, max_len, 0), d->W, d->d_ids.p, (int32_t)d->ids.size(), max_len, 0,
+                         nullptr, ctx->stream2, ctx->d_queue + 16, d->e0, d->e1, d->ring, &d->launched);
+        if (r != TSU_OK) { (void)hipStreamSynchronize(ctx->stream2); return r; }
+        ++launches;
+        deferred.push_back(std::move(d));
+    }
+    // collect the deferred re-runs; pairs that overflow again widen further
+    HIPCHK(hipStreamSynchronize(ctx->stream2), "hipStreamSynchronize");
+    for (auto& d : deferred) {
+        if (d->launched) {
+            float ms = 0.f;
+            HIPCHK(hipEventElapsedTime(&ms, d->e0, d->e1), "hipEventElapsedTime");
+            kms_total += ms;
+        }
+    }
+    if (!deferred.empty())
+        HIPCHK(hipMemcpy(tmp.data(), b->d_scores, (size_t)b->n * sizeof(int32_t), hipMemcpyDeviceToHost), "hipMemcpy D2H (scores)");
+    for (auto& d : deferred) {
+        std::vector<int32_t> again;
+        for (int32_t i : d->ids) {
+            scores[i] = tmp[(size_t)i];
+            if (tmp[(size_t)i] == TSU_EKOVERFLOW) again.push_back(i);
+        }
+        if (again.empty()) continue;
+        if (d->W >= capacity_full(d->max_len, d->max_len)) return fail(ctx, TSU_EINTERNAL, "overflow at full capacity");
+        retried += (int)again.size();
+        int r = settle(again, (int32_t)std::min<int64_t>(2ll * d->W, capacity_full(d->max_len, d->max_len)), d->max_len, 0);
+        if (r != TSU_OK) return r;
     }
     if (t) { t->kernel_ms += kms_total; t->launches += launches; t->retried += retried; }
     return TSU_OK;
