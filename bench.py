@@ -52,6 +52,8 @@ WORKLOADS = {
     "cfg2": (1_000_000, 1000, 1000, 0.05, 0.05, 2),
     "cfg3": (50_000, 10_000, 10_000, 0.10, 0.10, 3),
     "cfg4": (500_000, 100, 20_000, 0.01, 0.15, 4),
+    # long reads for the backtrace-memory comparison (--form cigar)
+    "long100k": (256, 100_000, 100_000, 0.05, 0.05, 6),
 }
 
 
@@ -288,6 +290,9 @@ def main():
     ap.add_argument("--no-autotune", action="store_true", help="batched: use the default shapes")
     ap.add_argument("--tune-sample", type=int, default=4096, help="batched: autotuner sample per bucket")
     ap.add_argument("--compare-plain", action="store_true", help="batched, served: also time the plain form")
+    ap.add_argument("--cigar-memory", default="auto", choices=["auto", "records", "checkpoints"],
+                    help="cigar: backtrace state (records for every score, or ring checkpoints)")
+    ap.add_argument("--checkpoint-every", type=int, default=0, help="cigar: scores between checkpoints, 0 = automatic")
     ap.add_argument("--requests", type=int, default=3000, help="served: requests per rate")
     ap.add_argument("--serve-budget", type=int, default=16384, help="served: HBM budget in MiB")
     ap.add_argument("--inflight", type=int, default=4, help="served: launches in flight")
@@ -342,6 +347,9 @@ def main():
     b = a.batch(ds)
     log(f"batch resident: {b.nbytes / 1e6:.1f} MB")
 
+    if args.form == "cigar":
+        mode = {"auto": a.CIGAR_AUTO, "records": a.CIGAR_RECORDS, "checkpoints": a.CIGAR_CHECKPOINTS}[args.cigar_memory]
+        a.set_cigar_memory(mode, args.checkpoint_every)
     tune = None
     if args.form == "batched" and not args.no_autotune:
         t0 = time.perf_counter()
@@ -434,6 +442,10 @@ def main():
         "cpu_baseline": cpu,
         "device": info["name"],
     }
+    if args.form == "cigar":
+        out["config"]["cigar_memory"] = args.cigar_memory
+        out["backtrace"] = {"state_bytes": t.get("state_bytes"), "record_launches": t.get("record_launches"),
+                            "checkpoint_launches": t.get("checkpoint_launches")}
     if tune is not None:
         out["tune"] = tune
     if plain is not None:

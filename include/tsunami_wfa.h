@@ -215,6 +215,28 @@ int tsu_cigars_get(const tsu_cigars* c, int64_t i, const uint32_t** runs, int32_
 int tsu_cigars_string(const tsu_cigars* c, int64_t i, char* buf, int32_t cap);
 void tsu_cigars_free(tsu_cigars* c);
 
+/* Backtrace memory of tsu_align_cigar: low-memory backtrace for long reads
+ * (the reference reserves the backtrace types and packed-CIGAR fields,
+ * common.h:10,17-23,48-49, but computes no CIGAR).  Per-cell records for
+ * every score take O(score * W) words per pair; ring checkpoints every K
+ * scores take O((score / K + K) * W) and recompute each window of K scores
+ * once, giving the same CIGARs.
+ *   mode             TSU_CIGAR_AUTO: records when a launch can hold them for
+ *                    at least one pair per CU within the budget, else
+ *                    checkpoints; TSU_CIGAR_RECORDS / TSU_CIGAR_CHECKPOINTS
+ *                    force one
+ *   checkpoint_every K, 0 = about sqrt(score * ring slots) per launch
+ *   budget_bytes     device memory one launch may hold for backtrace state,
+ *                    0 = half the free device memory */
+#define TSU_CIGAR_AUTO         0
+#define TSU_CIGAR_RECORDS      1
+#define TSU_CIGAR_CHECKPOINTS  2
+int tsu_set_cigar_memory(tsu_ctx* ctx, int32_t mode, int32_t checkpoint_every, int64_t budget_bytes);
+/* What the call that made c used: the largest backtrace state of one launch
+ * (device bytes) and the launches of each kind.  Any pointer may be NULL. */
+int tsu_cigars_stats(const tsu_cigars* c, int64_t* state_bytes, int32_t* record_launches,
+                     int32_t* checkpoint_launches);
+
 /* ---- serving (no reference counterpart) ---------------------------------- */
 
 typedef struct {

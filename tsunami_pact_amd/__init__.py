@@ -45,6 +45,7 @@ EXPORTS = (
     "tsu_batch_from_dataset", "tsu_batch_create", "tsu_batch_size", "tsu_batch_bytes", "tsu_batch_free",
     "tsu_align_plain", "tsu_align_fixed", "tsu_align_batched", "tsu_autotune", "tsu_tune_table",
     "tsu_align_cigar", "tsu_cigars_get", "tsu_cigars_string", "tsu_cigars_free",
+    "tsu_set_cigar_memory", "tsu_cigars_stats",
     "tsu_serve_open", "tsu_serve_submit", "tsu_serve_submit_packed", "tsu_serve_poll", "tsu_serve_wait",
     "tsu_serve_close",
     "tsu_calibrate_copy", "tsu_sync",
@@ -143,6 +144,8 @@ def library() -> ctypes.CDLL:
         "tsu_cigars_get": (ctypes.c_int, [vp, i64, P(P(ctypes.c_uint32)), P(i32)]),
         "tsu_cigars_string": (ctypes.c_int, [vp, i64, ctypes.c_char_p, i32]),
         "tsu_cigars_free": (None, [vp]),
+        "tsu_set_cigar_memory": (ctypes.c_int, [vp, i32, i32, i64]),
+        "tsu_cigars_stats": (ctypes.c_int, [vp, P(i64), P(i32), P(i32)]),
         "tsu_serve_open": (ctypes.c_int, [vp, P(ServeCfg), P(vp)]),
         "tsu_serve_submit": (ctypes.c_int, [vp, P(ctypes.c_char_p), P(i32), P(ctypes.c_char_p), P(i32), i32, P(ctypes.c_uint64)]),
         "tsu_serve_submit_packed": (ctypes.c_int, [vp, P(ctypes.c_uint32), i64, P(Pair), i32, P(ctypes.c_uint64)]),
@@ -399,10 +402,23 @@ class Aligner:
         return [{"waves_per_pair": arr[i].waves_per_pair, "capacity": arr[i].capacity,
                  "bucket_bound": arr[i].bucket_bound} for i in range(min(n.value, 64))]
 
+    CIGAR_AUTO, CIGAR_RECORDS, CIGAR_CHECKPOINTS = 0, 1, 2
+
+    def set_cigar_memory(self, mode: int = 0, checkpoint_every: int = 0, budget_bytes: int = 0) -> None:
+        """Backtrace memory of align_cigar: CIGAR_AUTO (records unless they
+        do not fit one pair per CU within the budget), CIGAR_RECORDS or
+        CIGAR_CHECKPOINTS (ring snapshots every checkpoint_every scores, 0 =
+        chosen per launch).  budget_bytes: device memory one launch may hold
+        for backtrace state, 0 = half the free memory."""
+        self._check(self._lib.tsu_set_cigar_memory(self._h, mode, checkpoint_every, budget_bytes),
+                    "tsu_set_cigar_memory")
+
     def align_cigar(self, batch: Batch, strings: bool = True):
         """(scores, CIGAR strings, timing).  With strings=False the CIGARs are
         computed and copied to the host by the library as usual but not
-        turned into Python strings (None is returned in their place)."""
+        turned into Python strings (None is returned in their place).  The
+        timing dict also holds the backtrace state: state_bytes (largest of
+        one launch), record_launches and checkpoint_launches."""
         n = len(batch)
         scores = np.zeros(max(n, 1), np.int32)
         t = Timing()
@@ -416,9 +432,14 @@ class Aligner:
             for i in range(n if strings else 0):
                 self._check(self._lib.tsu_cigars_get(h, i, ctypes.byref(rp), ctypes.byref(nr)), "tsu_cigars_get")
                 cigars.append(cigar_string(rp[j] for j in range(nr.value)))
+            sb, rl, cl = ctypes.c_int64(), ctypes.c_int32(), ctypes.c_int32()
+            self._check(self._lib.tsu_cigars_stats(h, ctypes.byref(sb), ctypes.byref(rl), ctypes.byref(cl)),
+                        "tsu_cigars_stats")
         finally:
             self._lib.tsu_cigars_free(h)
-        return scores[:n], cigars, t.as_dict()
+        td = t.as_dict()
+        td.update(state_bytes=sb.value, record_launches=rl.value, checkpoint_launches=cl.value)
+        return scores[:n], cigars, td
 
     def calibrate_copy(self, nbytes: int) -> float:
         ms = ctypes.c_double()

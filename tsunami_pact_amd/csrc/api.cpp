@@ -879,7 +879,8 @@ extern "C" int tsu_align_cigar(tsu_ctx* ctx, tsu_batch* b, int32_t* scores, tsu_
     if (r != TSU_OK) return r;
     for (int32_t i = 0; i < b->n; ++i)
         if (scores[i] < 0) return fail(ctx, scores[i], "pair " + std::to_string(i) + ": " + tsu_strerror(scores[i]));
-    tsu_cigars* cg = new (std::nothrow) tsu_cigars();
+    // freed on every early return; handed to the caller at the end
+    std::unique_ptr<tsu_cigars> cg(new (std::nothrow) tsu_cigars());
     if (!cg) return fail(ctx, TSU_ENOMEM, "host allocation");
     cg->off.assign((size_t)b->n, 0);
     cg->n.assign((size_t)b->n, 0);
@@ -907,7 +908,7 @@ extern "C" int tsu_align_cigar(tsu_ctx* ctx, tsu_batch* b, int32_t* scores, tsu_
     int launches = 0;
     size_t free_b = 0, total_b = 0;
     HIPCHK(hipMemGetInfo(&free_b, &total_b), "hipMemGetInfo");
-    const int64_t budget = (int64_t)(0.5 * (double)free_b);
+    const int64_t budget = ctx->cigar_budget > 0 ? ctx->cigar_budget : (int64_t)(0.5 * (double)free_b);
     while (pos < order.size()) {
         const int32_t s0 = scores[order[pos]] / ctx->g;  // widest in this group (sorted desc)
         int32_t max_len = 0;
@@ -923,9 +924,11 @@ extern "C" int tsu_align_cigar(tsu_ctx* ctx, tsu_batch* b, int32_t* scores, tsu_
         }
         a.W = W;
         a.seq_words = seq_words_for(max_len);
+        const int64_t group_n = (int64_t)(end - pos);
         Shape sh{default_waves(W), max_len <= 32000, true, true, true};
         sh.fixed_242 = tsu_host::fixed_242(ctx);
         int64_t recs = 0;
+        int32_t step_cap = steps;
         if (int nwl = tsu_host::ml_waves(ctx, W, max_len)) {
             // 16-chunk multi-wave kernel: records at a fixed place per step and
             // chunk, (s * chunks + j) * 64 + lane
@@ -938,27 +941,60 @@ extern "C" int tsu_align_cigar(tsu_ctx* ctx, tsu_batch* b, int32_t* scores, tsu_
             recs += 64;
             if (tsu::lds_bytes(sh, a) > ctx->max_lds) { sh.lds_ring = false; sh.nw = 16; }
         }
+        int64_t per_wg = recs * 4 + (int64_t)steps * 12 + (sh.lds_ring || sh.reg_c ? 0 : tsu::ring_bytes(sh, W, ctx->rm, ctx->ri));
+        // checkpoints instead of records when records do not fit one pair per CU
+        const bool ck = ctx->cigar_mode == TSU_CIGAR_CHECKPOINTS ||
+                        (ctx->cigar_mode == TSU_CIGAR_AUTO && s0 > 0 &&
+                         per_wg * std::min<int64_t>(group_n, ctx->cu_count) > budget);
+        if (ck) {
+            sh = Shape{std::max(4, default_waves(W)), max_len <= 32000, true, true, true};
+            sh.fixed_242 = tsu_host::fixed_242(ctx);
+            sh.ck = true;
+            if (tsu::lds_bytes(sh, a) > ctx->max_lds) { sh.lds_ring = false; sh.nw = 16; }
+            const int32_t nslot = ctx->rm + 2 * ctx->ri;
+            const int32_t osz = sh.off16 ? 2 : 4;
+            // window K balancing snapshots (s0 / K of nslot * W cells) against
+            // records (K * W words)
+            int32_t K = ctx->cigar_ck_every;
+            if (K <= 0) K = (int32_t)std::max(8.0, std::round(std::sqrt((double)s0 * nslot * osz / 4.0)));
+            K = std::max<int32_t>(1, std::min<int32_t>(K, std::max<int32_t>(s0, 1)));
+            a.ck_every = K;
+            a.ck_cap = (std::max<int32_t>(s0, 1) - 1) / K + 1;
+            a.ck_head = (nslot * 8 + 15) / 16 * 16;
+            a.ck_bytes = a.ck_head + ((int64_t)nslot * W * osz + 15) / 16 * 16;
+            recs = (int64_t)K * W + 64;
+            step_cap = K + 1;
+            per_wg = recs * 4 + (int64_t)step_cap * 12 + (int64_t)a.ck_cap * a.ck_bytes +
+                     (sh.lds_ring ? 0 : tsu::ring_bytes(sh, W, ctx->rm, ctx->ri));
+        }
         int per_cu = 0;
         int rr = tsu::occupancy(sh, tsu::lds_bytes(sh, a), &per_cu);
-        if (rr != 0 || per_cu <= 0) { tsu_cigars_free(cg); return fail(ctx, TSU_EINTERNAL, "backtrace shape has no occupancy"); }
-        const int64_t per_wg = recs * 4 + (int64_t)steps * 12 + (sh.lds_ring ? 0 : tsu::ring_bytes(sh, W, ctx->rm, ctx->ri));
-        int64_t grid = std::min<int64_t>((int64_t)per_cu * ctx->cu_count, (int64_t)(end - pos));
+        if (rr != 0 || per_cu <= 0) { return fail(ctx, TSU_EINTERNAL, "backtrace shape has no occupancy"); }
+        int64_t grid = std::min<int64_t>((int64_t)per_cu * ctx->cu_count, group_n);
         grid = std::max<int64_t>(1, std::min<int64_t>(grid, budget / std::max<int64_t>(per_wg, 1)));
+        cg->state_bytes = std::max(cg->state_bytes, per_wg * grid);
+        if (ck) ++cg->checkpoint_launches; else ++cg->record_launches;
         DevBuf<uint32_t> d_rec;
         DevBuf<int32_t> d_slo;
         DevBuf<int64_t> d_soff;
+        DevBuf<uint8_t> d_ck;
         HIPCHK(d_rec.alloc((size_t)(recs * grid)), "hipMalloc (records)");
-        HIPCHK(d_slo.alloc((size_t)((int64_t)steps * grid)), "hipMalloc (steps)");
-        HIPCHK(d_soff.alloc((size_t)((int64_t)steps * grid)), "hipMalloc (steps)");
+        HIPCHK(d_slo.alloc((size_t)((int64_t)step_cap * grid)), "hipMalloc (steps)");
+        HIPCHK(d_soff.alloc((size_t)((int64_t)step_cap * grid)), "hipMalloc (steps)");
+        if (ck) {
+            a.ck_stride = (int64_t)a.ck_cap * a.ck_bytes;
+            HIPCHK(d_ck.alloc((size_t)(a.ck_stride * grid)), "hipMalloc (checkpoints)");
+            a.ck_g = d_ck.p;
+        }
         std::vector<int32_t> ids(order.begin() + (ptrdiff_t)pos, order.begin() + (ptrdiff_t)end);
         HIPCHK(hipMemcpyAsync(d_order.p, ids.data(), ids.size() * sizeof(int32_t), hipMemcpyHostToDevice, ctx->stream), "hipMemcpyAsync");
         a.rec_g = d_rec.p; a.rec_cap = recs;
-        a.step_lo_g = d_slo.p; a.step_off_g = d_soff.p; a.step_cap = steps;
+        a.step_lo_g = d_slo.p; a.step_off_g = d_soff.p; a.step_cap = step_cap;
         a.runs = d_runs.p; a.runs_cap = std::max<int64_t>(runs_total, 1); a.runs_used = d_used.p;
         a.run_off = d_run_off.p; a.run_n = d_run_n.p;
         double kms = 0;
         r = run_shape(ctx, b, sh, W, d_order.p, (int32_t)ids.size(), max_len, grid, &a, &kms);
-        if (r != TSU_OK) { tsu_cigars_free(cg); return r; }
+        if (r != TSU_OK) return r;
         kms_total += kms;
         ++launches;
         pos = end;
@@ -967,7 +1003,7 @@ extern "C" int tsu_align_cigar(tsu_ctx* ctx, tsu_batch* b, int32_t* scores, tsu_
     HIPCHK(hipMemcpy(sc2.data(), b->d_scores, (size_t)b->n * sizeof(int32_t), hipMemcpyDeviceToHost), "hipMemcpy D2H");
     unsigned long long used = 0;
     HIPCHK(hipMemcpy(&used, d_used.p, sizeof(used), hipMemcpyDeviceToHost), "hipMemcpy D2H");
-    try { cg->runs.resize((size_t)std::max<unsigned long long>(used, 1)); } catch (...) { tsu_cigars_free(cg); return fail(ctx, TSU_ENOMEM, "host allocation"); }
+    try { cg->runs.resize((size_t)std::max<unsigned long long>(used, 1)); } catch (...) { return fail(ctx, TSU_ENOMEM, "host allocation"); }
     if (used) HIPCHK(hipMemcpy(cg->runs.data(), d_runs.p, (size_t)used * 4, hipMemcpyDeviceToHost), "hipMemcpy D2H (runs)");
     if (b->n) {
         HIPCHK(hipMemcpy(cg->off.data(), d_run_off.p, (size_t)b->n * 8, hipMemcpyDeviceToHost), "hipMemcpy D2H");
@@ -978,12 +1014,30 @@ extern "C" int tsu_align_cigar(tsu_ctx* ctx, tsu_batch* b, int32_t* scores, tsu_
     for (int32_t i = 0; i < b->n; ++i) {
         if (sc2[(size_t)i] != scores[i]) {
             const int st = sc2[(size_t)i] < 0 ? sc2[(size_t)i] : TSU_EINTERNAL;
-            tsu_cigars_free(cg);
             return fail(ctx, st, "pair " + std::to_string(i) + ": backtrace pass returned " + std::to_string(sc2[(size_t)i]) +
                                      " (score pass " + std::to_string(scores[i]) + ")");
         }
     }
-    *out = cg;
+    *out = cg.release();
+    return TSU_OK;
+}
+
+extern "C" int tsu_set_cigar_memory(tsu_ctx* ctx, int32_t mode, int32_t checkpoint_every, int64_t budget_bytes) {
+    if (!ctx) return TSU_EINVAL;
+    if (mode < TSU_CIGAR_AUTO || mode > TSU_CIGAR_CHECKPOINTS || checkpoint_every < 0 || budget_bytes < 0)
+        return fail(ctx, TSU_EINVAL, "tsu_set_cigar_memory: bad arguments");
+    ctx->cigar_mode = mode;
+    ctx->cigar_ck_every = checkpoint_every;
+    ctx->cigar_budget = budget_bytes;
+    return TSU_OK;
+}
+
+extern "C" int tsu_cigars_stats(const tsu_cigars* c, int64_t* state_bytes, int32_t* record_launches,
+                                int32_t* checkpoint_launches) {
+    if (!c) return TSU_EINVAL;
+    if (state_bytes) *state_bytes = c->state_bytes;
+    if (record_launches) *record_launches = c->record_launches;
+    if (checkpoint_launches) *checkpoint_launches = c->checkpoint_launches;
     return TSU_OK;
 }
 

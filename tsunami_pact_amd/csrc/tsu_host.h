@@ -28,6 +28,10 @@ struct tsu_ctx {
     char name[256] = {0};
     std::string err;
     std::vector<tsu_tune_entry> tune;   // per-bucket shapes (autotuner)
+    // backtrace memory of tsu_align_cigar (tsu_set_cigar_memory)
+    int32_t cigar_mode = TSU_CIGAR_AUTO;
+    int32_t cigar_ck_every = 0;         // 0 = chosen per launch
+    int64_t cigar_budget = 0;           // 0 = half the free device memory
     // scratch reused across calls
     int32_t* d_queue = nullptr;
 };
@@ -49,6 +53,9 @@ struct tsu_cigars {
     std::vector<uint32_t> runs;
     std::vector<int64_t> off;
     std::vector<int32_t> n;
+    int64_t state_bytes = 0;       // largest backtrace state of one launch (device)
+    int32_t record_launches = 0;   // launches with records for every score
+    int32_t checkpoint_launches = 0;
 };
 
 namespace tsu_host {

@@ -58,6 +58,15 @@ struct KArgs {
     unsigned long long* runs_used;
     int64_t* run_off;          // per pair
     int32_t* run_n;            // per pair
+    // checkpointed backtrace: ring snapshots every ck_every score steps, one
+    // region of ck_cap snapshots (ck_bytes each) per workgroup; the records
+    // above (rec_cap, step_cap) then hold one window of ck_every steps
+    uint8_t* ck_g;
+    int64_t ck_stride;         // bytes per workgroup
+    int64_t ck_bytes;          // bytes per snapshot: slot limits (ck_head bytes), then W cells per slot
+    int32_t ck_head;
+    int32_t ck_every;
+    int32_t ck_cap;
 };
 
 struct Shape {
@@ -70,6 +79,7 @@ struct Shape {
     int reg_c = 0;           // > 0: register-resident kernel, reg_c chunks of 64 diagonals per wave (nw waves)
     bool lring = false;      // one wave: the older M scores in an int16 LDS ring (sequences up to 16000 bases)
     bool two_piece = false;  // gap-affine 2-piece kernel (one wave per pair, rings in LDS)
+    bool ck = false;         // backtrace from ring checkpoints (bt and queue set, LDS kernel)
 };
 
 // LDS bytes a launch of this shape needs.

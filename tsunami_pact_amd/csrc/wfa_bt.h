@@ -20,76 +20,111 @@
 
 namespace tsu {
 
-// record(ss, kk, r) returns false when (ss, kk) has no record.  Penalties are
-// the scaled ones of the score loop.  Returns 0 or a negative status; on
-// success *out_off / *out_n locate the runs in a.runs.
-template <typename RecFn>
-__device__ __forceinline__ int bt_walk(RecFn record, int score, int p, int t, int x, int o, int e, const KArgs& a,
-                                       int64_t* out_off, int32_t* out_n) {
-    int st = 0;
-    const int unit = min(x, o + e);
-    const int64_t need = 2 * (int64_t)(score / unit) + 2;
-    const unsigned long long base = atomicAdd(a.runs_used, (unsigned long long)need);
-    if ((int64_t)base + need > a.runs_cap) return TSU_ECIGAR;
-    uint32_t* out = a.runs + base;
-    int64_t nr = 0;
-    uint32_t cur_op = 4, cur_len = 0;
-    bool full = false;
-    auto put = [&](uint32_t run) __attribute__((always_inline)) {
+// Walk state.  A walk may run in pieces: bt_run stops before it needs a
+// record at a score <= floor, so a kernel that keeps only the records of a
+// window of scores (recomputed from a checkpoint) refills the window below
+// and resumes.  bt_walk is the one-piece form.
+struct BtWalk {
+    uint32_t* out;
+    int64_t base, need, nr;
+    uint32_t cur_op, cur_len;
+    bool full;
+    int k, off, ss, type, v, h;  // type 0 M, 1 I, 2 D
+    int x, o, e;
+
+    __device__ __forceinline__ void put(uint32_t run) {
         if (nr >= need) { full = true; return; }
         out[need - 1 - nr] = run;
         ++nr;
-    };
-    auto emit = [&](uint32_t op, int n) __attribute__((always_inline)) {
+    }
+    __device__ __forceinline__ void emit(uint32_t op, int n) {
         if (n <= 0) return;
         if (op == cur_op) { cur_len += (uint32_t)n; return; }
         if (cur_len) put((cur_len << 2) | cur_op);
         cur_op = op; cur_len = (uint32_t)n;
-    };
-    int k = t - p, off = t, ss = score, type = 0;  // 0 M, 1 I, 2 D
-    int v = p, h = t;
-    while (v > 0 && h > 0 && ss > 0) {
+    }
+};
+
+// Reserves the walk's runs (one atomic add) and sets it at the end cell.
+// Penalties are the scaled ones of the score loop.  Returns 0 or TSU_ECIGAR.
+__device__ __forceinline__ int bt_begin(BtWalk& w, int score, int p, int t, int x, int o, int e, const KArgs& a) {
+    const int unit = min(x, o + e);
+    w.need = 2 * (int64_t)(score / unit) + 2;
+    w.base = (int64_t)atomicAdd(a.runs_used, (unsigned long long)w.need);
+    if (w.base + w.need > a.runs_cap) return TSU_ECIGAR;
+    w.out = a.runs + w.base;
+    w.nr = 0;
+    w.cur_op = 4; w.cur_len = 0;
+    w.full = false;
+    w.k = t - p; w.off = t; w.ss = score; w.type = 0;
+    w.v = p; w.h = t;
+    w.x = x; w.o = o; w.e = e;
+    return 0;
+}
+
+// record(ss, kk, r) returns false when (ss, kk) has no record.  Returns 0
+// when the walk has reached its start (row or column 0, or score 0), 1 when
+// it stopped before a record at a score <= floor, or a negative status.
+template <typename RecFn>
+__device__ __forceinline__ int bt_run(BtWalk& w, RecFn record, int floor) {
+    while (w.v > 0 && w.h > 0 && w.ss > 0) {
+        if (w.ss <= floor) return 1;
         uint32_t r;
-        if (!record(ss, k, r)) { st = TSU_EINTERNAL; break; }
-        if (type == 0) {
+        if (!record(w.ss, w.k, r)) return TSU_EINTERNAL;
+        if (w.type == 0) {
             const int pre = (int)(r >> 4);
             const uint32_t mw = r & 3u;
-            if (mw == 0 || pre > off) { st = TSU_EINTERNAL; break; }
-            emit(0, off - pre);
-            off = pre; v = off - k; h = off;
-            if (v <= 0 || h <= 0) break;
-            if (mw == 1) { emit(1, 1); ss -= x; --off; v = off - k; h = off; continue; }
-            type = (mw == 2) ? 2 : 1;
+            if (mw == 0 || pre > w.off) return TSU_EINTERNAL;
+            w.emit(0, w.off - pre);
+            w.off = pre; w.v = w.off - w.k; w.h = w.off;
+            if (w.v <= 0 || w.h <= 0) break;
+            if (mw == 1) { w.emit(1, 1); w.ss -= w.x; --w.off; w.v = w.off - w.k; w.h = w.off; continue; }
+            w.type = (mw == 2) ? 2 : 1;
         }
-        if (type == 1) {
+        if (w.type == 1) {
             const bool ext = (r & 4u) != 0;
-            emit(2, 1); --k; --off;
-            ss -= ext ? e : (o + e);
-            type = ext ? 1 : 0;
+            w.emit(2, 1); --w.k; --w.off;
+            w.ss -= ext ? w.e : (w.o + w.e);
+            w.type = ext ? 1 : 0;
         } else {
             const bool ext = (r & 8u) != 0;
-            emit(3, 1); ++k;
-            ss -= ext ? e : (o + e);
-            type = ext ? 2 : 0;
+            w.emit(3, 1); ++w.k;
+            w.ss -= ext ? w.e : (w.o + w.e);
+            w.type = ext ? 2 : 0;
         }
-        v = off - k; h = off;
+        w.v = w.off - w.k; w.h = w.off;
     }
-    if (st == 0) {
-        if (type == 0) {
-            if (v > 0 && h > 0) { const int nm = min(v, h); emit(0, nm); v -= nm; h -= nm; }
-            emit(3, v);
-            emit(2, h);
-        } else if (v != 0 || h != 0) {
-            st = TSU_EINTERNAL;
-        }
+    return 0;
+}
+
+// Emits the leading gap / match runs of a finished walk and locates its runs
+// in a.runs (*out_off, *out_n).  Returns 0 or a negative status.
+__device__ __forceinline__ int bt_end(BtWalk& w, int64_t* out_off, int32_t* out_n) {
+    if (w.type == 0) {
+        if (w.v > 0 && w.h > 0) { const int nm = min(w.v, w.h); w.emit(0, nm); w.v -= nm; w.h -= nm; }
+        w.emit(3, w.v);
+        w.emit(2, w.h);
+    } else if (w.v != 0 || w.h != 0) {
+        return TSU_EINTERNAL;
     }
-    if (st == 0) {
-        if (cur_len) put((cur_len << 2) | cur_op);
-        if (full) return TSU_ECIGAR;
-        *out_off = (int64_t)base + need - nr;
-        *out_n = (int32_t)nr;
-    }
-    return st;
+    if (w.cur_len) w.put((w.cur_len << 2) | w.cur_op);
+    if (w.full) return TSU_ECIGAR;
+    *out_off = w.base + w.need - w.nr;
+    *out_n = (int32_t)w.nr;
+    return 0;
+}
+
+// One-piece walk over records that are all present.  Returns 0 or a
+// negative status; on success *out_off / *out_n locate the runs in a.runs.
+template <typename RecFn>
+__device__ __forceinline__ int bt_walk(RecFn record, int score, int p, int t, int x, int o, int e, const KArgs& a,
+                                       int64_t* out_off, int32_t* out_n) {
+    BtWalk w;
+    int st = bt_begin(w, score, p, t, x, o, e, a);
+    if (st) return st;
+    st = bt_run(w, record, 0);
+    if (st) return st;
+    return bt_end(w, out_off, out_n);
 }
 
 }  // namespace tsu

@@ -36,6 +36,17 @@
 // walk in wfa_bt.h), stored step after step with the step's lowest diagonal
 // and first record kept per step; thread 0 walks them back from the end cell
 // into run-length CIGARs.
+//
+// Checkpointed backtrace (CK, with BT): records for every score take
+// O(score * W) words per pair.  Instead, the forward pass records nothing
+// and saves the ring (every slot's live range and limits) after every
+// ck_every-th score; the backward pass then restores the snapshot below the
+// walk's current score, recomputes that window of ck_every scores with
+// records, and lets the walk continue through it (checkpointing as in
+// Grice, Hughey & Speck, "Reduced space sequence alignment", CABIOS 1997).
+// The recomputed cells are the forward pass's cells, so the CIGAR is the one
+// full records give, for O((score / ck_every + ck_every) * W) words and
+// about twice the score pass's work.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -102,8 +113,9 @@ struct Pen {
     static constexpr int ri = fixed ? ES + 1 : 0;                  // I/D ring depth
 };
 
-template <int NW, typename OffT, bool LDS_RING, bool BT, bool QUEUE, int XS, int OS, int ES>
+template <int NW, typename OffT, bool LDS_RING, bool BT, bool QUEUE, int XS, int OS, int ES, bool CK = false>
 __global__ __launch_bounds__(NW * 64) void wfa_kernel(KArgs a) {
+    static_assert(!CK || (BT && QUEUE), "checkpointed backtrace runs in persistent backtrace launches");
     using PP = Pen<XS, OS, ES>;
     extern __shared__ __align__(16) uint32_t smem[];
     constexpr int NT = NW * 64;
@@ -135,6 +147,8 @@ __global__ __launch_bounds__(NW * 64) void wfa_kernel(KArgs a) {
         stepLo = a.step_lo_g + (int64_t)blockIdx.x * a.step_cap;
         stepOff = a.step_off_g + (int64_t)blockIdx.x * a.step_cap;
     }
+    uint8_t* ckb = nullptr;
+    if constexpr (CK) ckb = a.ck_g + (int64_t)blockIdx.x * a.ck_stride;
 
     const int x = PP::fixed ? XS : a.x;
     const int o = PP::fixed ? OS : a.o;
@@ -162,7 +176,7 @@ __global__ __launch_bounds__(NW * 64) void wfa_kernel(KArgs a) {
             sP[i] = i < pw ? a.words[pr.p_off + i] : 0u;
             sT[i] = i < tw ? a.words[pr.t_off + i] : 0u;
         }
-        {
+        auto fill_ring = [&]() __attribute__((always_inline)) {
             // slots are padded to a multiple of 16 bytes: fill with uint4
             const int64_t nvec = (int64_t)(nslot + 1) * stride * (int64_t)sizeof(OffT) / 16;
             const uint32_t nw32 = sizeof(OffT) == 2 ? ((uint32_t)(uint16_t)NUL | ((uint32_t)(uint16_t)NUL << 16))
@@ -170,7 +184,8 @@ __global__ __launch_bounds__(NW * 64) void wfa_kernel(KArgs a) {
             const uint4 fill = make_uint4(nw32, nw32, nw32, nw32);
             uint4* r4 = reinterpret_cast<uint4*>(ring);
             for (int64_t i = tid; i < nvec; i += NT) r4[i] = fill;
-        }
+        };
+        fill_ring();
         for (int i = tid; i <= nslot; i += NT) sLim[i] = make_int2(1, 0);
         __syncthreads();
 
@@ -196,16 +211,20 @@ __global__ __launch_bounds__(NW * 64) void wfa_kernel(KArgs a) {
         int status = 0;
         int score = -1;
         int64_t rec_next = 0;
+        // CK: scores (sb, s] of the current window have records; the forward
+        // pass records nothing
+        int sb = 0;
+        bool rec_on = !CK;
 
         // ---- score 0: M[0][0] = 0, extended --------------------------------
         if (tid == 0) {
             const int ext = lcp_packed(sP, sT, 0, 0, min(p, t));
             ring[0 * stride + (0 - kmin + 1)] = (OffT)ext;
             sMisc[1] = (p == t && ext == t) ? 1 : 0;
-            if constexpr (BT) { rec[0] = 0u; stepLo[0] = 0; stepOff[0] = 0; }
+            if constexpr (BT && !CK) { rec[0] = 0u; stepLo[0] = 0; stepOff[0] = 0; }
         }
         lim_set(0, make_int2(0, 0));
-        if constexpr (BT) rec_next = 1;
+        if constexpr (BT && !CK) rec_next = 1;
         __syncthreads();
         if (uni(sMisc[1])) score = 0;
 
@@ -253,8 +272,10 @@ __global__ __launch_bounds__(NW * 64) void wfa_kernel(KArgs a) {
                 clear_except(It, Oi.x, Oi.y, 1, 0);
                 clear_except(Dt, Od.x, Od.y, 1, 0);
                 if constexpr (BT) {
-                    if (s >= a.step_cap) { status = TSU_ECIGAR; return false; }
-                    if (tid == 0) { stepLo[s] = 0; stepOff[s] = rec_next; }
+                    if (rec_on) {
+                        if (s - sb >= a.step_cap) { status = TSU_ECIGAR; return false; }
+                        if (tid == 0) { stepLo[s - sb] = 0; stepOff[s - sb] = rec_next; }
+                    }
                 }
                 __syncthreads();
                 push(make_int2(1, 0), make_int2(1, 0), make_int2(1, 0));
@@ -269,7 +290,7 @@ __global__ __launch_bounds__(NW * 64) void wfa_kernel(KArgs a) {
             if (lo < kmin || hi > kmax) { status = TSU_EKOVERFLOW; return false; }
             const int width = hi - lo + 1;
             if constexpr (BT) {
-                if (s >= a.step_cap || rec_next + width > a.rec_cap) { status = TSU_ECIGAR; return false; }
+                if (rec_on && (s - sb >= a.step_cap || rec_next + width > a.rec_cap)) { status = TSU_ECIGAR; return false; }
             }
             const bool hasI = !no || !ni;
             const bool hasD = !no || !nd;
@@ -305,7 +326,7 @@ __global__ __launch_bounds__(NW * 64) void wfa_kernel(KArgs a) {
                 int32_t M = max(D, max(X, I));
                 if ((uint32_t)M > (uint32_t)t || (uint32_t)(M - k) > (uint32_t)p || !inr) M = NUL;
                 if constexpr (BT) {
-                    if (inr) {
+                    if (rec_on && inr) {
                         uint32_t r = 0;
                         if (M >= 0) {
                             const uint32_t mw = (X >= D && X >= I) ? 1u : (D >= I ? 2u : 3u);
@@ -405,19 +426,111 @@ __global__ __launch_bounds__(NW * 64) void wfa_kernel(KArgs a) {
             if (tloD != lo || thiD != hi) clear_except(Dt, lo, hi, tloD, thiD);
             push(make_int2(tloM, thiM), make_int2(tloI, thiI), make_int2(tloD, thiD));
             if constexpr (BT) {
-                if (tid == 0) { stepLo[s] = lo; stepOff[s] = rec_next; }
-                rec_next += width;
+                if (rec_on) {
+                    if (tid == 0) { stepLo[s - sb] = lo; stepOff[s - sb] = rec_next; }
+                    rec_next += width;
+                }
             }
             __syncthreads();
             if (fin) { score = s; return false; }
             return true;
         };
 
-        if (score < 0)
-            while (step(0)) {}
+        // CK snapshot c: limits of every slot, then W cells per slot (the
+        // live range of each; the ring is NULL elsewhere)
+        auto ck_save = [&](int c) __attribute__((always_inline)) {
+            uint8_t* base = ckb + (int64_t)c * a.ck_bytes;
+            int2* cl = reinterpret_cast<int2*>(base);
+            OffT* cc = reinterpret_cast<OffT*>(base + a.ck_head);
+            for (int i = 0; i < nslot; ++i) {
+                const int2 L = lim_get(i);
+                if (tid == 0) cl[i] = L;
+                for (int k = L.x + tid; k <= L.y; k += NT) cc[(int64_t)i * a.W + (k - kmin)] = ring[i * stride + (k - kmin + 1)];
+            }
+            __syncthreads();  // the next step overwrites the oldest slots
+        };
+        auto ck_load = [&](int c) __attribute__((always_inline)) {
+            const uint8_t* base = ckb + (int64_t)c * a.ck_bytes;
+            const int2* cl = reinterpret_cast<const int2*>(base);
+            const OffT* cc = reinterpret_cast<const OffT*>(base + a.ck_head);
+            __syncthreads();  // the previous window's readers are done
+            fill_ring();
+            __syncthreads();
+            for (int i = 0; i < nslot; ++i) {
+                const int2 L = cl[i];
+                const int lo = uni(L.x), hi = uni(L.y);
+                if (tid == 0) sLim[i] = make_int2(lo, hi);
+                for (int k = lo + tid; k <= hi; k += NT) ring[i * stride + (k - kmin + 1)] = cc[(int64_t)i * a.W + (k - kmin)];
+            }
+            __syncthreads();
+        };
+
+        if (score < 0) {
+            if constexpr (CK) {
+                ck_save(0);
+                while (step(0)) {
+                    if (s % a.ck_every == 0) {
+                        if (s / a.ck_every >= a.ck_cap) { status = TSU_ECIGAR; break; }
+                        ck_save(s / a.ck_every);
+                    }
+                }
+            } else {
+                while (step(0)) {}
+            }
+        }
 
         // ---- outputs --------------------------------------------------------
-        if constexpr (BT) {
+        if constexpr (CK) {
+            // backward pass: window w covers scores (w*K, min((w+1)*K, score)]
+            int32_t st = status;
+            int64_t out_off = 0;
+            int32_t out_n = 0;
+            BtWalk wk;
+            if (st == 0 && tid == 0) st = bt_begin(wk, score, p, t, x, o, e, a);
+            if (tid == 0) sMisc[9] = st;
+            __syncthreads();
+            st = uni(sMisc[9]);
+            const int K = a.ck_every;
+            int w = score > 0 ? (score - 1) / K : 0;
+            while (st == 0 && score > 0) {
+                ck_load(w);
+                s = w * K; sb = s;
+                rcM = s % rm; rcI = s % ri;
+                rec_next = 0;
+                rec_on = true;
+                const int s_end = min((w + 1) * K, score);
+                while (s < s_end && step(0)) {}
+                if (status) { st = status; break; }
+                if (tid == 0) {
+                    auto record = [&](int ss, int kk, uint32_t& r) __attribute__((always_inline)) -> bool {
+                        if (ss <= sb || ss > s) return false;
+                        const int l0 = stepLo[ss - sb];
+                        const int64_t o0 = stepOff[ss - sb];
+                        const int64_t o1 = (ss < s) ? stepOff[ss - sb + 1] : rec_next;
+                        const int64_t at = o0 + (kk - l0);
+                        if (kk < l0 || at >= o1) return false;
+                        r = rec[at];
+                        return true;
+                    };
+                    int rr = bt_run(wk, record, sb);
+                    // 1: continue in the window below the walk's score
+                    sMisc[9] = rr < 0 ? rr : 0;
+                    sMisc[10] = rr == 1 ? (wk.ss - 1) / K : -1;
+                }
+                __syncthreads();
+                st = uni(sMisc[9]);
+                const int nw_ = uni(sMisc[10]);
+                __syncthreads();
+                if (st || nw_ < 0) break;
+                w = nw_;
+            }
+            if (tid == 0) {
+                if (st == 0) st = bt_end(wk, &out_off, &out_n);
+                a.run_off[pi] = out_off;
+                a.run_n[pi] = out_n;
+                a.scores[pi] = st ? st : score * a.g;
+            }
+        } else if constexpr (BT) {
             if (tid == 0) {
                 int32_t st = status;
                 int64_t out_off = 0;
@@ -454,9 +567,20 @@ inline KernelFn fn(bool fixed_242) {
     return fixed_242 ? wfa_kernel<NW, OffT, LDS, BT, Q, 2, 3, 1> : wfa_kernel<NW, OffT, LDS, BT, Q, 0, 0, 0>;
 }
 
+template <int NW, typename OffT, bool LDS>
+inline KernelFn fn_ck(bool fixed_242) {
+    return fixed_242 ? wfa_kernel<NW, OffT, LDS, true, true, 2, 3, 1, true> : wfa_kernel<NW, OffT, LDS, true, true, 0, 0, 0, true>;
+}
+
 template <int NW>
 KernelFn pick_nw_impl(const Shape& sh) {
     const bool f = sh.fixed_242;
+    if (sh.ck) {
+        // checkpointed backtrace: persistent launches of 4 or more waves
+        if (!sh.queue || !sh.bt || NW < 4) return nullptr;
+        if (sh.off16) return sh.lds_ring ? fn_ck<NW, int16_t, true>(f) : fn_ck<NW, int16_t, false>(f);
+        return sh.lds_ring ? fn_ck<NW, int32_t, true>(f) : fn_ck<NW, int32_t, false>(f);
+    }
     if (!sh.queue) {
         // plain launches: ring in LDS, no backtrace
         if (!sh.lds_ring || sh.bt) return nullptr;
