@@ -82,6 +82,11 @@ __global__ __launch_bounds__(64) void wfa_reg1l_kernel(KArgs a) {
         }
         // step masks of scores s-1 .. s-5 (fields as in wfa_reg.h)
         uint32_t F1 = 0, F2 = 0, F3 = 0, F4 = 0, F5 = 0;
+        // per chunk: one past the last doubled offset in bounds,
+        // min(2(k + p + 1), 2(t + 1)), kept across the score loop
+        int32_t MT2[C];
+#pragma unroll
+        for (int c = 0; c < C; ++c) MT2[c] = min(2 * (kmin + 64 * c) + lane2 + 2 * (p + 1), 2 * (t + 1));
 
         // score 0: M[0][0] = 0, extended (every lane computes it)
         const int ext0 = lcp_interleaved(sS, 0, 0, min(p, t));
@@ -125,7 +130,7 @@ __global__ __launch_bounds__(64) void wfa_reg1l_kernel(KArgs a) {
             for (int c = 0; c < C; ++c) {
                 if (act & (1u << c)) {
                     const int k2 = 2 * (kmin + zero_s + 64 * c) + lane2;
-                    const int mt2 = min(k2 + 2 * (p + 1), 2 * (t + 1));
+                    const int mt2 = MT2[c];
                     const int32_t mx = Mr[PAR][c];
                     mwr[64 * c] = (int16_t)max(mx, -32768);
                     const int32_t mol = mrd[64 * c];
