@@ -14,14 +14,19 @@
 // operations of a wave complete in order, so no barrier is needed.
 //
 // Doubled offsets up to 2t must fit int16 (sequences of at most 16000
-// bases; the host uses wfa_reg.h beyond), and NULL is -32768 in the ring, so
-// the kernel gives up past kMlMaxScore scaled scores with TSU_EKOVERFLOW
-// (the same bound as the multi-wave kernel, for the same reason).
+// bases; the host uses wfa_reg.h beyond).  NULL is -32768 in the ring and
+// -2^30 in registers (an inline constant: the kernel is bound by VALU issue,
+// and a literal NULL costs more than clamping M on its way to the ring); a
+// doubled value fed only by the ring's NULL grows by two per score, so the
+// kernel gives up past kMl1MaxScore scaled scores with TSU_EKOVERFLOW.
 #pragma once
 
 #include "wfa_regml.h"
 
 namespace tsu {
+
+// -32768 + 2 * kMl1MaxScore stays negative
+constexpr int kMl1MaxScore = 16000;
 
 template <int C, bool QUEUE>
 __global__ __launch_bounds__(64) void wfa_reg1l_kernel(KArgs a) {
@@ -107,7 +112,7 @@ __global__ __launch_bounds__(64) void wfa_reg1l_kernel(KArgs a) {
             constexpr int PAR = decltype(PC)::value;
             ++s;
             if (s > s_bound) return TSU_EINTERNAL;
-            if (s > kMlMaxScore) return TSU_EKOVERFLOW;
+            if (s > kMl1MaxScore) return TSU_EKOVERFLOW;
             l3 = l3 == 2 ? 0 : l3 + 1;
             const int rd = l3 == 0 ? 2 : l3 - 1;   // ring slot of s-4
             const int wrs = l3 == 2 ? 0 : l3 + 1;  // ring slot of s-2 (formerly s-5)

@@ -23,8 +23,10 @@
 // same as for the other register kernels: a chunk with a non-NULL value at
 // score s is computed again at s+1, s+2, s+4 and s+5.
 //
-// NULL is -2^30 in registers and -32768 in the int16 ring.  An I or D value
-// fed only by the ring's NULL grows by one per score, so the kernel gives up
+// NULL is kRingNull in registers and -32768 in the int16 ring: its low half
+// is the ring's NULL, so an M register (a valid offset or exactly NULL) goes
+// to the ring by plain truncation.  An I or D value fed only by the ring's
+// NULL grows by one per score, so the kernel gives up
 // (TSU_EKOVERFLOW, the host re-runs the pair on another kernel) beyond
 // kMlMaxScore scaled scores, where such a value would still be negative;
 // offsets must fit int16, so sequences are at most 32000 bases.
@@ -44,6 +46,9 @@
 namespace tsu {
 
 constexpr int kMlMaxScore = 30000;
+// Register NULL of the kernels with an int16 M ring: large and negative, low
+// 16 bits 0x8000 (the int16 ring's NULL).
+constexpr int32_t kRingNull = (int32_t)0xC0008000u;
 
 // LDS layout (int32 words) after the interleaved sequence words.
 template <int NW, int C>
@@ -76,7 +81,7 @@ template <int NW, int C, bool QUEUE, bool BT = false>
 __global__ __launch_bounds__(NW * 64) void wfa_regml_kernel(KArgs a) {
     static_assert(NW >= 2 && NW <= 16, "2..16 waves");
     static_assert(C >= 1 && C <= 16, "16 chunk bits per wave");
-    constexpr int32_t NUL = kRegNull;
+    constexpr int32_t NUL = kRingNull;
     constexpr int NC = NW * C;
     constexpr int W = 64 * NC;
     constexpr int NT = NW * 64;
@@ -231,7 +236,7 @@ __global__ __launch_bounds__(NW * 64) void wfa_regml_kernel(KArgs a) {
                     // (>= 0 on every diagonal of the window, kmin >= -p-1)
                     const int mt1 = min(kl + (p + 1 + 64 * NW * c), t + 1);
                     const int32_t mx = Mr[PAR][c];
-                    mwr[64 * NW * c] = (int16_t)max(mx, -32768);
+                    mwr[64 * NW * c] = (int16_t)mx;  // exactly NULL or an offset below 32768
                     const int32_t mol = mrd[64 * NW * c];
                     const int32_t mor = mrd[64 * NW * c + 2];
                     const int32_t fil = ei1[c * NW];
