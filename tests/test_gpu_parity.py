@@ -325,6 +325,39 @@ def test_wide_windows_lds_ring_kernel(aligner, tsu, oracle):
     assert (got == want).all()
 
 
+def test_wide_windows_doubled_offsets(aligner, tsu, oracle):
+    """Sequences of at most 16383 bases run the 16-chunk kernel with doubled
+    offsets: 4-16 kbp pairs at 10-15% (plus random, very unequal and
+    all-mismatch pairs: I/D out of bounds at both ends) through plain windows
+    of 3000-16384 diagonals and the batched form.  An all-mismatch pair of
+    8190 bases scores 16380 scaled, just under the variant's score limit."""
+    ds = tsu.Dataset.generate(12, seed=SEED + 26, len_min=4000, len_max=16383, err_min=0.10, err_max=0.15)
+    pairs = ds.pairs()
+    rng = np.random.default_rng(26)
+    for _ in range(3):
+        pairs.append(("".join(rng.choice(list("ACGT"), int(rng.integers(2000, 6000)))),
+                      "".join(rng.choice(list("ACGT"), int(rng.integers(2000, 6000))))))
+    pairs += [("A" * 300, "C" * 7000), ("G" * 16383, "G" * 16000 + "T" * 383)]
+    want, _, _ = oracle.wfa_batch(pairs, 4, 6, 2)
+    b = aligner.batch(pairs)
+    for W in (3000, 9000, 16384):
+        say(f"doubled W={W}")
+        got, _ = aligner.align_plain(b, W, raise_on_error=False)
+        ok = got != tsu.TSU_EKOVERFLOW
+        assert (got[ok] == want[ok]).all(), W
+        if W == 16384:
+            assert ok.all()
+    say("doubled batched")
+    got, _ = aligner.align_batched(b)
+    assert (got == want).all()
+    mm = [("A" * 8190, "C" * 8190)]
+    say("doubled all-mismatch 8190")
+    got, _ = aligner.align_plain(aligner.batch(mm), 16383)
+    assert int(got[0]) == 4 * 8190
+    got, _ = aligner.align_batched(aligner.batch(mm))
+    assert int(got[0]) == 4 * 8190
+
+
 def test_autotune_then_batched(tsu, oracle):
     """The autotuner fills a per-bucket table; batched results stay exact."""
     ds = tsu.Dataset.generate(3000, seed=SEED + 12, len_min=100, len_max=3000, err_min=0.01, err_max=0.15)

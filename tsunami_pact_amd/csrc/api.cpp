@@ -409,6 +409,7 @@ static void use_register_kernel(const tsu_ctx* ctx, int32_t W, int32_t max_len, 
     } else if (int nwl = tsu_host::ml_waves(ctx, W, max_len)) {
         r.reg_c = 16;
         r.nw = nwl;
+        r.dbl = max_len <= tsu::kMlDblMaxLen;
     } else if (int nw = tsu_host::reg_waves(ctx, W)) {
         r.reg_c = 8;
         r.nw = nw;
@@ -549,7 +550,8 @@ struct Bucket {
 };
 
 // Bucket bounds on max(p, t).
-const int32_t kBucketBounds[] = {128, 256, 512, 1024, 2048, 4096, 8192, 16384, 32000, 1 << 26};
+// (16383: the longest sequence the doubled-offset 16-chunk kernel takes)
+const int32_t kBucketBounds[] = {128, 256, 512, 1024, 2048, 4096, 8192, 16383, 32000, 1 << 26};
 
 }  // namespace
 
@@ -934,6 +936,7 @@ extern "C" int tsu_align_cigar(tsu_ctx* ctx, tsu_batch* b, int32_t* scores, tsu_
             // chunk, (s * chunks + j) * 64 + lane
             sh.nw = nwl;
             sh.reg_c = 16;
+            sh.dbl = max_len <= tsu::kMlDblMaxLen && s0 <= tsu::kMlDblMaxScore;
             recs = (int64_t)steps * nwl * 16 * 64;
         } else {
             for (int64_t s = 0; s <= s0; ++s)

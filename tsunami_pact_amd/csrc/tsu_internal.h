@@ -80,7 +80,14 @@ struct Shape {
     bool lring = false;      // one wave: the older M scores in an int16 LDS ring (sequences up to 16000 bases)
     bool two_piece = false;  // gap-affine 2-piece kernel (one wave per pair, rings in LDS)
     bool ck = false;         // backtrace from ring checkpoints (bt and queue set, LDS kernel)
+    bool dbl = false;        // 16-chunk kernel with doubled offsets (sequences up to kMlDblMaxLen bases)
 };
+
+// Doubled-offset 16-chunk kernel: longest sequence (2t fits int16) and the
+// highest scaled score it runs to (-32768 + 2s, the ring's NULL after s
+// scores, stays negative).
+constexpr int32_t kMlDblMaxLen = 16383;
+constexpr int32_t kMlDblMaxScore = 16383;
 
 // LDS bytes a launch of this shape needs.
 int64_t lds_bytes(const Shape& sh, const KArgs& a);

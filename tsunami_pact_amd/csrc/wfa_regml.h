@@ -31,6 +31,14 @@
 // kMlMaxScore scaled scores, where such a value would still be negative;
 // offsets must fit int16, so sequences are at most 32000 bases.
 //
+// With DBL the offsets are kept doubled, as in the single-wave kernels: a
+// doubled offset is the bit position of its base in the packed words, which
+// saves the shifts around every extend.  Doubled offsets up to 2t fit the
+// int16 ring for sequences of at most 16383 bases, and a doubled value fed
+// only by the ring's NULL grows by two per score, so that variant gives up
+// past kMlMaxScoreDbl = 16383 scaled scores (still negative: -32768 + 2s).
+// A pair whose sequences hold at most 8190 bases cannot score more.
+//
 // Recurrences, bounds test, extend, trim and end test are those of the
 // other kernels (wfa_gpu.cu:253-331,610-631,633-719).
 //
@@ -46,6 +54,7 @@
 namespace tsu {
 
 constexpr int kMlMaxScore = 30000;
+constexpr int kMlMaxScoreDbl = kMlDblMaxScore;
 // Register NULL of the kernels with an int16 M ring: large and negative, low
 // 16 bits 0x8000 (the int16 ring's NULL).
 constexpr int32_t kRingNull = (int32_t)0xC0008000u;
@@ -77,8 +86,9 @@ constexpr uint32_t kMlEdgeBit = 1u << 16;
 
 typedef __attribute__((address_space(3))) const int32_t lds_ci32;
 
-template <int NW, int C, bool QUEUE, bool BT = false>
+template <int NW, int C, bool QUEUE, bool BT = false, bool DBL = false>
 __global__ __launch_bounds__(NW * 64) void wfa_regml_kernel(KArgs a) {
+    constexpr int U = DBL ? 2 : 1;  // offset unit: 2 when offsets are kept doubled
     static_assert(NW >= 2 && NW <= 16, "2..16 waves");
     static_assert(C >= 1 && C <= 16, "16 chunk bits per wave");
     constexpr int32_t NUL = kRingNull;
@@ -164,7 +174,7 @@ __global__ __launch_bounds__(NW * 64) void wfa_regml_kernel(KArgs a) {
                 const int r0 = j0 / NW;
 #pragma unroll
                 for (int r = 0; r < C; ++r)
-                    if (r == r0) Mr[0][r] = lane == l0 ? ext0 : NUL;
+                    if (r == r0) Mr[0][r] = lane == l0 ? U * ext0 : NUL;
                 F1 = 1u << r0;
             }
             if (lane == 0) FWp[0 * NW + w] = F1;
@@ -188,7 +198,7 @@ __global__ __launch_bounds__(NW * 64) void wfa_regml_kernel(KArgs a) {
             constexpr int PAR = decltype(PC)::value;
             ++s;
             if (s > s_bound) return TSU_EINTERNAL;
-            if (s > kMlMaxScore) return TSU_EKOVERFLOW;
+            if (s > (DBL ? kMlMaxScoreDbl : kMlMaxScore)) return TSU_EKOVERFLOW;
             if constexpr (BT) {
                 if (s >= a.step_cap) return TSU_ECIGAR;
             }
@@ -204,7 +214,7 @@ __global__ __launch_bounds__(NW * 64) void wfa_regml_kernel(KArgs a) {
             int ln = lane;
             asm volatile("" : "+s"(pb));
             asm volatile("" : "+v"(ln));
-            const int kl = kmin + pb + ln;  // diagonal of this lane in chunk 0
+            const int kl = U * (kmin + pb + ln);  // diagonal of this lane in chunk 0 (in offset units)
             uint32_t hiL = nbL & ALL;
             uint32_t loR = nbR & ALL;
             if (w == 0) hiL <<= 1;
@@ -234,7 +244,7 @@ __global__ __launch_bounds__(NW * 64) void wfa_regml_kernel(KArgs a) {
                 if (act & (1u << c)) {
                     // one past the last offset in bounds: min(k + p + 1, t + 1)
                     // (>= 0 on every diagonal of the window, kmin >= -p-1)
-                    const int mt1 = min(kl + (p + 1 + 64 * NW * c), t + 1);
+                    const int mt1 = min(kl + U * (p + 1 + 64 * NW * c), U * (t + 1));
                     const int32_t mx = Mr[PAR][c];
                     mwr[64 * NW * c] = (int16_t)mx;  // exactly NULL or an offset below 32768
                     const int32_t mol = mrd[64 * NW * c];
@@ -245,18 +255,23 @@ __global__ __launch_bounds__(NW * 64) void wfa_regml_kernel(KArgs a) {
                     const int32_t der = dpp_shl1(fdr, Dr[c]);
                     uint32_t rb = 0;  // backtrace record bits of the gap sources
                     if constexpr (BT) rb = (iel >= mol ? 4u : 0u) | (der >= mor ? 8u : 0u);
-                    const int32_t I = max(mol, iel) + 1;
+                    const int32_t I = max(mol, iel) + U;
                     const int32_t D = max(mor, der);
-                    const int32_t X = mx + 1;
+                    const int32_t X = mx + U;
                     int32_t M = max(max(D, X), I);
                     const bool valid = (uint32_t)M < (uint32_t)mt1;
                     if constexpr (BT) {
-                        const uint32_t r = rb | (valid ? ((uint32_t)M << 4) | ((X >= D && X >= I) ? 1u : (D >= I ? 2u : 3u)) : 0u);
+                        const uint32_t r = rb | (valid ? (((uint32_t)M >> (U - 1)) << 4) | ((X >= D && X >= I) ? 1u : (D >= I ? 2u : 3u)) : 0u);
                         recs[(uint32_t)(64 * NW * c) + rl] = r;
                     }
                     M = valid ? M : NUL;
-                    const int lim1 = mt1 - M;  // bases left + 1
-                    if (valid && lim1 > 1) M += extend_len((uint32_t)(M - kl - 64 * NW * c), (uint32_t)M, lim1 - 1);
+                    if constexpr (DBL) {
+                        const int lim2 = mt1 - 2 - M;  // 2 x bases left
+                        if (valid && lim2 > 0) M += extend2((uint32_t)(M - kl - 128 * NW * c), (uint32_t)M, lim2);
+                    } else {
+                        const int lim1 = mt1 - M;  // bases left + 1
+                        if (valid && lim1 > 1) M += extend_len((uint32_t)(M - kl - 64 * NW * c), (uint32_t)M, lim1 - 1);
+                    }
                     Mr[PAR][c] = M;
                     Ir[c] = I;
                     Dr[c] = D;
@@ -272,7 +287,7 @@ __global__ __launch_bounds__(NW * 64) void wfa_regml_kernel(KArgs a) {
                 }
             }
             uint32_t Fs = acc >> 1;
-            const uint32_t flags = (oob ? kFlOob : 0u) | (mend >= t ? kFlEnd : 0u) | (edge ? kFlEdge : 0u);
+            const uint32_t flags = (oob ? kFlOob : 0u) | (mend >= U * t ? kFlEnd : 0u) | (edge ? kFlEdge : 0u);
             if (lane == 0) {
                 if (flags) atomicOr(reinterpret_cast<uint32_t*>(X + L::FL + fidx), flags);
                 if (w == 0) X[L::FL + (fidx + 1 == 3 ? 0 : fidx + 1)] = 0;  // the next score's word
@@ -295,7 +310,7 @@ __global__ __launch_bounds__(NW * 64) void wfa_regml_kernel(KArgs a) {
                     int lc = ln;
                     asm volatile("" : "+v"(lc));
                     const int k = kmin + pb + 64 * NW * c + lc;
-                    const int mt1 = min(k + p + 1, t + 1);
+                    const int mt1 = U * min(k + p + 1, t + 1);
                     const int base = pb + 64 * NW * c;
                     const uint64_t bi = vote((uint32_t)Ir[c] < (uint32_t)mt1);
                     const uint64_t bd = vote((uint32_t)Dr[c] < (uint32_t)mt1);
