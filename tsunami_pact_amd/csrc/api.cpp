@@ -67,6 +67,8 @@ struct DevBuf {
         if (count == 0) return hipSuccess;
         return hipMalloc(reinterpret_cast<void**>(&p), count * sizeof(T));
     }
+    // at least count elements, keeping the buffer when it is large enough
+    hipError_t reserve(size_t count) { return p && n >= count ? hipSuccess : alloc(count); }
 };
 
 // Diagonal capacity that holds every k-range up to score bound s (scaled):
@@ -151,6 +153,7 @@ extern "C" void tsu_destroy(tsu_ctx* ctx) {
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     if (ctx->stream2) (void)hipStreamSynchronize(ctx->stream2);
     if (ctx->d_queue) (void)hipFree(ctx->d_queue);
+    if (ctx->h_stage) (void)hipHostFree(ctx->h_stage);
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
     if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -555,6 +558,10 @@ const int32_t kBucketBounds[] = {128, 256, 512, 1024, 2048, 4096, 8192, 16383, 3
 
 }  // namespace
 
+void tsu_host::order_longest_first(std::vector<int32_t>& idx, const tsu_pair* pairs) {
+    order_desc(idx, [&](int32_t i) -> int64_t { return (int64_t)pairs[(size_t)i].p_len + pairs[(size_t)i].t_len; });
+}
+
 int tsu_host::bucket_count() { return (int)(sizeof(kBucketBounds) / sizeof(kBucketBounds[0])); }
 int32_t tsu_host::bucket_bound(int i) { return kBucketBounds[i]; }
 int tsu_host::bucket_of(int32_t len) {
@@ -645,17 +652,10 @@ static int batched_core(tsu_ctx* ctx, tsu_batch* b, const std::vector<int32_t>& 
     HIPCHK(d_order.alloc(std::max<size_t>(sel.size(), 1)), "hipMalloc (order)");
     double kms_total = 0;
     int launches = 0, retried = 0;
-    // every bucket ordered longest first, all sorted up front (one thread per
+    // every bucket ordered longest first, all ordered up front (one thread per
     // bucket), so no sort sits between two launches
 #pragma omp parallel for schedule(dynamic, 1)
-    for (int bi = 0; bi < nb; ++bi) {
-        Bucket& bk = buckets[(size_t)bi];
-        std::stable_sort(bk.idx.begin(), bk.idx.end(), [&](int32_t a1, int32_t a2) {
-            const tsu_pair& x1 = b->pairs[(size_t)a1];
-            const tsu_pair& x2 = b->pairs[(size_t)a2];
-            return (int64_t)x1.p_len * x1.t_len > (int64_t)x2.p_len * x2.t_len;
-        });
-    }
+    for (int bi = 0; bi < nb; ++bi) tsu_host::order_longest_first(buckets[(size_t)bi].idx, b->pairs.data());
     std::vector<int32_t> tmp((size_t)b->n);
     // Launches `todo` at capacity W (widened while pairs overflow) and waits;
     // scores go to `scores` at their pair index.
@@ -867,6 +867,32 @@ extern "C" int tsu_tune_table(const tsu_ctx* ctx, tsu_tune_entry* entries, int32
 // ---------------------------------------------------------------------------
 // CIGAR form
 // ---------------------------------------------------------------------------
+// Device-to-host copy of a large buffer into pageable memory: slices go
+// through a pinned staging buffer kept by the context, and host threads move
+// each slice on (touching the destination's fresh pages in parallel).
+static int copy_to_host(tsu_ctx* ctx, void* dst, const void* src, size_t bytes) {
+    constexpr size_t kSlice = 64u << 20;
+    const size_t want = std::min(bytes, kSlice);
+    if (ctx->h_stage_bytes < want) {
+        if (ctx->h_stage) (void)hipHostFree(ctx->h_stage);
+        ctx->h_stage = nullptr;
+        ctx->h_stage_bytes = 0;
+        HIPCHK(hipHostMalloc(&ctx->h_stage, want, hipHostMallocDefault), "hipHostMalloc (staging)");
+        ctx->h_stage_bytes = want;
+    }
+    for (size_t off = 0; off < bytes; off += kSlice) {
+        const size_t n = std::min(kSlice, bytes - off);
+        HIPCHK(hipMemcpy(ctx->h_stage, static_cast<const uint8_t*>(src) + off, n, hipMemcpyDeviceToHost), "hipMemcpy D2H");
+        const size_t part = (n + 15) / 16;
+#pragma omp parallel for schedule(static)
+        for (int64_t i = 0; i < 16; ++i) {
+            const size_t b0 = std::min(n, (size_t)i * part), b1 = std::min(n, b0 + part);
+            if (b1 > b0) std::memcpy(static_cast<uint8_t*>(dst) + off + b0, static_cast<const uint8_t*>(ctx->h_stage) + b0, b1 - b0);
+        }
+    }
+    return TSU_OK;
+}
+
 extern "C" int tsu_align_cigar(tsu_ctx* ctx, tsu_batch* b, int32_t* scores, tsu_cigars** out, tsu_timing* t) {
     if (!ctx || !b || !scores || !out || b->ctx != ctx) return fail(ctx, TSU_EINVAL, "tsu_align_cigar: bad arguments");
     *out = nullptr;
@@ -890,7 +916,7 @@ extern "C" int tsu_align_cigar(tsu_ctx* ctx, tsu_batch* b, int32_t* scores, tsu_
     const int32_t unit = std::min(ctx->xs, ctx->os + ctx->es);
     std::vector<int32_t> order((size_t)b->n);
     std::iota(order.begin(), order.end(), 0);
-    std::stable_sort(order.begin(), order.end(), [&](int32_t a1, int32_t a2) { return scores[a1] > scores[a2]; });
+    tsu_host::order_desc(order, [&](int32_t i) -> int64_t { return scores[i]; });
     int64_t runs_total = 0;
     for (int32_t i = 0; i < b->n; ++i) runs_total += 2ll * (scores[i] / ctx->g / unit) + 2;
     DevBuf<uint32_t> d_runs;
@@ -911,6 +937,12 @@ extern "C" int tsu_align_cigar(tsu_ctx* ctx, tsu_batch* b, int32_t* scores, tsu_
     size_t free_b = 0, total_b = 0;
     HIPCHK(hipMemGetInfo(&free_b, &total_b), "hipMemGetInfo");
     const int64_t budget = ctx->cigar_budget > 0 ? ctx->cigar_budget : (int64_t)(0.5 * (double)free_b);
+    // backtrace state, grown as groups need it (the first group, of the
+    // highest scores, usually needs the most)
+    DevBuf<uint32_t> d_rec;
+    DevBuf<int32_t> d_slo;
+    DevBuf<int64_t> d_soff;
+    DevBuf<uint8_t> d_ck;
     while (pos < order.size()) {
         const int32_t s0 = scores[order[pos]] / ctx->g;  // widest in this group (sorted desc)
         int32_t max_len = 0;
@@ -977,16 +1009,12 @@ extern "C" int tsu_align_cigar(tsu_ctx* ctx, tsu_batch* b, int32_t* scores, tsu_
         grid = std::max<int64_t>(1, std::min<int64_t>(grid, budget / std::max<int64_t>(per_wg, 1)));
         cg->state_bytes = std::max(cg->state_bytes, per_wg * grid);
         if (ck) ++cg->checkpoint_launches; else ++cg->record_launches;
-        DevBuf<uint32_t> d_rec;
-        DevBuf<int32_t> d_slo;
-        DevBuf<int64_t> d_soff;
-        DevBuf<uint8_t> d_ck;
-        HIPCHK(d_rec.alloc((size_t)(recs * grid)), "hipMalloc (records)");
-        HIPCHK(d_slo.alloc((size_t)((int64_t)step_cap * grid)), "hipMalloc (steps)");
-        HIPCHK(d_soff.alloc((size_t)((int64_t)step_cap * grid)), "hipMalloc (steps)");
+        HIPCHK(d_rec.reserve((size_t)(recs * grid)), "hipMalloc (records)");
+        HIPCHK(d_slo.reserve((size_t)((int64_t)step_cap * grid)), "hipMalloc (steps)");
+        HIPCHK(d_soff.reserve((size_t)((int64_t)step_cap * grid)), "hipMalloc (steps)");
         if (ck) {
             a.ck_stride = (int64_t)a.ck_cap * a.ck_bytes;
-            HIPCHK(d_ck.alloc((size_t)(a.ck_stride * grid)), "hipMalloc (checkpoints)");
+            HIPCHK(d_ck.reserve((size_t)(a.ck_stride * grid)), "hipMalloc (checkpoints)");
             a.ck_g = d_ck.p;
         }
         std::vector<int32_t> ids(order.begin() + (ptrdiff_t)pos, order.begin() + (ptrdiff_t)end);
@@ -1006,8 +1034,12 @@ extern "C" int tsu_align_cigar(tsu_ctx* ctx, tsu_batch* b, int32_t* scores, tsu_
     HIPCHK(hipMemcpy(sc2.data(), b->d_scores, (size_t)b->n * sizeof(int32_t), hipMemcpyDeviceToHost), "hipMemcpy D2H");
     unsigned long long used = 0;
     HIPCHK(hipMemcpy(&used, d_used.p, sizeof(used), hipMemcpyDeviceToHost), "hipMemcpy D2H");
-    try { cg->runs.resize((size_t)std::max<unsigned long long>(used, 1)); } catch (...) { return fail(ctx, TSU_ENOMEM, "host allocation"); }
-    if (used) HIPCHK(hipMemcpy(cg->runs.data(), d_runs.p, (size_t)used * 4, hipMemcpyDeviceToHost), "hipMemcpy D2H (runs)");
+    cg->runs.reset(new (std::nothrow) uint32_t[(size_t)std::max<unsigned long long>(used, 1)]);
+    if (!cg->runs) return fail(ctx, TSU_ENOMEM, "host allocation");
+    if (used) {
+        int r2 = copy_to_host(ctx, cg->runs.get(), d_runs.p, (size_t)used * 4);
+        if (r2 != TSU_OK) return r2;
+    }
     if (b->n) {
         HIPCHK(hipMemcpy(cg->off.data(), d_run_off.p, (size_t)b->n * 8, hipMemcpyDeviceToHost), "hipMemcpy D2H");
         HIPCHK(hipMemcpy(cg->n.data(), d_run_n.p, (size_t)b->n * 4, hipMemcpyDeviceToHost), "hipMemcpy D2H");
@@ -1046,7 +1078,7 @@ extern "C" int tsu_cigars_stats(const tsu_cigars* c, int64_t* state_bytes, int32
 
 extern "C" int tsu_cigars_get(const tsu_cigars* c, int64_t i, const uint32_t** runs, int32_t* n_runs) {
     if (!c || i < 0 || i >= (int64_t)c->n.size()) return TSU_EINVAL;
-    if (runs) *runs = c->runs.data() + c->off[(size_t)i];
+    if (runs) *runs = c->runs.get() + c->off[(size_t)i];
     if (n_runs) *n_runs = c->n[(size_t)i];
     return TSU_OK;
 }
@@ -1055,7 +1087,7 @@ extern "C" int tsu_cigars_string(const tsu_cigars* c, int64_t i, char* buf, int3
     if (!c || i < 0 || i >= (int64_t)c->n.size()) return TSU_EINVAL;
     static const char ops[4] = {'M', 'X', 'I', 'D'};
     std::string s;
-    const uint32_t* r = c->runs.data() + c->off[(size_t)i];
+    const uint32_t* r = c->runs.get() + c->off[(size_t)i];
     for (int32_t j = 0; j < c->n[(size_t)i]; ++j) {
         s += std::to_string(r[j] >> 2);
         s += ops[r[j] & 3u];

@@ -175,11 +175,7 @@ int64_t plan_flight(tsu_server* s, Slot& sl, int* status) {
     for (int bi = nb - 1; bi >= 0; --bi) {
         std::vector<int32_t>& v = by[(size_t)bi];
         if (v.empty()) continue;
-        std::stable_sort(v.begin(), v.end(), [&](int32_t a, int32_t b) {
-            const tsu_pair& x = sl.fpairs[(size_t)a];
-            const tsu_pair& y = sl.fpairs[(size_t)b];
-            return (int64_t)x.p_len * x.t_len > (int64_t)y.p_len * y.t_len;
-        });
+        tsu_host::order_longest_first(v, sl.fpairs.data());
         LaunchPlan lp;
         lp.first = (int32_t)sl.order.size();
         lp.count = (int32_t)v.size();

@@ -3,6 +3,9 @@
 
 #include <hip/hip_runtime_api.h>
 
+#include <algorithm>
+#include <climits>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -34,6 +37,8 @@ struct tsu_ctx {
     int64_t cigar_budget = 0;           // 0 = half the free device memory
     // scratch reused across calls
     int32_t* d_queue = nullptr;
+    void* h_stage = nullptr;            // pinned staging for large device-to-host copies
+    size_t h_stage_bytes = 0;
 };
 
 struct tsu_batch {
@@ -50,7 +55,7 @@ struct tsu_batch {
 };
 
 struct tsu_cigars {
-    std::vector<uint32_t> runs;
+    std::unique_ptr<uint32_t[]> runs;  // run pool as copied from the device (not zero-filled)
     std::vector<int64_t> off;
     std::vector<int32_t> n;
     int64_t state_bytes = 0;       // largest backtrace state of one launch (device)
@@ -92,5 +97,29 @@ int reg_waves(const tsu_ctx* ctx, int32_t W);
 // Waves of the 16-chunk multi-wave kernel (older M scores in an int16 LDS
 // ring) for capacity W and sequences up to max_len (0: not eligible).
 int ml_waves(const tsu_ctx* ctx, int32_t W, int32_t max_len);
+
+// Reorders idx by key(i) descending, keeping the order of equal keys (a
+// counting sort over the key range, O(n + range), or a comparison sort when
+// the range is much wider than n).  Keys must be non-negative.
+template <typename KeyFn>
+void order_desc(std::vector<int32_t>& idx, KeyFn key) {
+    if (idx.size() < 2) return;
+    int64_t lo = INT64_MAX, hi = INT64_MIN;
+    for (int32_t i : idx) { const int64_t k = key(i); lo = k < lo ? k : lo; hi = k > hi ? k : hi; }
+    const int64_t range = hi - lo + 1;
+    if (range > 4 * (int64_t)idx.size() + 65536) {
+        std::stable_sort(idx.begin(), idx.end(), [&](int32_t a, int32_t b) { return key(a) > key(b); });
+        return;
+    }
+    std::vector<int64_t> start((size_t)range + 1, 0);
+    for (int32_t i : idx) ++start[(size_t)(hi - key(i)) + 1];
+    for (int64_t r = 1; r <= range; ++r) start[(size_t)r] += start[(size_t)r - 1];
+    std::vector<int32_t> out(idx.size());
+    for (int32_t i : idx) out[(size_t)start[(size_t)(hi - key(i))]++] = i;
+    idx.swap(out);
+}
+
+// Longest first (p + t descending): the order persistent launches pop pairs in.
+void order_longest_first(std::vector<int32_t>& idx, const tsu_pair* pairs);
 
 }  // namespace tsu_host
