@@ -413,6 +413,7 @@ static void use_register_kernel(const tsu_ctx* ctx, int32_t W, int32_t max_len, 
         r.reg_c = 16;
         r.nw = nwl;
         r.dbl = max_len <= tsu::kMlDblMaxLen;
+        r.ns = true;
     } else if (int nw = tsu_host::reg_waves(ctx, W)) {
         r.reg_c = 8;
         r.nw = nw;

@@ -70,7 +70,9 @@ struct MlLayout {
     static constexpr int FL = FW + 5 * NW;                  // [3] step flags, rotating
     static constexpr int TR = FL + 3;                       // [4] trim bounds: I first, I last, D first, D last
     static constexpr int MISC = TR + 4;                     // [4] queue index
-    static constexpr int SK = MISC + 4;                     // [64+NC] write sink of the lanes that publish nothing
+    static constexpr int PG = MISC + 4;                     // [NW] neighbour sync: last score each wave finished
+    static constexpr int EV = PG + NW;                      // [4] neighbour sync: first event, score * 4 + kind
+    static constexpr int SK = EV + 4;                       // [64+NC] write sink of the lanes that publish nothing
     static constexpr int MR = (SK + 64 + NC + 3) & ~3;      // int16 [3][W+2] M ring, NULL guards at both ends
     static constexpr int MR_STRIDE = W + 2;                 // int16 entries per slot
     static constexpr int WORDS = MR + (3 * MR_STRIDE + 1) / 2;
@@ -84,10 +86,33 @@ struct MlLayout {
 // masks would, for a third of the scalar work per chunk.
 constexpr uint32_t kMlEdgeBit = 1u << 16;
 
+// Neighbour synchronisation (NS).  Wave w's step s reads, besides its own
+// registers, only what the waves owning its neighbour chunks (w-1 and w+1,
+// cyclically) wrote at steps s-1 and s-3: their edge lanes, their ring
+// entries and their step masks.  So instead of a workgroup barrier per score,
+// each wave publishes the last score it finished and starts score s+1 once
+// both neighbours have finished s.  Neighbours are then never more than one
+// score apart, which is what the double-buffered edges, the 3-slot M ring
+// and the 5-slot masks need: a wave at s+1 overwrites only data of scores its
+// neighbours have finished reading.  What needs the whole window is turned
+// into per-wave events kept as a minimum (score * 4 + kind): the end cell
+// (kind 0), a non-NULL cell at the window edge (1), the score limit (2) and
+// an internal error (3).  Every wave goes on until it has finished the
+// lowest recorded score, so an event at a lower score in a slower wave is
+// never missed, and the pair's result is that minimum.  The exact trim is
+// replaced by NULLing every out-of-bounds I or D cell where it is computed;
+// the score is unchanged (checked against the reference semantics on every
+// pair of up to 4 bases and on random pairs), but cells differ, so the
+// backtrace variant keeps the barrier.  Spins are bounded (kMlSpinCap
+// sleeps); past it the pair fails with TSU_EINTERNAL instead of hanging.
+constexpr int kEvEnd = 0, kEvEdge = 1, kEvMax = 2, kEvInternal = 3;
+constexpr int kMlSpinCap = 1 << 20;
+
 typedef __attribute__((address_space(3))) const int32_t lds_ci32;
 
-template <int NW, int C, bool QUEUE, bool BT = false, bool DBL = false>
+template <int NW, int C, bool QUEUE, bool BT = false, bool DBL = false, bool NS = false>
 __global__ __launch_bounds__(NW * 64) void wfa_regml_kernel(KArgs a) {
+    static_assert(!(NS && BT), "the backtrace variant keeps the exact trim and the barrier");
     constexpr int U = DBL ? 2 : 1;  // offset unit: 2 when offsets are kept doubled
     static_assert(NW >= 2 && NW <= 16, "2..16 waves");
     static_assert(C >= 1 && C <= 16, "16 chunk bits per wave");
@@ -140,7 +165,9 @@ __global__ __launch_bounds__(NW * 64) void wfa_regml_kernel(KArgs a) {
             if (i >= L::FW) v = 0;                                    // masks, flags
             if (i >= L::TR && i < L::MISC) v = ((i - L::TR) & 1) ? -kBig : kBig;  // trim bounds
             if (i >= L::MR) v = (int32_t)0x80008000u;                 // two int16 NULLs
-            if (i < L::MISC || i >= L::MR) X[i] = v;
+            if (i >= L::PG && i < L::EV) v = 0;                       // progress: score 0 done
+            if (i == L::EV) v = 0x7fffffff;                           // no event yet
+            if (i < L::MISC || i >= L::PG) X[i] = v;
         }
         __syncthreads();
 
@@ -196,7 +223,35 @@ __global__ __launch_bounds__(NW * 64) void wfa_regml_kernel(KArgs a) {
         // a barrier.
         auto step = [&](auto PC) __attribute__((always_inline)) -> int {
             constexpr int PAR = decltype(PC)::value;
+            if constexpr (NS) {
+                // both neighbours have finished score s; stop once an event
+                // is recorded at a score this wave has finished
+                for (int spins = 0;; ++spins) {
+                    const int ev = __builtin_amdgcn_readfirstlane(
+                        __hip_atomic_load(X + L::EV, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP));
+                    if ((ev >> 2) <= s) return 1;
+                    const int g0 = __builtin_amdgcn_readfirstlane(
+                        __hip_atomic_load(X + L::PG + wl, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP));
+                    const int g1 = __builtin_amdgcn_readfirstlane(
+                        __hip_atomic_load(X + L::PG + wr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP));
+                    if (g0 >= s && g1 >= s) break;
+                    if (spins >= kMlSpinCap) {
+                        atomicMin(X + L::EV, ((s + 1) << 2) | kEvInternal);  // uniform: one lane's atomic
+                        return TSU_EINTERNAL;
+                    }
+                    __builtin_amdgcn_s_sleep(1);
+                }
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+                const int n3 = l5 >= 3 ? l5 - 3 : l5 + 2;  // slot of s-3
+                nbL = (uint32_t)__builtin_amdgcn_readfirstlane(FWp[l5 * NW + wl] | FWp[n3 * NW + wl]);
+                nbR = (uint32_t)__builtin_amdgcn_readfirstlane(FWp[l5 * NW + wr] | FWp[n3 * NW + wr]);
+            }
             ++s;
+            if constexpr (NS) {
+                // every wave reaches the score limits at the same score: no event needed
+                if (s > s_bound) return TSU_EINTERNAL;
+                if (s > (DBL ? kMlMaxScoreDbl : kMlMaxScore)) return TSU_EKOVERFLOW;
+            }
             if (s > s_bound) return TSU_EINTERNAL;
             if (s > (DBL ? kMlMaxScoreDbl : kMlMaxScore)) return TSU_EKOVERFLOW;
             if constexpr (BT) {
@@ -255,8 +310,13 @@ __global__ __launch_bounds__(NW * 64) void wfa_regml_kernel(KArgs a) {
                     const int32_t der = dpp_shl1(fdr, Dr[c]);
                     uint32_t rb = 0;  // backtrace record bits of the gap sources
                     if constexpr (BT) rb = (iel >= mol ? 4u : 0u) | (der >= mor ? 8u : 0u);
-                    const int32_t I = max(mol, iel) + U;
-                    const int32_t D = max(mor, der);
+                    int32_t I = max(mol, iel) + U;
+                    int32_t D = max(mor, der);
+                    if constexpr (NS) {
+                        // out-of-bounds I / D cells are NULL where computed
+                        I = (uint32_t)I < (uint32_t)mt1 ? I : NUL;
+                        D = (uint32_t)D < (uint32_t)mt1 ? D : NUL;
+                    }
                     const int32_t X = mx + U;
                     int32_t M = max(max(D, X), I);
                     const bool valid = (uint32_t)M < (uint32_t)mt1;
@@ -283,10 +343,20 @@ __global__ __launch_bounds__(NW * 64) void wfa_regml_kernel(KArgs a) {
                     if (c == 0 && w == 0) edge |= (uint32_t)(b & 1ull);
                     if (c == C - 1 && w == NW - 1) edge |= (uint32_t)(b >> 63);
                     if (em & (1u << c)) mend = __builtin_amdgcn_readlane(M, ltp);
-                    oob |= vote(max(I, D) >= mt1);
+                    if constexpr (!NS) oob |= vote(max(I, D) >= mt1);
                 }
             }
             uint32_t Fs = acc >> 1;
+            if constexpr (NS) {
+                if (mend >= U * t) atomicMin(X + L::EV, (s << 2) | kEvEnd);  // uniform conditions and values
+                if (edge) atomicMin(X + L::EV, (s << 2) | kEvEdge);
+                if (lane == 0) FWp[l5 * NW + w] = Fs;
+                // this step's ring entries, edges and mask before the progress word
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+                if (lane == 0) __hip_atomic_store(X + L::PG + w, s, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                F5 = F4; F4 = F3; F3 = F2; F2 = F1; F1 = Fs;
+                return 0;
+            }
             const uint32_t flags = (oob ? kFlOob : 0u) | (mend >= U * t ? kFlEnd : 0u) | (edge ? kFlEdge : 0u);
             if (lane == 0) {
                 if (flags) atomicOr(reinterpret_cast<uint32_t*>(X + L::FL + fidx), flags);
@@ -379,7 +449,16 @@ __global__ __launch_bounds__(NW * 64) void wfa_regml_kernel(KArgs a) {
             r = step(std::integral_constant<int, 1>{});
             if (r == 0) r = step(std::integral_constant<int, 0>{});
         }
-        const int score = (r > 0 && s == 0) ? 0 : r;
+        int score = (r > 0 && s == 0) ? 0 : r;
+        if constexpr (NS) {
+            __syncthreads();  // every wave has stopped: the first event is final
+            const int ev = __builtin_amdgcn_readfirstlane(X[L::EV]);
+            const int kind = ev & 3;
+            if (p == t && ext0 == t) score = 0;
+            else if ((ev >> 2) <= s) score = kind == kEvEnd ? (ev >> 2) : kind == kEvInternal ? TSU_EINTERNAL : TSU_EKOVERFLOW;
+            // else: no event at or below the score this wave reached, so it
+            // stopped on a score limit (r holds that status)
+        }
         if constexpr (BT) {
             __syncthreads();  // every wave's records written
             if (tid == 0) {

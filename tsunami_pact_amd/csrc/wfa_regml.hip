@@ -19,6 +19,14 @@ static KernelFn pick_regml_u(const Shape& sh) {
         default: return nullptr;
         }
     }
+    if (sh.ns) {
+        switch (sh.nw) {
+        case 4: return sh.queue ? wfa_regml_kernel<4, 16, true, false, DBL, true> : wfa_regml_kernel<4, 16, false, false, DBL, true>;
+        case 8: return sh.queue ? wfa_regml_kernel<8, 16, true, false, DBL, true> : wfa_regml_kernel<8, 16, false, false, DBL, true>;
+        case 16: return sh.queue ? wfa_regml_kernel<16, 16, true, false, DBL, true> : wfa_regml_kernel<16, 16, false, false, DBL, true>;
+        default: return nullptr;
+        }
+    }
     switch (sh.nw) {
     case 4: return sh.queue ? wfa_regml_kernel<4, 16, true, false, DBL> : wfa_regml_kernel<4, 16, false, false, DBL>;
     case 8: return sh.queue ? wfa_regml_kernel<8, 16, true, false, DBL> : wfa_regml_kernel<8, 16, false, false, DBL>;
