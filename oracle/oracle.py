@@ -48,6 +48,9 @@ def lib() -> ctypes.CDLL:
                               ctypes.c_char_p, ctypes.c_int, ctypes.POINTER(ctypes.c_int),
                               ctypes.POINTER(Counters)]
         P = ctypes.POINTER
+        l.orc_wfa_nulled.restype = ctypes.c_int
+        l.orc_wfa_nulled.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_char_p, ctypes.c_int,
+                                     ctypes.c_int, ctypes.c_int, ctypes.c_int]
         l.orc_wfa_batch.restype = ctypes.c_int
         l.orc_wfa_batch.argtypes = [ctypes.c_char_p, P(ctypes.c_int64), P(ctypes.c_int32), P(ctypes.c_int64),
                                     P(ctypes.c_int32), ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
@@ -69,6 +72,15 @@ def lib() -> ctypes.CDLL:
 
 def gotoh(p: str, t: str, x: int = 4, o: int = 6, e: int = 2) -> int:
     return int(lib().orc_gotoh(p.encode(), len(p), t.encode(), len(t), x, o, e))
+
+
+def wfa_nulled(p: str, t: str, x: int = 4, o: int = 6, e: int = 2) -> int:
+    """Score with every out-of-bounds cell NULLed where it is computed and no
+    trim (the neighbour-synchronised GPU kernel's semantics)."""
+    r = lib().orc_wfa_nulled(p.encode(), len(p), t.encode(), len(t), x, o, e)
+    if r < 0:
+        raise RuntimeError(f"orc_wfa_nulled failed: {r}")
+    return r
 
 
 def wfa(p: str, t: str, x: int = 4, o: int = 6, e: int = 2, cigar: bool = False) -> Tuple[int, Optional[str], dict]:

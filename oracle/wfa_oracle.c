@@ -12,6 +12,11 @@
  *   orc_wfa()        exact gap-affine WFA with WFA2-lib semantics, unscaled
  *                    penalties (odd-score null steps included, exactly as the
  *                    reference processes them), optional CIGAR backtrace.
+ *   orc_wfa_nulled() the same WFA with every out-of-bounds I, D and M cell
+ *                    NULL where it is computed and no trim: the semantics of
+ *                    the neighbour-synchronised GPU kernel, which has no
+ *                    whole-window view per score.  Tests pin its score to
+ *                    orc_wfa's (and so to Gotoh's).
  *   orc_wfa_batch()  OpenMP driver over many pairs (CPU baseline; mirrors the
  *                    reference's check loop, wfa_gpu.cu:150-166, but reuses one
  *                    scratch state per thread).
@@ -306,7 +311,7 @@ static int orc_backtrace(const orc_state* st, const char* p, int plen, const cha
 
 static int orc_wfa_impl(orc_state* st, const char* p, int plen, const char* t, int tlen,
                         int x, int o, int e, int want_cigar, char* ops, int ops_cap, int* ops_len,
-                        orc_counters* cnt)
+                        orc_counters* cnt, int null_oob)
 {
     if (plen < 0 || tlen < 0 || x <= 0 || o < 0 || e <= 0) return ORC_EINVAL;
     const int ring = ORC_MAX(x, o + e) + 1;          /* wfa_gpu.cu:83-84 */
@@ -379,6 +384,10 @@ static int orc_wfa_impl(orc_state* st, const char* p, int plen, const char* t, i
             int32_t ins = ORC_MAX(wf_get(Mo, k - 1), wf_get(Ie, k - 1)) + 1;
             int32_t del = ORC_MAX(wf_get(Mo, k + 1), wf_get(De, k + 1));
             int32_t mis = wf_get(Mx, k) + 1;
+            if (null_oob) {
+                if ((uint32_t)ins > (uint32_t)tlen || (uint32_t)(ins - k) > (uint32_t)plen) ins = ORC_NULL;
+                if ((uint32_t)del > (uint32_t)tlen || (uint32_t)(del - k) > (uint32_t)plen) del = ORC_NULL;
+            }
             int32_t m = ORC_MAX(del, ORC_MAX(mis, ins));
             if ((uint32_t)m > (uint32_t)tlen || (uint32_t)(m - k) > (uint32_t)plen) m = ORC_NULL;
             if (hasI) oI->off[k - lo] = ins;
@@ -386,9 +395,11 @@ static int orc_wfa_impl(orc_state* st, const char* p, int plen, const char* t, i
             oM->off[k - lo] = m;
         }
         /* TRIM ENDS (wfa_gpu.cu:633-719) */
-        wf_trim(oM, plen, tlen);
-        if (hasI) wf_trim(oI, plen, tlen);
-        if (hasD) wf_trim(oD, plen, tlen);
+        if (!null_oob) {
+            wf_trim(oM, plen, tlen);
+            if (hasI) wf_trim(oI, plen, tlen);
+            if (hasD) wf_trim(oD, plen, tlen);
+        }
     }
     if (want_cigar) {
         int n = orc_backtrace(st, p, plen, t, tlen, x, o, e, score, ops, ops_cap);
@@ -407,7 +418,19 @@ int orc_wfa(const char* p, int plen, const char* t, int tlen, int x, int o, int 
             int want_cigar, char* ops, int ops_cap, int* ops_len, orc_counters* cnt)
 {
     orc_state st = { NULL, NULL, NULL, 0 };
-    int r = orc_wfa_impl(&st, p, plen, t, tlen, x, o, e, want_cigar, ops, ops_cap, ops_len, cnt);
+    int r = orc_wfa_impl(&st, p, plen, t, tlen, x, o, e, want_cigar, ops, ops_cap, ops_len, cnt, 0);
+    state_destroy(&st);
+    return r;
+}
+
+/*
+ * Score of one pair with out-of-bounds cells NULLed where computed instead
+ * of trimmed (the neighbour-synchronised GPU kernel's semantics).
+ */
+int orc_wfa_nulled(const char* p, int plen, const char* t, int tlen, int x, int o, int e)
+{
+    orc_state st = { NULL, NULL, NULL, 0 };
+    int r = orc_wfa_impl(&st, p, plen, t, tlen, x, o, e, 0, NULL, 0, NULL, NULL, 1);
     state_destroy(&st);
     return r;
 }
@@ -442,7 +465,7 @@ int orc_wfa_batch(const char* seqs, const int64_t* p_off, const int32_t* p_len,
             int olen = 0;
             int r = orc_wfa_impl(&st, seqs + p_off[i], p_len[i], seqs + t_off[i], t_len[i], x, o, e,
                                  want_cigar, want_cigar ? ops + ops_off[i] : NULL,
-                                 p_len[i] + t_len[i], &olen, total ? &c : NULL);
+                                 p_len[i] + t_len[i], &olen, total ? &c : NULL, 0);
             scores[i] = r;
             if (want_cigar && ops_len) ops_len[i] = olen;
             if (r < 0) failed++;

@@ -12,6 +12,7 @@ forms, the reduction to one piece, WFA == 2-piece Gotoh):
   - the committed golden fixtures.
 """
 import functools
+import itertools
 import json
 import os
 import random
@@ -231,3 +232,43 @@ def test_2p_wfa_equals_gotoh_on_longer_pairs(oracle, tsu):
         got, _ = oracle.wfa2p_batch(pairs, *pen)
         want = [oracle.gotoh2p(p, t, *pen) for p, t in pairs[:40]]
         assert list(got[:40]) == want, pen
+
+
+@pytest.mark.parametrize("pen", [(2, 3, 1), (4, 6, 2), (1, 1, 1), (3, 5, 1)])
+def test_nulled_out_of_bounds_equals_trimmed(oracle, pen):
+    """The neighbour-synchronised GPU kernel NULLs every out-of-bounds I, D
+    and M cell where it is computed instead of trimming the wavefront ends
+    (it has no whole-window view per score).  Its score equals the trimmed
+    WFA's on every pair of up to 3 bases, on ragged random pairs (unequal
+    lengths push cells out of bounds at both ends) and, through the trimmed
+    WFA, equals Gotoh's."""
+    alphabet = "ACGT"
+    seqs = [""]
+    for n in range(1, 4):
+        seqs += ["".join(c) for c in itertools.product(alphabet, repeat=n)]
+    for p in seqs:
+        for t in seqs:
+            assert oracle.wfa_nulled(p, t, *pen) == oracle.wfa(p, t, *pen)[0], (p, t)
+    rng = random.Random(0x4E55 + sum(pen))
+    for _ in range(1500):
+        p = rand_seq(rng, rng.randint(1, 150))
+        mode = rng.randrange(3)
+        if mode == 0:
+            t = rand_seq(rng, rng.randint(1, 150))
+        else:
+            t = list(p)
+            for _ in range(rng.randint(0, len(p) // 3)):
+                i = rng.randrange(len(t) + 1)
+                r = rng.randrange(3)
+                if r == 0 and i < len(t):
+                    t[i] = rng.choice([c for c in alphabet if c != t[i]])
+                elif r == 1:
+                    t.insert(i, rng.choice(alphabet))
+                elif i < len(t):
+                    del t[i]
+            t = "".join(t) + (rand_seq(rng, rng.randint(0, 40)) if mode == 2 else "")
+        want = oracle.wfa(p, t, *pen)[0]
+        assert oracle.wfa_nulled(p, t, *pen) == want, (p, t)
+    for _ in range(100):
+        p, t = rand_seq(rng, rng.randint(1, 60)), rand_seq(rng, rng.randint(1, 60))
+        assert oracle.wfa_nulled(p, t, *pen) == oracle.gotoh(p, t, *pen)
