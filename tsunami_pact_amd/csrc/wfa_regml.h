@@ -101,9 +101,10 @@ constexpr uint32_t kMlEdgeBit = 1u << 16;
 // lowest recorded score, so an event at a lower score in a slower wave is
 // never missed, and the pair's result is that minimum.  The exact trim is
 // replaced by NULLing every out-of-bounds I or D cell where it is computed;
-// the score is unchanged (checked against the reference semantics on every
-// pair of up to 4 bases and on random pairs), but cells differ, so the
-// backtrace variant keeps the barrier.  Spins are bounded (kMlSpinCap
+// the score is unchanged (the oracle's orc_wfa_nulled restates this variant
+// and its tests pin it to the trimmed WFA on every pair of up to 3 bases and
+// on ragged random pairs), but cells differ, so the backtrace variant keeps
+// the barrier.  Spins are bounded (kMlSpinCap
 // sleeps); past it the pair fails with TSU_EINTERNAL instead of hanging.
 constexpr int kEvEnd = 0, kEvEdge = 1, kEvMax = 2, kEvInternal = 3;
 constexpr int kMlSpinCap = 1 << 20;
