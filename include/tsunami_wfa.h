@@ -232,6 +232,15 @@ void tsu_cigars_free(tsu_cigars* c);
 #define TSU_CIGAR_RECORDS      1
 #define TSU_CIGAR_CHECKPOINTS  2
 int tsu_set_cigar_memory(tsu_ctx* ctx, int32_t mode, int32_t checkpoint_every, int64_t budget_bytes);
+/* Waves of one pair that synchronise with their neighbours (score-only
+ * launches of 2,049-16,384 diagonals) poll a neighbour's progress word at
+ * most this many times per score before the pair fails with TSU_EINTERNAL
+ * instead of waiting for ever; 0 restores the default (2^20 polls, far above
+ * any legitimate wait).  No reference counterpart (the reference's kernel
+ * has one warp per pair); for callers that want a tighter bound and for
+ * tests of that failure path. */
+int tsu_set_poll_limit(tsu_ctx* ctx, int32_t polls);
+
 /* What the call that made c used: the largest backtrace state of one launch
  * (device bytes) and the launches of each kind.  Any pointer may be NULL. */
 int tsu_cigars_stats(const tsu_cigars* c, int64_t* state_bytes, int32_t* record_launches,

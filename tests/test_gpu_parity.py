@@ -402,3 +402,31 @@ def test_wide_cigars_on_register_kernel(aligner, tsu, oracle):
         assert cig[i] == oracle.rle(ops[i]), i
         assert oracle.rescore(oracle.expand(cig[i]), p, t, 4, 6, 2) == got[i]
 
+
+
+def test_neighbour_sync_poll_limit_fails_cleanly(tsu, oracle):
+    """Score-only launches of 2,049-16,384 diagonals synchronise each wave
+    with its two neighbours through bounded polls.  With the bound forced
+    down to one poll, waves that would have to wait give up: every pair still
+    ends, with its exact score or TSU_EINTERNAL, and the default bound gives
+    exact scores again on the same context."""
+    ds = tsu.Dataset.generate(6, seed=SEED + 33, len_min=9000, len_max=14000, err_min=0.10, err_max=0.14)
+    pairs = ds.pairs()
+    want, _, _ = oracle.wfa_batch(pairs, 4, 6, 2)
+    a = tsu.Aligner(4, 6, 2)
+    try:
+        b = a.batch(pairs)
+        a.set_poll_limit(1)
+        say("poll limit 1, W=16384")
+        got, _ = a.align_plain(b, 16384, raise_on_error=False)
+        ok = got >= 0
+        assert (got[ok] == want[ok]).all()
+        assert (got[~ok] == tsu.TSU_EINTERNAL).all()
+        a.set_poll_limit(0)
+        say("default poll limit, W=16384")
+        got, _ = a.align_plain(b, 16384)
+        assert (got == want).all()
+        with pytest.raises(tsu.TsuError):
+            a.set_poll_limit(-1)
+    finally:
+        a.close()

@@ -45,7 +45,7 @@ EXPORTS = (
     "tsu_batch_from_dataset", "tsu_batch_create", "tsu_batch_size", "tsu_batch_bytes", "tsu_batch_free",
     "tsu_align_plain", "tsu_align_fixed", "tsu_align_batched", "tsu_autotune", "tsu_tune_table",
     "tsu_align_cigar", "tsu_cigars_get", "tsu_cigars_string", "tsu_cigars_free",
-    "tsu_set_cigar_memory", "tsu_cigars_stats",
+    "tsu_set_cigar_memory", "tsu_cigars_stats", "tsu_set_poll_limit",
     "tsu_serve_open", "tsu_serve_submit", "tsu_serve_submit_packed", "tsu_serve_poll", "tsu_serve_wait",
     "tsu_serve_close",
     "tsu_calibrate_copy", "tsu_sync",
@@ -146,6 +146,7 @@ def library() -> ctypes.CDLL:
         "tsu_cigars_free": (None, [vp]),
         "tsu_set_cigar_memory": (ctypes.c_int, [vp, i32, i32, i64]),
         "tsu_cigars_stats": (ctypes.c_int, [vp, P(i64), P(i32), P(i32)]),
+        "tsu_set_poll_limit": (ctypes.c_int, [vp, i32]),
         "tsu_serve_open": (ctypes.c_int, [vp, P(ServeCfg), P(vp)]),
         "tsu_serve_submit": (ctypes.c_int, [vp, P(ctypes.c_char_p), P(i32), P(ctypes.c_char_p), P(i32), i32, P(ctypes.c_uint64)]),
         "tsu_serve_submit_packed": (ctypes.c_int, [vp, P(ctypes.c_uint32), i64, P(Pair), i32, P(ctypes.c_uint64)]),
@@ -412,6 +413,12 @@ class Aligner:
         for backtrace state, 0 = half the free memory."""
         self._check(self._lib.tsu_set_cigar_memory(self._h, mode, checkpoint_every, budget_bytes),
                     "tsu_set_cigar_memory")
+
+    def set_poll_limit(self, polls: int = 0) -> None:
+        """Polls of a neighbour's progress before a pair fails with
+        TSU_EINTERNAL (score-only launches of 2,049-16,384 diagonals); 0 =
+        the default."""
+        self._check(self._lib.tsu_set_poll_limit(self._h, polls), "tsu_set_poll_limit")
 
     def align_cigar(self, batch: Batch, strings: bool = True):
         """(scores, CIGAR strings, timing).  With strings=False the CIGARs are

@@ -265,6 +265,7 @@ KArgs tsu_host::base_args(const tsu_ctx* ctx) {
     a.x = ctx->xs; a.o = ctx->os; a.e = ctx->es; a.g = ctx->g;
     a.rm = ctx->rm; a.ri = ctx->ri;
     a.o2 = ctx->o2s; a.e2 = ctx->e2s; a.r2 = ctx->r2;
+    a.poll_limit = ctx->poll_limit;
     return a;
 }
 
@@ -1065,6 +1066,13 @@ extern "C" int tsu_set_cigar_memory(tsu_ctx* ctx, int32_t mode, int32_t checkpoi
     ctx->cigar_mode = mode;
     ctx->cigar_ck_every = checkpoint_every;
     ctx->cigar_budget = budget_bytes;
+    return TSU_OK;
+}
+
+extern "C" int tsu_set_poll_limit(tsu_ctx* ctx, int32_t polls) {
+    if (!ctx) return TSU_EINVAL;
+    if (polls < 0) return fail(ctx, TSU_EINVAL, "tsu_set_poll_limit: negative limit");
+    ctx->poll_limit = polls;
     return TSU_OK;
 }
 

@@ -35,6 +35,7 @@ struct tsu_ctx {
     int32_t cigar_mode = TSU_CIGAR_AUTO;
     int32_t cigar_ck_every = 0;         // 0 = chosen per launch
     int64_t cigar_budget = 0;           // 0 = half the free device memory
+    int32_t poll_limit = 0;             // neighbour-sync polls before a pair fails (0 = default)
     // scratch reused across calls
     int32_t* d_queue = nullptr;
     void* h_stage = nullptr;            // pinned staging for large device-to-host copies

@@ -67,6 +67,8 @@ struct KArgs {
     int32_t ck_head;
     int32_t ck_every;
     int32_t ck_cap;
+    // neighbour-synchronised kernels: polls before a pair fails (0 = default)
+    int32_t poll_limit;
 };
 
 struct Shape {

@@ -104,8 +104,8 @@ constexpr uint32_t kMlEdgeBit = 1u << 16;
 // the score is unchanged (the oracle's orc_wfa_nulled restates this variant
 // and its tests pin it to the trimmed WFA on every pair of up to 3 bases and
 // on ragged random pairs), but cells differ, so the backtrace variant keeps
-// the barrier.  Spins are bounded (kMlSpinCap
-// sleeps); past it the pair fails with TSU_EINTERNAL instead of hanging.
+// the barrier.  Spins are bounded (kMlSpinCap sleeps, or KArgs::poll_limit);
+// past it the pair fails with TSU_EINTERNAL instead of hanging.
 constexpr int kEvEnd = 0, kEvEdge = 1, kEvMax = 2, kEvInternal = 3;
 constexpr int kMlSpinCap = 1 << 20;
 
@@ -137,6 +137,7 @@ __global__ __launch_bounds__(NW * 64) void wfa_regml_kernel(KArgs a) {
     // per computed chunk every lane stores once: lane 63 its I edge, lane 0
     // its D edge, the others into a sink (cheaper than masking the store)
     const int pub_off = lane == 0 ? L::ED - L::EI : 0;
+    const int poll_limit = a.poll_limit > 0 ? a.poll_limit : kMlSpinCap;
     uint32_t* rec = nullptr;
     if constexpr (BT) rec = a.rec_g + (int64_t)blockIdx.x * a.rec_cap;
 
@@ -236,7 +237,7 @@ __global__ __launch_bounds__(NW * 64) void wfa_regml_kernel(KArgs a) {
                     const int g1 = __builtin_amdgcn_readfirstlane(
                         __hip_atomic_load(X + L::PG + wr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP));
                     if (g0 >= s && g1 >= s) break;
-                    if (spins >= kMlSpinCap) {
+                    if (spins >= poll_limit) {
                         atomicMin(X + L::EV, ((s + 1) << 2) | kEvInternal);  // uniform: one lane's atomic
                         return TSU_EINTERNAL;
                     }
