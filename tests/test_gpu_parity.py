@@ -430,3 +430,49 @@ def test_neighbour_sync_poll_limit_fails_cleanly(tsu, oracle):
             a.set_poll_limit(-1)
     finally:
         a.close()
+
+
+def test_neighbour_sync_ragged_random_pairs(aligner, tsu, oracle):
+    """The neighbour-synchronised 16-chunk launches on 120 ragged pairs: a
+    seeded pattern of 1-6 kbp, a text that is an edited copy (0-20%) with a
+    random head or tail cut or added, or an unrelated sequence, so cells go
+    out of bounds at both ends at many scores.  Plain windows of 3000, 6000
+    and 16384 diagonals (4, 8 and 16 waves) and the batched form give the
+    oracle's scores wherever no overflow is reported."""
+    rng = np.random.default_rng(35)
+    alphabet = np.array(list("ACGT"))
+    pairs = []
+    for i in range(120):
+        n = int(rng.integers(1000, 6000))
+        p = alphabet[rng.integers(0, 4, n)]
+        kind = i % 4
+        if kind == 3:
+            t = alphabet[rng.integers(0, 4, int(rng.integers(500, 6000)))]
+        else:
+            t = list(p)
+            for _ in range(int(rng.random() * 0.2 * n)):
+                j = int(rng.integers(0, len(t)))
+                r = int(rng.integers(0, 3))
+                if r == 0:
+                    t[j] = alphabet[(int(np.nonzero(alphabet == t[j])[0][0]) + 1 + int(rng.integers(0, 3))) % 4]
+                elif r == 1:
+                    t.insert(j, alphabet[int(rng.integers(0, 4))])
+                else:
+                    del t[j]
+            cut = int(rng.integers(0, n // 3))
+            if kind == 1:
+                t = t[cut:]
+            elif kind == 2:
+                t = t + list(alphabet[rng.integers(0, 4, cut)])
+        pairs.append(("".join(p), "".join(t)))
+    want, _, _ = oracle.wfa_batch(pairs, 4, 6, 2)
+    b = aligner.batch(pairs)
+    for W in (3000, 6000, 16384):
+        say(f"ragged W={W}")
+        got, _ = aligner.align_plain(b, W, raise_on_error=False)
+        ok = got != tsu.TSU_EKOVERFLOW
+        assert ok.sum() > 0
+        assert (got[ok] == want[ok]).all(), (W, np.nonzero(got[ok] != want[ok])[0][:5])
+    say("ragged batched")
+    got, _ = aligner.align_batched(b)
+    assert (got == want).all()
