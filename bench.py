@@ -287,7 +287,7 @@ def main():
     ap.add_argument("--workload", default="cfg2", choices=sorted(WORKLOADS))
     ap.add_argument("--pairs", type=int, default=0, help="override the pair count (testing only)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--no-autotune", action="store_true", help="batched: use the default shapes")
+    ap.add_argument("--no-autotune", action="store_true", help="batched, cigar: use the default shapes")
     ap.add_argument("--tune-sample", type=int, default=4096, help="batched: autotuner sample per bucket")
     ap.add_argument("--compare-plain", action="store_true", help="batched, served: also time the plain form")
     ap.add_argument("--cigar-memory", default="auto", choices=["auto", "records", "checkpoints"],
@@ -351,7 +351,8 @@ def main():
         mode = {"auto": a.CIGAR_AUTO, "records": a.CIGAR_RECORDS, "checkpoints": a.CIGAR_CHECKPOINTS}[args.cigar_memory]
         a.set_cigar_memory(mode, args.checkpoint_every)
     tune = None
-    if args.form == "batched" and not args.no_autotune:
+    # the CIGAR form's score pass is the batched form: it uses the same table
+    if args.form in ("batched", "cigar") and not args.no_autotune:
         t0 = time.perf_counter()
         tune = a.autotune(b, args.tune_sample)
         log(f"autotune ({time.perf_counter() - t0:.1f} s): {tune}")
