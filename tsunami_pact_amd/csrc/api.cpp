@@ -965,7 +965,15 @@ extern "C" int tsu_align_cigar(tsu_ctx* ctx, tsu_batch* b, int32_t* scores, tsu_
         sh.fixed_242 = tsu_host::fixed_242(ctx);
         int64_t recs = 0;
         int32_t step_cap = steps;
-        if (int nwl = tsu_host::ml_waves(ctx, W, max_len)) {
+        const int c1 = tsu_host::reg_chunks(ctx, W);
+        if (c1 > 0 && max_len <= 16000 && s0 <= 16000) {
+            // single-wave kernel with the int16 LDS ring: records at a fixed
+            // place per step and chunk, (s * chunks + c) * 64 + lane
+            sh.nw = 1;
+            sh.reg_c = c1;
+            sh.lring = true;
+            recs = (int64_t)steps * c1 * 64;
+        } else if (int nwl = tsu_host::ml_waves(ctx, W, max_len)) {
             // 16-chunk multi-wave kernel: records at a fixed place per step and
             // chunk, (s * chunks + j) * 64 + lane
             sh.nw = nwl;

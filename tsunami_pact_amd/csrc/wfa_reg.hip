@@ -7,7 +7,17 @@
 namespace tsu {
 
 KernelFn pick_reg(const Shape& sh) {
-    if (!sh.fixed_242 || sh.bt) return nullptr;
+    if (!sh.fixed_242) return nullptr;
+    if (sh.bt) {  // CIGAR launches: the int16-ring variant with records, queue-fed
+        if (!sh.lring || !sh.queue) return nullptr;
+        switch (sh.reg_c) {
+        case 1: return wfa_reg1l_kernel<1, true, true>;
+        case 2: return wfa_reg1l_kernel<2, true, true>;
+        case 4: return wfa_reg1l_kernel<4, true, true>;
+        case 8: return wfa_reg1l_kernel<8, true, true>;
+        default: return nullptr;
+        }
+    }
     if (sh.lring) {
         switch (sh.reg_c) {
         case 1: return sh.queue ? wfa_reg1l_kernel<1, true> : wfa_reg1l_kernel<1, false>;

@@ -19,6 +19,12 @@
 // and a literal NULL costs more than clamping M on its way to the ring); a
 // doubled value fed only by the ring's NULL grows by two per score, so the
 // kernel gives up past kMl1MaxScore scaled scores with TSU_EKOVERFLOW.
+//
+// With BT every computed chunk also writes its 64 cell records (wfa_bt.h)
+// at ((s * C + c) * 64 + lane) of the workgroup's region, one coalesced
+// 256-byte store per chunk and step, and lane 0 walks them into CIGAR runs
+// once the end cell is reached (the same records and walk as the 16-chunk
+// kernel's, so the CIGARs are the same).
 #pragma once
 
 #include "wfa_regml.h"
@@ -28,7 +34,7 @@ namespace tsu {
 // -32768 + 2 * kMl1MaxScore stays negative
 constexpr int kMl1MaxScore = 16000;
 
-template <int C, bool QUEUE>
+template <int C, bool QUEUE, bool BT = false>
 __global__ __launch_bounds__(64) void wfa_reg1l_kernel(KArgs a) {
     static_assert(C >= 1 && C <= 8, "window of 64..512 diagonals (8-bit chunk fields)");
     constexpr int32_t NUL = kRegNull;
@@ -42,6 +48,8 @@ __global__ __launch_bounds__(64) void wfa_reg1l_kernel(KArgs a) {
     int32_t* sMisc = reinterpret_cast<int32_t*>(sS + 2 * a.seq_words);
     int16_t* MR = reinterpret_cast<int16_t*>(sMisc + 4);
     const bool lds_at_zero = (uint32_t)(uintptr_t)sS == 0u;
+    uint32_t* rec = nullptr;
+    if constexpr (BT) rec = a.rec_g + (int64_t)blockIdx.x * a.rec_cap;
 
     for (int iter = 0;; ++iter) {
         int idx;
@@ -113,6 +121,9 @@ __global__ __launch_bounds__(64) void wfa_reg1l_kernel(KArgs a) {
             ++s;
             if (s > s_bound) return TSU_EINTERNAL;
             if (s > kMl1MaxScore) return TSU_EKOVERFLOW;
+            if constexpr (BT) {
+                if (s >= a.step_cap) return TSU_ECIGAR;
+            }
             l3 = l3 == 2 ? 0 : l3 + 1;
             const int rd = l3 == 0 ? 2 : l3 - 1;   // ring slot of s-4
             const int wrs = l3 == 2 ? 0 : l3 + 1;  // ring slot of s-2 (formerly s-5)
@@ -127,6 +138,8 @@ __global__ __launch_bounds__(64) void wfa_reg1l_kernel(KArgs a) {
             const uint32_t act = ((src | F2 | F5) | ((src >> 16) << 1) | (((src >> 8) & 0xffu) >> 1)) & ALL;
             const int16_t* mrd = MR + rd * MRS + lane;       // entry of k-1
             int16_t* mwr = MR + wrs * MRS + lane + 1;        // entry of k
+            uint32_t* recs = nullptr;                        // records of this step
+            if constexpr (BT) recs = rec + (int64_t)s * C * 64;
             uint32_t acc = 0;
             uint64_t oob = 0;
             int32_t mend = NUL;
@@ -145,8 +158,16 @@ __global__ __launch_bounds__(64) void wfa_reg1l_kernel(KArgs a) {
                     carryI = __builtin_amdgcn_readlane(Ir[c], 63);
                     const int32_t I = max(mol, iel) + 2;
                     const int32_t D = max(mor, der);
-                    int32_t M = max(max(D, mx + 2), I);
+                    const int32_t X = mx + 2;
+                    int32_t M = max(max(D, X), I);
                     const bool valid = (uint32_t)M < (uint32_t)mt2;
+                    if constexpr (BT) {
+                        // origin of M (ties M > D > I), gap extension bits, M before extension
+                        const uint32_t rb = (iel >= mol ? 4u : 0u) | (der >= mor ? 8u : 0u);
+                        recs[64 * c + lane] = rb | (valid ? (((uint32_t)M >> 1) << 4) |
+                                                                ((X >= D && X >= I) ? 1u : (D >= I ? 2u : 3u))
+                                                          : 0u);
+                    }
                     M = valid ? M : NUL;
                     const int lim2 = mt2 - 2 - M;
                     if (valid && lim2 > 0) M += extend2((uint32_t)(M - k2), (uint32_t)M, lim2);
@@ -209,7 +230,29 @@ __global__ __launch_bounds__(64) void wfa_reg1l_kernel(KArgs a) {
             if (r == 0) r = step(std::integral_constant<int, 0>{});
         }
         const int score = (r > 0 && s == 0) ? 0 : r;
-        if (lane == 0) a.scores[pi] = !lds_at_zero ? TSU_EINTERNAL : score < 0 ? score : score * a.g;
+        if constexpr (BT) {
+            __syncthreads();  // every lane's records written
+            if (lane == 0) {
+                int st = !lds_at_zero ? TSU_EINTERNAL : score < 0 ? score : 0;
+                int64_t out_off = 0;
+                int32_t out_n = 0;
+                if (st == 0) {
+                    auto record = [&](int ss, int kk, uint32_t& rr) __attribute__((always_inline)) -> bool {
+                        if (ss < 0 || ss > s) return false;
+                        const int j = kk - kmin;
+                        if (j < 0 || j >= W) return false;
+                        rr = rec[((int64_t)ss * C + (j >> 6)) * 64 + (j & 63)];
+                        return true;
+                    };
+                    st = bt_walk(record, score, p, t, 2, 3, 1, a, &out_off, &out_n);
+                }
+                a.run_off[pi] = out_off;
+                a.run_n[pi] = out_n;
+                a.scores[pi] = st ? st : score * a.g;
+            }
+        } else {
+            if (lane == 0) a.scores[pi] = !lds_at_zero ? TSU_EINTERNAL : score < 0 ? score : score * a.g;
+        }
         __syncthreads();
     }
 }
