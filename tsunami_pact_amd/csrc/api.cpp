@@ -385,9 +385,10 @@ int tsu_host::reg_waves(const tsu_ctx* ctx, int32_t W) {
 }
 
 // Capacities above this use the 16-chunk waves with the int16 LDS M ring
-// rather than 8-chunk waves with the whole ring in registers (measured
-// faster on 2-20 kbp pairs: fewer waves per pair, more pairs per CU).
-constexpr int32_t kMlMinCapacity = 2048;
+// rather than 8-chunk waves with the whole ring in registers: measured
+// faster from 1,025 diagonals on (the 4-wave, neighbour-synchronised
+// instance fits 5 pairs per CU), slower at 850.
+constexpr int32_t kMlMinCapacity = 1024;
 
 int tsu_host::ml_waves(const tsu_ctx* ctx, int32_t W, int32_t max_len) {
     if (!tsu_host::fixed_242(ctx) || W <= kMlMinCapacity || W > 16 * 1024 || max_len > 32000) return 0;
