@@ -273,8 +273,9 @@ def test_register_and_lds_kernels_agree(aligner, tsu, oracle, fixtures):
 
 
 def test_multiwave_register_kernel(aligner, tsu, oracle):
-    """Capacities of 513..8192 diagonals run on the multi-wave register kernel
-    (2-16 waves, chunks dealt round-robin, edges exchanged through LDS).
+    """Capacities of 513..8192 diagonals run on the multi-wave register kernels
+    (8-chunk waves up to 1024 diagonals, 16-chunk waves beyond; chunks dealt
+    round-robin, edges exchanged through LDS).
     Unequal lengths put I/D cells out of bounds at both sequence ends (the
     trim across waves); narrow windows report overflow, wide ones never."""
     ds = tsu.Dataset.generate(300, seed=SEED + 14, len_min=200, len_max=4000, err_min=0.0, err_max=0.2)
@@ -298,7 +299,7 @@ def test_multiwave_register_kernel(aligner, tsu, oracle):
 
 
 def test_wide_windows_lds_ring_kernel(aligner, tsu, oracle):
-    """Capacities above 2048 diagonals (up to 16384) run on waves of 16
+    """Capacities above 1024 diagonals (up to 16384) run on waves of 16
     chunks whose older M scores sit in an int16 LDS ring (4, 8 or 16 waves).
     Long divergent pairs (12-20 kbp at 12-15%) need 9.5k-14.8k diagonals;
     random and very unequal pairs put I/D cells out of bounds at both ends
@@ -388,7 +389,7 @@ def test_long_read_cigars(aligner, tsu, oracle):
 
 
 def test_wide_cigars_on_register_kernel(aligner, tsu, oracle):
-    """CIGARs of pairs whose capacity lands above 2048 diagonals come from the
+    """CIGARs of pairs whose capacity lands above 1024 diagonals come from the
     16-chunk multi-wave kernel's records (8 and 16 waves here): 12-14 kbp at
     12% and 6 kbp at 15%; same CIGARs as the oracle, which re-score to the
     score."""
@@ -405,7 +406,7 @@ def test_wide_cigars_on_register_kernel(aligner, tsu, oracle):
 
 
 def test_neighbour_sync_poll_limit_fails_cleanly(tsu, oracle):
-    """Score-only launches of 2,049-16,384 diagonals synchronise each wave
+    """Score-only launches of 1,025-16,384 diagonals synchronise each wave
     with its two neighbours through bounded polls.  With the bound forced
     down to one poll, waves that would have to wait give up: every pair still
     ends, with its exact score or TSU_EINTERNAL, and the default bound gives

@@ -416,7 +416,7 @@ class Aligner:
 
     def set_poll_limit(self, polls: int = 0) -> None:
         """Polls of a neighbour's progress before a pair fails with
-        TSU_EINTERNAL (score-only launches of 2,049-16,384 diagonals); 0 =
+        TSU_EINTERNAL (score-only launches of 1,025-16,384 diagonals); 0 =
         the default."""
         self._check(self._lib.tsu_set_poll_limit(self._h, polls), "tsu_set_poll_limit")
 
