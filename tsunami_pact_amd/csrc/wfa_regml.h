@@ -249,11 +249,8 @@ __global__ __launch_bounds__(NW * 64) void wfa_regml_kernel(KArgs a) {
                 nbR = (uint32_t)__builtin_amdgcn_readfirstlane(FWp[l5 * NW + wr] | FWp[n3 * NW + wr]);
             }
             ++s;
-            if constexpr (NS) {
-                // every wave reaches the score limits at the same score: no event needed
-                if (s > s_bound) return TSU_EINTERNAL;
-                if (s > (DBL ? kMlMaxScoreDbl : kMlMaxScore)) return TSU_EKOVERFLOW;
-            }
+            // every wave reaches the score limits at the same score, so with
+            // neighbour synchronisation too they need no event
             if (s > s_bound) return TSU_EINTERNAL;
             if (s > (DBL ? kMlMaxScoreDbl : kMlMaxScore)) return TSU_EKOVERFLOW;
             if constexpr (BT) {
