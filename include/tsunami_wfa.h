@@ -232,8 +232,8 @@ void tsu_cigars_free(tsu_cigars* c);
 #define TSU_CIGAR_RECORDS      1
 #define TSU_CIGAR_CHECKPOINTS  2
 int tsu_set_cigar_memory(tsu_ctx* ctx, int32_t mode, int32_t checkpoint_every, int64_t budget_bytes);
-/* Waves of one pair that synchronise with their neighbours (score-only
- * launches of 1,025-16,384 diagonals) poll a neighbour's progress word at
+/* Waves of one pair that synchronise with their neighbours (score-only and
+ * CIGAR launches of 1,025-16,384 diagonals) poll a neighbour's progress word at
  * most this many times per score before the pair fails with TSU_EINTERNAL
  * instead of waiting for ever; 0 restores the default (2^20 polls, far above
  * any legitimate wait).  No reference counterpart (the reference's kernel

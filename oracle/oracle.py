@@ -50,7 +50,8 @@ def lib() -> ctypes.CDLL:
         P = ctypes.POINTER
         l.orc_wfa_nulled.restype = ctypes.c_int
         l.orc_wfa_nulled.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_char_p, ctypes.c_int,
-                                     ctypes.c_int, ctypes.c_int, ctypes.c_int]
+                                     ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                     ctypes.c_int, ctypes.c_char_p, ctypes.c_int, P(ctypes.c_int)]
         l.orc_wfa_batch.restype = ctypes.c_int
         l.orc_wfa_batch.argtypes = [ctypes.c_char_p, P(ctypes.c_int64), P(ctypes.c_int32), P(ctypes.c_int64),
                                     P(ctypes.c_int32), ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
@@ -74,13 +75,18 @@ def gotoh(p: str, t: str, x: int = 4, o: int = 6, e: int = 2) -> int:
     return int(lib().orc_gotoh(p.encode(), len(p), t.encode(), len(t), x, o, e))
 
 
-def wfa_nulled(p: str, t: str, x: int = 4, o: int = 6, e: int = 2) -> int:
-    """Score with every out-of-bounds cell NULLed where it is computed and no
-    trim (the neighbour-synchronised GPU kernel's semantics)."""
-    r = lib().orc_wfa_nulled(p.encode(), len(p), t.encode(), len(t), x, o, e)
+def wfa_nulled(p: str, t: str, x: int = 4, o: int = 6, e: int = 2, cigar: bool = False):
+    """Score (and, with cigar=True, (score, ops)) with every out-of-bounds
+    cell NULLed where it is computed and no trim (the neighbour-synchronised
+    GPU kernel's semantics)."""
+    cap = len(p) + len(t) + 1
+    buf = ctypes.create_string_buffer(cap) if cigar else None
+    n = ctypes.c_int(0)
+    r = lib().orc_wfa_nulled(p.encode(), len(p), t.encode(), len(t), x, o, e, 1 if cigar else 0,
+                             buf, cap if cigar else 0, ctypes.byref(n))
     if r < 0:
         raise RuntimeError(f"orc_wfa_nulled failed: {r}")
-    return r
+    return (r, buf.raw[:n.value].decode()) if cigar else r
 
 
 def wfa(p: str, t: str, x: int = 4, o: int = 6, e: int = 2, cigar: bool = False) -> Tuple[int, Optional[str], dict]:

@@ -15,8 +15,8 @@
  *   orc_wfa_nulled() the same WFA with every out-of-bounds I, D and M cell
  *                    NULL where it is computed and no trim: the semantics of
  *                    the neighbour-synchronised GPU kernel, which has no
- *                    whole-window view per score.  Tests pin its score to
- *                    orc_wfa's (and so to Gotoh's).
+ *                    whole-window view per score.  Tests pin its score and
+ *                    CIGAR to orc_wfa's (and the score to Gotoh's).
  *   orc_wfa_batch()  OpenMP driver over many pairs (CPU baseline; mirrors the
  *                    reference's check loop, wfa_gpu.cu:150-166, but reuses one
  *                    scratch state per thread).
@@ -424,13 +424,15 @@ int orc_wfa(const char* p, int plen, const char* t, int tlen, int x, int o, int 
 }
 
 /*
- * Score of one pair with out-of-bounds cells NULLed where computed instead
- * of trimmed (the neighbour-synchronised GPU kernel's semantics).
+ * One pair with out-of-bounds cells NULLed where computed instead of
+ * trimmed (the neighbour-synchronised GPU kernel's semantics); the CIGAR, if
+ * asked for, is walked over those cells.
  */
-int orc_wfa_nulled(const char* p, int plen, const char* t, int tlen, int x, int o, int e)
+int orc_wfa_nulled(const char* p, int plen, const char* t, int tlen, int x, int o, int e,
+                   int want_cigar, char* ops, int ops_cap, int* ops_len)
 {
     orc_state st = { NULL, NULL, NULL, 0 };
-    int r = orc_wfa_impl(&st, p, plen, t, tlen, x, o, e, 0, NULL, 0, NULL, NULL, 1);
+    int r = orc_wfa_impl(&st, p, plen, t, tlen, x, o, e, want_cigar, ops, ops_cap, ops_len, NULL, 1);
     state_destroy(&st);
     return r;
 }

@@ -303,8 +303,8 @@ def test_wide_windows_lds_ring_kernel(aligner, tsu, oracle):
     chunks whose older M scores sit in an int16 LDS ring (4, 8 or 16 waves).
     Long divergent pairs (12-20 kbp at 12-15%) need 9.5k-14.8k diagonals;
     random and very unequal pairs put I/D cells out of bounds at both ends
-    (score-only launches synchronise neighbour waves and NULL such cells
-    where computed; backtrace launches trim across waves).  Narrow windows report overflow; 16384 covers
+    (the waves synchronise with their neighbours only and NULL such cells
+    where computed, in score-only and backtrace launches).  Narrow windows report overflow; 16384 covers
     every pair here.  The batched form (initial capacity 16384) agrees."""
     ds = tsu.Dataset.generate(16, seed=SEED + 16, len_min=12000, len_max=20000, err_min=0.12, err_max=0.15)
     pairs = ds.pairs()

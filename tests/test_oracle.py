@@ -238,17 +238,18 @@ def test_2p_wfa_equals_gotoh_on_longer_pairs(oracle, tsu):
 def test_nulled_out_of_bounds_equals_trimmed(oracle, pen):
     """The neighbour-synchronised GPU kernel NULLs every out-of-bounds I, D
     and M cell where it is computed instead of trimming the wavefront ends
-    (it has no whole-window view per score).  Its score equals the trimmed
-    WFA's on every pair of up to 3 bases, on ragged random pairs (unequal
-    lengths push cells out of bounds at both ends) and, through the trimmed
-    WFA, equals Gotoh's."""
+    (it has no whole-window view per score).  Its score and its CIGAR (walked
+    over the NULLed cells with the same priority) equal the trimmed WFA's on
+    every pair of up to 3 bases and on ragged random pairs (unequal lengths
+    push cells out of bounds at both ends); the score equals Gotoh's."""
     alphabet = "ACGT"
     seqs = [""]
     for n in range(1, 4):
         seqs += ["".join(c) for c in itertools.product(alphabet, repeat=n)]
     for p in seqs:
         for t in seqs:
-            assert oracle.wfa_nulled(p, t, *pen) == oracle.wfa(p, t, *pen)[0], (p, t)
+            s, ops, _ = oracle.wfa(p, t, *pen, cigar=True)
+            assert oracle.wfa_nulled(p, t, *pen, cigar=True) == (s, ops), (p, t)
     rng = random.Random(0x4E55 + sum(pen))
     for _ in range(1500):
         p = rand_seq(rng, rng.randint(1, 150))
@@ -267,8 +268,8 @@ def test_nulled_out_of_bounds_equals_trimmed(oracle, pen):
                 elif i < len(t):
                     del t[i]
             t = "".join(t) + (rand_seq(rng, rng.randint(0, 40)) if mode == 2 else "")
-        want = oracle.wfa(p, t, *pen)[0]
-        assert oracle.wfa_nulled(p, t, *pen) == want, (p, t)
+        s, ops, _ = oracle.wfa(p, t, *pen, cigar=True)
+        assert oracle.wfa_nulled(p, t, *pen, cigar=True) == (s, ops), (p, t)
     for _ in range(100):
         p, t = rand_seq(rng, rng.randint(1, 60)), rand_seq(rng, rng.randint(1, 60))
         assert oracle.wfa_nulled(p, t, *pen) == oracle.gotoh(p, t, *pen)

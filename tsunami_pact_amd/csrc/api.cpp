@@ -980,6 +980,7 @@ extern "C" int tsu_align_cigar(tsu_ctx* ctx, tsu_batch* b, int32_t* scores, tsu_
             sh.nw = nwl;
             sh.reg_c = 16;
             sh.dbl = max_len <= tsu::kMlDblMaxLen && s0 <= tsu::kMlDblMaxScore;
+            sh.ns = true;
             recs = (int64_t)steps * nwl * 16 * 64;
         } else {
             for (int64_t s = 0; s <= s0; ++s)

@@ -100,11 +100,13 @@ constexpr uint32_t kMlEdgeBit = 1u << 16;
 // an internal error (3).  Every wave goes on until it has finished the
 // lowest recorded score, so an event at a lower score in a slower wave is
 // never missed, and the pair's result is that minimum.  The exact trim is
-// replaced by NULLing every out-of-bounds I or D cell where it is computed;
-// the score is unchanged (the oracle's orc_wfa_nulled restates this variant
-// and its tests pin it to the trimmed WFA on every pair of up to 3 bases and
-// on ragged random pairs), but cells differ, so the backtrace variant keeps
-// the barrier.  Spins are bounded (kMlSpinCap sleeps, or KArgs::poll_limit);
+// replaced by NULLing every out-of-bounds I or D cell where it is computed.
+// Cells can differ from the trimmed computation, but neither the score nor
+// the CIGAR walked over the records does: the oracle's orc_wfa_nulled
+// restates this variant and its tests pin both to the trimmed WFA on every
+// pair of up to 3 bases and on ragged random pairs.  With BT, the records of
+// a score past the end cell (a wave may start one more score before it sees
+// the event) are written but never walked.  Spins are bounded (kMlSpinCap sleeps, or KArgs::poll_limit);
 // past it the pair fails with TSU_EINTERNAL instead of hanging.
 constexpr int kEvEnd = 0, kEvEdge = 1, kEvMax = 2, kEvInternal = 3;
 constexpr int kMlSpinCap = 1 << 20;
@@ -113,7 +115,6 @@ typedef __attribute__((address_space(3))) const int32_t lds_ci32;
 
 template <int NW, int C, bool QUEUE, bool BT = false, bool DBL = false, bool NS = false>
 __global__ __launch_bounds__(NW * 64) void wfa_regml_kernel(KArgs a) {
-    static_assert(!(NS && BT), "the backtrace variant keeps the exact trim and the barrier");
     constexpr int U = DBL ? 2 : 1;  // offset unit: 2 when offsets are kept doubled
     static_assert(NW >= 2 && NW <= 16, "2..16 waves");
     static_assert(C >= 1 && C <= 16, "16 chunk bits per wave");

@@ -13,9 +13,9 @@ static KernelFn pick_regml_u(const Shape& sh) {
     if (sh.bt) {  // CIGAR launches are always queue-fed
         if (!sh.queue) return nullptr;
         switch (sh.nw) {
-        case 4: return wfa_regml_kernel<4, 16, true, true, DBL>;
-        case 8: return wfa_regml_kernel<8, 16, true, true, DBL>;
-        case 16: return wfa_regml_kernel<16, 16, true, true, DBL>;
+        case 4: return sh.ns ? wfa_regml_kernel<4, 16, true, true, DBL, true> : wfa_regml_kernel<4, 16, true, true, DBL>;
+        case 8: return sh.ns ? wfa_regml_kernel<8, 16, true, true, DBL, true> : wfa_regml_kernel<8, 16, true, true, DBL>;
+        case 16: return sh.ns ? wfa_regml_kernel<16, 16, true, true, DBL, true> : wfa_regml_kernel<16, 16, true, true, DBL>;
         default: return nullptr;
         }
     }
