@@ -415,7 +415,6 @@ static void use_register_kernel(const tsu_ctx* ctx, int32_t W, int32_t max_len, 
         r.reg_c = 16;
         r.nw = nwl;
         r.dbl = max_len <= tsu::kMlDblMaxLen;
-        r.ns = true;
     } else if (int nw = tsu_host::reg_waves(ctx, W)) {
         r.reg_c = 8;
         r.nw = nw;
@@ -980,7 +979,6 @@ extern "C" int tsu_align_cigar(tsu_ctx* ctx, tsu_batch* b, int32_t* scores, tsu_
             sh.nw = nwl;
             sh.reg_c = 16;
             sh.dbl = max_len <= tsu::kMlDblMaxLen && s0 <= tsu::kMlDblMaxScore;
-            sh.ns = true;
             recs = (int64_t)steps * nwl * 16 * 64;
         } else {
             for (int64_t s = 0; s <= s0; ++s)

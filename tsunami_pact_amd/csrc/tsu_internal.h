@@ -83,7 +83,6 @@ struct Shape {
     bool two_piece = false;  // gap-affine 2-piece kernel (one wave per pair, rings in LDS)
     bool ck = false;         // backtrace from ring checkpoints (bt and queue set, LDS kernel)
     bool dbl = false;        // 16-chunk kernel with doubled offsets (sequences up to kMlDblMaxLen bases)
-    bool ns = false;         // 16-chunk kernel: neighbour-synchronised waves instead of a barrier per score
 };
 
 // Doubled-offset 16-chunk kernel: longest sequence (2t fits int16) and the

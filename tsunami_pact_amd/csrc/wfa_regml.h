@@ -11,8 +11,9 @@
 // slots (the scores s-2, s-3 and s-4 of the current step), from which the
 // gap-open values M[s-4][k-1] and M[s-4][k+1] are read straight at their
 // diagonals, across chunk and wave boundaries alike.  The slot written at
-// step s (score s-2) is never the one read (score s-4), and one workgroup
-// barrier per step orders the writes before the reads two steps later.
+// step s (score s-2) is never the one read (score s-4); the waves of a pair
+// synchronise with their neighbours only (below), which orders the writes
+// before the reads two steps later.
 //
 // Global chunk j belongs to wave j mod NW, register slot j / NW, as in the
 // multi-wave register kernel (wfa_regmw.h).  I and D cross chunk edges
@@ -39,8 +40,9 @@
 // past kMlMaxScoreDbl = 16383 scaled scores (still negative: -32768 + 2s).
 // A pair whose sequences hold at most 8190 bases cannot score more.
 //
-// Recurrences, bounds test, extend, trim and end test are those of the
-// other kernels (wfa_gpu.cu:253-331,610-631,633-719).
+// Recurrences, bounds test, extend and end test are those of the other
+// kernels (wfa_gpu.cu:253-331,610-631); the trim (wfa_gpu.cu:633-719) is
+// replaced by NULLing out-of-bounds cells where they are computed (below).
 //
 // With BT every computed chunk also writes its 64 cell records (wfa_bt.h)
 // to global memory at ((s * NC + j) * 64 + lane) of the workgroup's region
@@ -67,9 +69,7 @@ struct MlLayout {
     static constexpr int EI = 0;                            // [2][NC+2] I lane 63 per chunk, by parity of s
     static constexpr int ED = EI + 2 * (NC + 2);            // [2][NC+2] D lane 0 per chunk
     static constexpr int FW = ED + 2 * (NC + 2);            // [5][NW] step masks, by s mod 5
-    static constexpr int FL = FW + 5 * NW;                  // [3] step flags, rotating
-    static constexpr int TR = FL + 3;                       // [4] trim bounds: I first, I last, D first, D last
-    static constexpr int MISC = TR + 4;                     // [4] queue index
+    static constexpr int MISC = FW + 5 * NW;                // [4] queue index
     static constexpr int PG = MISC + 4;                     // [NW] neighbour sync: last score each wave finished
     static constexpr int EV = PG + NW;                      // [4] neighbour sync: first event, score * 4 + kind
     static constexpr int SK = EV + 4;                       // [64+NC] write sink of the lanes that publish nothing
@@ -78,15 +78,13 @@ struct MlLayout {
     static constexpr int WORDS = MR + (3 * MR_STRIDE + 1) / 2;
 };
 
-// Step masks (32 bits): bit c (c < 16) is set when chunk c holds a
-// non-NULL M, I or D value; bit 16 when the window's edge lane does (lane 0
-// of chunk 0 in wave 0, lane 63 of the last chunk in the last wave).  A
-// neighbour chunk counts as a source whenever it holds any non-NULL value,
-// which computes a few chunks more at the wavefront's ends than edge-lane
-// masks would, for a third of the scalar work per chunk.
-constexpr uint32_t kMlEdgeBit = 1u << 16;
+// Step masks (one bit per chunk): bit c is set when chunk c holds a
+// non-NULL M, I or D value.  A neighbour chunk counts as a source whenever
+// it holds any non-NULL value, which computes a few chunks more at the
+// wavefront's ends than edge-lane masks would, for a third of the scalar
+// work per chunk.
 
-// Neighbour synchronisation (NS).  Wave w's step s reads, besides its own
+// Neighbour synchronisation.  Wave w's step s reads, besides its own
 // registers, only what the waves owning its neighbour chunks (w-1 and w+1,
 // cyclically) wrote at steps s-1 and s-3: their edge lanes, their ring
 // entries and their step masks.  So instead of a workgroup barrier per score,
@@ -113,7 +111,7 @@ constexpr int kMlSpinCap = 1 << 20;
 
 typedef __attribute__((address_space(3))) const int32_t lds_ci32;
 
-template <int NW, int C, bool QUEUE, bool BT = false, bool DBL = false, bool NS = false>
+template <int NW, int C, bool QUEUE, bool BT = false, bool DBL = false>
 __global__ __launch_bounds__(NW * 64) void wfa_regml_kernel(KArgs a) {
     constexpr int U = DBL ? 2 : 1;  // offset unit: 2 when offsets are kept doubled
     static_assert(NW >= 2 && NW <= 16, "2..16 waves");
@@ -165,8 +163,7 @@ __global__ __launch_bounds__(NW * 64) void wfa_regml_kernel(KArgs a) {
         }
         for (int i = tid; i < L::WORDS; i += NT) {
             int32_t v = NUL;                                          // I / D edges
-            if (i >= L::FW) v = 0;                                    // masks, flags
-            if (i >= L::TR && i < L::MISC) v = ((i - L::TR) & 1) ? -kBig : kBig;  // trim bounds
+            if (i >= L::FW) v = 0;                                    // masks
             if (i >= L::MR) v = (int32_t)0x80008000u;                 // two int16 NULLs
             if (i >= L::PG && i < L::EV) v = 0;                       // progress: score 0 done
             if (i == L::EV) v = 0x7fffffff;                           // no event yet
@@ -211,22 +208,21 @@ __global__ __launch_bounds__(NW * 64) void wfa_regml_kernel(KArgs a) {
         }
         __syncthreads();
         int s = 0;
-        int fidx = 0;  // flag word of score s+1 (s+1 mod 3)
-        int l5 = 0;    // s mod 5
-        int l3 = 0;    // s mod 3
+        int l5 = 0;  // s mod 5
+        int l3 = 0;  // s mod 3
         // neighbour waves' masks of scores s and s-3 (the sources of the
-        // next step), read together with the flag word after each barrier
+        // next step), read after each neighbour wait
         uint32_t nbL = (uint32_t)__builtin_amdgcn_readfirstlane(FWp[wl]);
         uint32_t nbR = (uint32_t)__builtin_amdgcn_readfirstlane(FWp[wr]);
 
         // One score step; PAR = s & 1 selects the register slot holding
-        // M[s-2], which receives M[s].  Returns 0 to go on, the score when
-        // the end cell is reached, or a negative status.  Every wave takes
-        // the same branches: they depend only on values read from LDS after
-        // a barrier.
+        // M[s-2], which receives M[s].  Returns 0 to go on, 1 once an event
+        // is recorded at a score this wave has finished, or a negative
+        // status (the score limits, reached by every wave at the same score,
+        // or a neighbour wait past its bound).
         auto step = [&](auto PC) __attribute__((always_inline)) -> int {
             constexpr int PAR = decltype(PC)::value;
-            if constexpr (NS) {
+            {
                 // both neighbours have finished score s; stop once an event
                 // is recorded at a score this wave has finished
                 for (int spins = 0;; ++spins) {
@@ -257,7 +253,6 @@ __global__ __launch_bounds__(NW * 64) void wfa_regml_kernel(KArgs a) {
             if constexpr (BT) {
                 if (s >= a.step_cap) return TSU_ECIGAR;
             }
-            fidx = fidx + 1 == 3 ? 0 : fidx + 1;
             l5 = l5 == 4 ? 0 : l5 + 1;
             l3 = l3 == 2 ? 0 : l3 + 1;
             const int rd = l3 == 0 ? 2 : l3 - 1;   // ring slot of s-4
@@ -292,7 +287,6 @@ __global__ __launch_bounds__(NW * 64) void wfa_regml_kernel(KArgs a) {
             int16_t* mwr = MR + wrs * L::MR_STRIDE + pb + ln + 1;         // entry of k
             uint32_t acc = 0;  // step mask, one bit up
             uint32_t edge = 0;
-            uint64_t oob = 0;
             int32_t mend = NUL;
 #pragma unroll
             for (int c = 0; c < C; ++c) {
@@ -312,11 +306,9 @@ __global__ __launch_bounds__(NW * 64) void wfa_regml_kernel(KArgs a) {
                     if constexpr (BT) rb = (iel >= mol ? 4u : 0u) | (der >= mor ? 8u : 0u);
                     int32_t I = max(mol, iel) + U;
                     int32_t D = max(mor, der);
-                    if constexpr (NS) {
-                        // out-of-bounds I / D cells are NULL where computed
-                        I = (uint32_t)I < (uint32_t)mt1 ? I : NUL;
-                        D = (uint32_t)D < (uint32_t)mt1 ? D : NUL;
-                    }
+                    // out-of-bounds I / D cells are NULL where computed
+                    I = (uint32_t)I < (uint32_t)mt1 ? I : NUL;
+                    D = (uint32_t)D < (uint32_t)mt1 ? D : NUL;
                     const int32_t X = mx + U;
                     int32_t M = max(max(D, X), I);
                     const bool valid = (uint32_t)M < (uint32_t)mt1;
@@ -343,104 +335,16 @@ __global__ __launch_bounds__(NW * 64) void wfa_regml_kernel(KArgs a) {
                     if (c == 0 && w == 0) edge |= (uint32_t)(b & 1ull);
                     if (c == C - 1 && w == NW - 1) edge |= (uint32_t)(b >> 63);
                     if (em & (1u << c)) mend = __builtin_amdgcn_readlane(M, ltp);
-                    if constexpr (!NS) oob |= vote(max(I, D) >= mt1);
                 }
             }
-            uint32_t Fs = acc >> 1;
-            if constexpr (NS) {
-                if (mend >= U * t) atomicMin(X + L::EV, (s << 2) | kEvEnd);  // uniform conditions and values
-                if (edge) atomicMin(X + L::EV, (s << 2) | kEvEdge);
-                if (lane == 0) FWp[l5 * NW + w] = Fs;
-                // this step's ring entries, edges and mask before the progress word
-                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-                if (lane == 0) __hip_atomic_store(X + L::PG + w, s, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-                F5 = F4; F4 = F3; F3 = F2; F2 = F1; F1 = Fs;
-                return 0;
-            }
-            const uint32_t flags = (oob ? kFlOob : 0u) | (mend >= U * t ? kFlEnd : 0u) | (edge ? kFlEdge : 0u);
-            if (lane == 0) {
-                if (flags) atomicOr(reinterpret_cast<uint32_t*>(X + L::FL + fidx), flags);
-                if (w == 0) X[L::FL + (fidx + 1 == 3 ? 0 : fidx + 1)] = 0;  // the next score's word
-                FWp[l5 * NW + w] = Fs | (edge ? kMlEdgeBit : 0u);
-            }
-            __syncthreads();
-            const int n4 = l5 >= 3 ? l5 - 3 : l5 + 2;  // slot of s-3
-            const uint32_t all = (uint32_t)__builtin_amdgcn_readfirstlane(X[L::FL + fidx]);
-            nbL = (uint32_t)__builtin_amdgcn_readfirstlane(FWp[l5 * NW + wl] | FWp[n4 * NW + wl]);
-            nbR = (uint32_t)__builtin_amdgcn_readfirstlane(FWp[l5 * NW + wr] | FWp[n4 * NW + wr]);
-            if (all & kFlEnd) return s;
-            if (all & kFlOob) {
-                // exact trim (wfa_gpu.cu:633-719): first / last in-bounds I and
-                // D over the whole window, cells outside them NULL
-                int b4[4] = {kBig, -kBig, kBig, -kBig};
-#pragma unroll
-                for (int c = 0; c < C; ++c) {
-                    // one chunk at a time (the opaque lane keeps the compiler
-                    // from issuing all 2C ballots first, out of scalar registers)
-                    int lc = ln;
-                    asm volatile("" : "+v"(lc));
-                    const int k = kmin + pb + 64 * NW * c + lc;
-                    const int mt1 = U * min(k + p + 1, t + 1);
-                    const int base = pb + 64 * NW * c;
-                    const uint64_t bi = vote((uint32_t)Ir[c] < (uint32_t)mt1);
-                    const uint64_t bd = vote((uint32_t)Dr[c] < (uint32_t)mt1);
-                    if (bi) {
-                        b4[0] = min(b4[0], base + (int)__builtin_ctzll(bi));
-                        b4[1] = max(b4[1], base + 63 - (int)__builtin_clzll(bi));
-                    }
-                    if (bd) {
-                        b4[2] = min(b4[2], base + (int)__builtin_ctzll(bd));
-                        b4[3] = max(b4[3], base + 63 - (int)__builtin_clzll(bd));
-                    }
-                    asm volatile("" : "+s"(b4[0]), "+s"(b4[1]), "+s"(b4[2]), "+s"(b4[3]));
-                }
-                if (lane == 0) {
-                    atomicMin(X + L::TR + 0, b4[0]);
-                    atomicMax(X + L::TR + 1, b4[1]);
-                    atomicMin(X + L::TR + 2, b4[2]);
-                    atomicMax(X + L::TR + 3, b4[3]);
-                }
-                __syncthreads();
-                const int fI = __builtin_amdgcn_readfirstlane(X[L::TR + 0]);
-                const int lI = __builtin_amdgcn_readfirstlane(X[L::TR + 1]);
-                const int fD = __builtin_amdgcn_readfirstlane(X[L::TR + 2]);
-                const int lD = __builtin_amdgcn_readfirstlane(X[L::TR + 3]);
-                uint32_t m2 = 0, e2 = 0;
-                int32_t* ep = (lane == 0 || lane == 63) ? X + L::EI + PAR * (NC + 2) + w + 1 + pub_off : X + L::SK + lane;
-#pragma unroll
-                for (int c = 0; c < C; ++c) {
-                    int lc = ln;
-                    asm volatile("" : "+v"(lc));
-                    const int pos = pb + 64 * NW * c + lc;
-                    if (pos < fI || pos > lI) Ir[c] = NUL;
-                    if (pos < fD || pos > lD) Dr[c] = NUL;
-                    ep[c * NW] = lane == 0 ? Dr[c] : Ir[c];
-                    const uint64_t b = vote(max(max(Mr[PAR][c], Ir[c]), Dr[c]) >= 0);
-                    m2 |= b ? 2u << c : 0u;
-                    asm volatile("" : "+s"(m2));
-                    if (c == 0 && w == 0) e2 |= (uint32_t)(b & 1ull);
-                    if (c == C - 1 && w == NW - 1) e2 |= (uint32_t)(b >> 63);
-                }
-                Fs = m2 >> 1;
-                if (lane == 0) FWp[l5 * NW + w] = Fs | (e2 ? kMlEdgeBit : 0u);
-                __syncthreads();
-                if (tid == 0) {
-                    X[L::TR + 0] = kBig;
-                    X[L::TR + 1] = -kBig;
-                    X[L::TR + 2] = kBig;
-                    X[L::TR + 3] = -kBig;
-                }
-                F5 = F4; F4 = F3; F3 = F2; F2 = F1; F1 = Fs;
-                nbL = (uint32_t)__builtin_amdgcn_readfirstlane(FWp[l5 * NW + wl] | FWp[n4 * NW + wl]);
-                nbR = (uint32_t)__builtin_amdgcn_readfirstlane(FWp[l5 * NW + wr] | FWp[n4 * NW + wr]);
-                // the window edges after the trim
-                const uint32_t e0 = (uint32_t)__builtin_amdgcn_readfirstlane(FWp[l5 * NW + 0]);
-                const uint32_t e1 = (uint32_t)__builtin_amdgcn_readfirstlane(FWp[l5 * NW + NW - 1]);
-                if ((e0 | e1) & kMlEdgeBit) return TSU_EKOVERFLOW;
-                return 0;
-            }
+            const uint32_t Fs = acc >> 1;
+            if (mend >= U * t) atomicMin(X + L::EV, (s << 2) | kEvEnd);  // uniform conditions and values
+            if (edge) atomicMin(X + L::EV, (s << 2) | kEvEdge);
+            if (lane == 0) FWp[l5 * NW + w] = Fs;
+            // this step's ring entries, edges and mask before the progress word
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+            if (lane == 0) __hip_atomic_store(X + L::PG + w, s, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
             F5 = F4; F4 = F3; F3 = F2; F2 = F1; F1 = Fs;
-            if (all & kFlEdge) return TSU_EKOVERFLOW;
             return 0;
         };
 
@@ -449,8 +353,8 @@ __global__ __launch_bounds__(NW * 64) void wfa_regml_kernel(KArgs a) {
             r = step(std::integral_constant<int, 1>{});
             if (r == 0) r = step(std::integral_constant<int, 0>{});
         }
-        int score = (r > 0 && s == 0) ? 0 : r;
-        if constexpr (NS) {
+        int score = r;
+        {
             __syncthreads();  // every wave has stopped: the first event is final
             const int ev = __builtin_amdgcn_readfirstlane(X[L::EV]);
             const int kind = ev & 3;

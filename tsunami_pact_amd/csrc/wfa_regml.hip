@@ -1,7 +1,8 @@
 // Instances of the multi-wave kernel with the older M scores in LDS (scaled
 // penalties 2/3/1, i.e. 4/6/2): 16 chunks of 64 diagonals per wave, 4 to 16
 // waves per pair (windows of 4096 to 16384 diagonals); offsets doubled for
-// sequences of at most 16383 bases.
+// sequences of at most 16383 bases; the waves of a pair synchronise with
+// their neighbours only.
 #include "wfa_regml.h"
 
 namespace tsu {
@@ -13,17 +14,9 @@ static KernelFn pick_regml_u(const Shape& sh) {
     if (sh.bt) {  // CIGAR launches are always queue-fed
         if (!sh.queue) return nullptr;
         switch (sh.nw) {
-        case 4: return sh.ns ? wfa_regml_kernel<4, 16, true, true, DBL, true> : wfa_regml_kernel<4, 16, true, true, DBL>;
-        case 8: return sh.ns ? wfa_regml_kernel<8, 16, true, true, DBL, true> : wfa_regml_kernel<8, 16, true, true, DBL>;
-        case 16: return sh.ns ? wfa_regml_kernel<16, 16, true, true, DBL, true> : wfa_regml_kernel<16, 16, true, true, DBL>;
-        default: return nullptr;
-        }
-    }
-    if (sh.ns) {
-        switch (sh.nw) {
-        case 4: return sh.queue ? wfa_regml_kernel<4, 16, true, false, DBL, true> : wfa_regml_kernel<4, 16, false, false, DBL, true>;
-        case 8: return sh.queue ? wfa_regml_kernel<8, 16, true, false, DBL, true> : wfa_regml_kernel<8, 16, false, false, DBL, true>;
-        case 16: return sh.queue ? wfa_regml_kernel<16, 16, true, false, DBL, true> : wfa_regml_kernel<16, 16, false, false, DBL, true>;
+        case 4: return wfa_regml_kernel<4, 16, true, true, DBL>;
+        case 8: return wfa_regml_kernel<8, 16, true, true, DBL>;
+        case 16: return wfa_regml_kernel<16, 16, true, true, DBL>;
         default: return nullptr;
         }
     }
