@@ -205,7 +205,10 @@ int tsu_autotune(tsu_ctx* ctx, tsu_batch* b, int32_t sample_pairs);
 int tsu_tune_table(const tsu_ctx* ctx, tsu_tune_entry* entries, int32_t cap, int32_t* n);
 
 /* Scores and CIGARs.  Backtrace follows WFA2-lib's piggy-backed priority
- * (common.h:67-77).  Each CIGAR is a run list, run = (length << 2) | op with
+ * (common.h:67-77).  Pairs whose length bucket runs on a register kernel are
+ * recorded in one pass, with state sized for the highest score the kernel's
+ * window holds; the others, and pairs beyond that bound, get their scores
+ * first (the batched form), which size the state exactly.  Each CIGAR is a run list, run = (length << 2) | op with
  * op 0 'M' match, 1 'X' mismatch, 2 'I' insertion (consumes text), 3 'D'
  * deletion (consumes pattern). */
 int tsu_align_cigar(tsu_ctx* ctx, tsu_batch* b, int32_t* scores, tsu_cigars** out, tsu_timing* t);
@@ -227,7 +230,10 @@ void tsu_cigars_free(tsu_cigars* c);
  *                    force one
  *   checkpoint_every K, 0 = about sqrt(score * ring slots) per launch
  *   budget_bytes     device memory one launch may hold for backtrace state,
- *                    0 = half the free device memory */
+ *                    0 = half the free device memory
+ * The backtrace state stays allocated in the context between calls (freeing
+ * tens of GB of device memory can take a second) and grows when a launch
+ * needs more; this call and tsu_destroy release it. */
 #define TSU_CIGAR_AUTO         0
 #define TSU_CIGAR_RECORDS      1
 #define TSU_CIGAR_CHECKPOINTS  2

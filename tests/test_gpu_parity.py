@@ -404,6 +404,27 @@ def test_wide_cigars_on_register_kernel(aligner, tsu, oracle):
         assert oracle.rescore(oracle.expand(cig[i]), p, t, 4, 6, 2) == got[i]
 
 
+def test_cigar_single_pass_falls_back(aligner, tsu, oracle):
+    """CIGARs are first recorded in one pass per length bucket, without a
+    score pass, with records sized for the highest score the bucket's window
+    holds.  Unrelated 2,500-3,500 bp sequences (scores far above that) among
+    3-3.5 kbp pairs at 5% overflow the window or pass that step bound and
+    take the score-first path; every CIGAR equals the oracle's and re-scores
+    to the score, and both paths ran record launches."""
+    rng = np.random.default_rng(41)
+    pairs = tsu.Dataset.generate(6, seed=SEED + 41, len_min=3000, len_max=3500, err_min=0.05).pairs()
+    for n in (2500, 3000, 3500):
+        pairs.append(("".join(rng.choice(list("ACGT"), n)), "".join(rng.choice(list("ACGT"), n))))
+    want, ops, _ = oracle.wfa_batch(pairs, 4, 6, 2, cigar=True)
+    say("cigar single pass + fallback, 9 pairs")
+    got, cig, t = aligner.align_cigar(aligner.batch(pairs))
+    assert (got == want).all(), np.nonzero(got != want)[0]
+    for i, (p, t_) in enumerate(pairs):
+        assert cig[i] == oracle.rle(ops[i]), i
+        assert oracle.rescore(oracle.expand(cig[i]), p, t_, 4, 6, 2) == got[i]
+    assert t["record_launches"] >= 2 and t["checkpoint_launches"] == 0
+
+
 
 def test_neighbour_sync_poll_limit_fails_cleanly(tsu, oracle):
     """Score-only launches of 1,025-16,384 diagonals synchronise each wave

@@ -147,11 +147,36 @@ extern "C" int tsu_create_2p(tsu_ctx** out, int device, int32_t mismatch, int32_
     return TSU_OK;
 }
 
+// Frees the backtrace state the context keeps between tsu_align_cigar calls.
+static void bt_release(tsu_ctx* ctx) {
+    for (int i = 0; i < 4; ++i) {
+        if (ctx->bt_buf[i]) (void)hipFree(ctx->bt_buf[i]);
+        ctx->bt_buf[i] = nullptr;
+        ctx->bt_bytes[i] = 0;
+    }
+}
+
+// Backtrace state buffer i with at least `bytes`, grown (never shrunk) in
+// the context.  The launches that used the old buffer have completed.
+static hipError_t bt_reserve(tsu_ctx* ctx, int i, size_t bytes, void** p) {
+    if (ctx->bt_bytes[i] < bytes) {
+        if (ctx->bt_buf[i]) (void)hipFree(ctx->bt_buf[i]);
+        ctx->bt_buf[i] = nullptr;
+        ctx->bt_bytes[i] = 0;
+        hipError_t e = hipMalloc(&ctx->bt_buf[i], bytes);
+        if (e != hipSuccess) { ctx->bt_buf[i] = nullptr; return e; }
+        ctx->bt_bytes[i] = bytes;
+    }
+    *p = ctx->bt_buf[i];
+    return hipSuccess;
+}
+
 extern "C" void tsu_destroy(tsu_ctx* ctx) {
     if (!ctx) return;
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     if (ctx->stream2) (void)hipStreamSynchronize(ctx->stream2);
+    bt_release(ctx);
     if (ctx->d_queue) (void)hipFree(ctx->d_queue);
     if (ctx->h_stage) (void)hipHostFree(ctx->h_stage);
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
@@ -895,6 +920,29 @@ static int copy_to_host(tsu_ctx* ctx, void* dst, const void* src, size_t bytes) 
     return TSU_OK;
 }
 
+// Backtrace state of one single-pass CIGAR launch at most: freeing a much
+// larger device buffer takes about a second on MI355X (137 GiB: 1.04 s,
+// 68 GiB: under 1 ms), more than the score pass the single pass saves.
+constexpr int64_t kSinglePassState = 64ll << 30;
+
+// Highest scaled score a pair of lengths (p, t) can have: min(p, t)
+// mismatches and one gap of |p - t|.
+static int64_t score_bound(const tsu_ctx* ctx, int32_t p, int32_t t) {
+    const int64_t d = std::abs((int64_t)p - t);
+    return (int64_t)ctx->xs * std::min(p, t) + (d ? ctx->os + ctx->es * d : 0);
+}
+
+// Highest scaled score whose diagonal range fits a window of W diagonals
+// (-1 if none does).
+static int64_t score_for_capacity(const tsu_ctx* ctx, int32_t W) {
+    int64_t lo = -1, hi = 1ll << 30;
+    while (hi - lo > 1) {
+        const int64_t mid = lo + (hi - lo) / 2;
+        if (capacity_for_score(ctx, mid) <= W) lo = mid; else hi = mid;
+    }
+    return lo;
+}
+
 extern "C" int tsu_align_cigar(tsu_ctx* ctx, tsu_batch* b, int32_t* scores, tsu_cigars** out, tsu_timing* t) {
     if (!ctx || !b || !scores || !out || b->ctx != ctx) return fail(ctx, TSU_EINVAL, "tsu_align_cigar: bad arguments");
     *out = nullptr;
@@ -902,145 +950,263 @@ extern "C" int tsu_align_cigar(tsu_ctx* ctx, tsu_batch* b, int32_t* scores, tsu_
     HIPCHK(hipSetDevice(ctx->device), "hipSetDevice");
     if (t) std::memset(t, 0, sizeof(*t));
     const double w0 = now_ms();
-    // 1) scores first (batched form): they size the backtrace buffers exactly
-    std::vector<int32_t> sel((size_t)b->n);
-    std::iota(sel.begin(), sel.end(), 0);
-    int r = batched_core(ctx, b, sel, scores, t, true);
-    if (r != TSU_OK) return r;
-    for (int32_t i = 0; i < b->n; ++i)
-        if (scores[i] < 0) return fail(ctx, scores[i], "pair " + std::to_string(i) + ": " + tsu_strerror(scores[i]));
     // freed on every early return; handed to the caller at the end
     std::unique_ptr<tsu_cigars> cg(new (std::nothrow) tsu_cigars());
     if (!cg) return fail(ctx, TSU_ENOMEM, "host allocation");
     cg->off.assign((size_t)b->n, 0);
     cg->n.assign((size_t)b->n, 0);
-    // 2) per pair: steps = s/g + 1, records <= sum of widths <= steps * W(s)
+    // a walk reserves 2 * (s / unit) + 2 run words for its scaled score s
     const int32_t unit = std::min(ctx->xs, ctx->os + ctx->es);
-    std::vector<int32_t> order((size_t)b->n);
-    std::iota(order.begin(), order.end(), 0);
-    tsu_host::order_desc(order, [&](int32_t i) -> int64_t { return scores[i]; });
-    int64_t runs_total = 0;
-    for (int32_t i = 0; i < b->n; ++i) runs_total += 2ll * (scores[i] / ctx->g / unit) + 2;
     DevBuf<uint32_t> d_runs;
     DevBuf<unsigned long long> d_used;
     DevBuf<int64_t> d_run_off;
     DevBuf<int32_t> d_run_n;
     DevBuf<int32_t> d_order;
-    HIPCHK(d_runs.alloc((size_t)std::max<int64_t>(runs_total, 1)), "hipMalloc (runs)");
     HIPCHK(d_used.alloc(1), "hipMalloc");
     HIPCHK(d_run_off.alloc((size_t)std::max(b->n, 1)), "hipMalloc");
     HIPCHK(d_run_n.alloc((size_t)std::max(b->n, 1)), "hipMalloc");
     HIPCHK(d_order.alloc((size_t)std::max(b->n, 1)), "hipMalloc");
     HIPCHK(hipMemsetAsync(d_used.p, 0, sizeof(unsigned long long), ctx->stream), "hipMemsetAsync");
-    // group pairs into launches of similar score so buffers fit the widest
-    size_t pos = 0;
+    int64_t runs_cap = 0;  // words in d_runs
     double kms_total = 0;
     int launches = 0;
     size_t free_b = 0, total_b = 0;
     HIPCHK(hipMemGetInfo(&free_b, &total_b), "hipMemGetInfo");
+    // the state this context already holds counts as free
+    for (int i = 0; i < 4; ++i) free_b += ctx->bt_bytes[i];
     const int64_t budget = ctx->cigar_budget > 0 ? ctx->cigar_budget : (int64_t)(0.5 * (double)free_b);
-    // backtrace state, grown as groups need it (the first group, of the
-    // highest scores, usually needs the most)
-    DevBuf<uint32_t> d_rec;
-    DevBuf<int32_t> d_slo;
-    DevBuf<int64_t> d_soff;
-    DevBuf<uint8_t> d_ck;
-    while (pos < order.size()) {
-        const int32_t s0 = scores[order[pos]] / ctx->g;  // widest in this group (sorted desc)
-        int32_t max_len = 0;
-        size_t end = pos;
-        // all pairs with score within this group share W and buffers
-        const int32_t W = std::max<int32_t>(capacity_for_score(ctx, s0), 16);
-        const int32_t steps = s0 + 2;
-        KArgs a = base_args(ctx);
-        for (end = pos; end < order.size(); ++end) {
-            const tsu_pair& pr = b->pairs[(size_t)order[end]];
-            max_len = std::max(max_len, std::max(pr.p_len, pr.t_len));
-            if (end - pos >= 65536) break;
-        }
-        a.W = W;
-        a.seq_words = seq_words_for(max_len);
-        const int64_t group_n = (int64_t)(end - pos);
-        Shape sh{default_waves(W), max_len <= 32000, true, true, true};
-        sh.fixed_242 = tsu_host::fixed_242(ctx);
-        int64_t recs = 0;
-        int32_t step_cap = steps;
-        const int c1 = tsu_host::reg_chunks(ctx, W);
-        if (c1 > 0 && max_len <= 16000 && s0 <= 16000) {
-            // single-wave kernel with the int16 LDS ring: records at a fixed
-            // place per step and chunk, (s * chunks + c) * 64 + lane
-            sh.nw = 1;
-            sh.reg_c = c1;
-            sh.lring = true;
-            recs = (int64_t)steps * c1 * 64;
-        } else if (int nwl = tsu_host::ml_waves(ctx, W, max_len)) {
-            // 16-chunk multi-wave kernel: records at a fixed place per step and
-            // chunk, (s * chunks + j) * 64 + lane
-            sh.nw = nwl;
-            sh.reg_c = 16;
-            sh.dbl = max_len <= tsu::kMlDblMaxLen && s0 <= tsu::kMlDblMaxScore;
-            recs = (int64_t)steps * nwl * 16 * 64;
-        } else {
-            for (int64_t s = 0; s <= s0; ++s)
-                recs += std::min<int64_t>(W, 2 * (std::max<int64_t>(0, s - ctx->os) / ctx->es + 2) + 1);
-            recs += 64;
-            if (tsu::lds_bytes(sh, a) > ctx->max_lds) { sh.lds_ring = false; sh.nw = 16; }
-        }
-        int64_t per_wg = recs * 4 + (int64_t)steps * 12 + (sh.lds_ring || sh.reg_c ? 0 : tsu::ring_bytes(sh, W, ctx->rm, ctx->ri));
-        // checkpoints instead of records when records do not fit one pair per CU
-        const bool ck = ctx->cigar_mode == TSU_CIGAR_CHECKPOINTS ||
-                        (ctx->cigar_mode == TSU_CIGAR_AUTO && s0 > 0 &&
-                         per_wg * std::min<int64_t>(group_n, ctx->cu_count) > budget);
-        if (ck) {
-            sh = Shape{std::max(4, default_waves(W)), max_len <= 32000, true, true, true};
-            sh.fixed_242 = tsu_host::fixed_242(ctx);
-            sh.ck = true;
-            if (tsu::lds_bytes(sh, a) > ctx->max_lds) { sh.lds_ring = false; sh.nw = 16; }
-            const int32_t nslot = ctx->rm + 2 * ctx->ri;
-            const int32_t osz = sh.off16 ? 2 : 4;
-            // window K balancing snapshots (s0 / K of nslot * W cells) against
-            // records (K * W words)
-            int32_t K = ctx->cigar_ck_every;
-            if (K <= 0) K = (int32_t)std::max(8.0, std::round(std::sqrt((double)s0 * nslot * osz / 4.0)));
-            K = std::max<int32_t>(1, std::min<int32_t>(K, std::max<int32_t>(s0, 1)));
-            a.ck_every = K;
-            a.ck_cap = (std::max<int32_t>(s0, 1) - 1) / K + 1;
-            a.ck_head = (nslot * 8 + 15) / 16 * 16;
-            a.ck_bytes = a.ck_head + ((int64_t)nslot * W * osz + 15) / 16 * 16;
-            recs = (int64_t)K * W + 64;
-            step_cap = K + 1;
-            per_wg = recs * 4 + (int64_t)step_cap * 12 + (int64_t)a.ck_cap * a.ck_bytes +
-                     (sh.lds_ring ? 0 : tsu::ring_bytes(sh, W, ctx->rm, ctx->ri));
-        }
+    // One backtrace launch of `ids` with `recs` record words and `step_cap`
+    // scores of state per workgroup (per_wg bytes in all).
+    auto launch_bt = [&](const Shape& sh, KArgs& a, int32_t W, const std::vector<int32_t>& ids, int32_t max_len,
+                         int64_t recs, int32_t step_cap, int64_t per_wg) -> int {
         int per_cu = 0;
         int rr = tsu::occupancy(sh, tsu::lds_bytes(sh, a), &per_cu);
-        if (rr != 0 || per_cu <= 0) { return fail(ctx, TSU_EINTERNAL, "backtrace shape has no occupancy"); }
-        int64_t grid = std::min<int64_t>((int64_t)per_cu * ctx->cu_count, group_n);
+        if (rr != 0 || per_cu <= 0) return fail(ctx, TSU_EINTERNAL, "backtrace shape has no occupancy");
+        int64_t grid = std::min<int64_t>((int64_t)per_cu * ctx->cu_count, (int64_t)ids.size());
         grid = std::max<int64_t>(1, std::min<int64_t>(grid, budget / std::max<int64_t>(per_wg, 1)));
         cg->state_bytes = std::max(cg->state_bytes, per_wg * grid);
-        if (ck) ++cg->checkpoint_launches; else ++cg->record_launches;
-        HIPCHK(d_rec.reserve((size_t)(recs * grid)), "hipMalloc (records)");
-        HIPCHK(d_slo.reserve((size_t)((int64_t)step_cap * grid)), "hipMalloc (steps)");
-        HIPCHK(d_soff.reserve((size_t)((int64_t)step_cap * grid)), "hipMalloc (steps)");
-        if (ck) {
+        if (sh.ck) ++cg->checkpoint_launches; else ++cg->record_launches;
+        void *rec = nullptr, *slo = nullptr, *soff = nullptr, *ckb = nullptr;
+        HIPCHK(bt_reserve(ctx, 0, (size_t)(recs * grid) * 4, &rec), "hipMalloc (records)");
+        HIPCHK(bt_reserve(ctx, 1, (size_t)((int64_t)step_cap * grid) * 4, &slo), "hipMalloc (steps)");
+        HIPCHK(bt_reserve(ctx, 2, (size_t)((int64_t)step_cap * grid) * 8, &soff), "hipMalloc (steps)");
+        if (sh.ck) {
             a.ck_stride = (int64_t)a.ck_cap * a.ck_bytes;
-            HIPCHK(d_ck.reserve((size_t)(a.ck_stride * grid)), "hipMalloc (checkpoints)");
-            a.ck_g = d_ck.p;
+            HIPCHK(bt_reserve(ctx, 3, (size_t)(a.ck_stride * grid), &ckb), "hipMalloc (checkpoints)");
+            a.ck_g = static_cast<uint8_t*>(ckb);
         }
-        std::vector<int32_t> ids(order.begin() + (ptrdiff_t)pos, order.begin() + (ptrdiff_t)end);
         HIPCHK(hipMemcpyAsync(d_order.p, ids.data(), ids.size() * sizeof(int32_t), hipMemcpyHostToDevice, ctx->stream), "hipMemcpyAsync");
-        a.rec_g = d_rec.p; a.rec_cap = recs;
-        a.step_lo_g = d_slo.p; a.step_off_g = d_soff.p; a.step_cap = step_cap;
-        a.runs = d_runs.p; a.runs_cap = std::max<int64_t>(runs_total, 1); a.runs_used = d_used.p;
+        a.rec_g = static_cast<uint32_t*>(rec); a.rec_cap = recs;
+        a.step_lo_g = static_cast<int32_t*>(slo); a.step_off_g = static_cast<int64_t*>(soff); a.step_cap = step_cap;
+        a.runs = d_runs.p; a.runs_cap = std::max<int64_t>(runs_cap, 1); a.runs_used = d_used.p;
         a.run_off = d_run_off.p; a.run_n = d_run_n.p;
         double kms = 0;
-        r = run_shape(ctx, b, sh, W, d_order.p, (int32_t)ids.size(), max_len, grid, &a, &kms);
+        int r = run_shape(ctx, b, sh, W, d_order.p, (int32_t)ids.size(), max_len, grid, &a, &kms);
         if (r != TSU_OK) return r;
         kms_total += kms;
         ++launches;
-        pos = end;
-    }
+        return TSU_OK;
+    };
     std::vector<int32_t> sc2((size_t)b->n);
+    std::vector<uint8_t> done((size_t)b->n, 0);
+
+    // 1) Record launches without a score pass, per length bucket of the
+    // batched form at its capacity, where a register kernel records: the
+    // state is sized for the highest score the window (or the lengths)
+    // allow.  A pair whose window overflows, or whose score passes that
+    // step bound, is left to the score-first path below.
+    if (ctx->cigar_mode != TSU_CIGAR_CHECKPOINTS && tsu_host::fixed_242(ctx)) {
+        struct Plan {
+            std::vector<int32_t> ids;
+            Shape sh;
+            KArgs a;
+            int32_t W, max_len, step_cap;
+            int64_t recs, per_wg;
+        };
+        std::vector<Plan> plans;
+        const int nb = tsu_host::bucket_count();
+        std::vector<std::vector<int32_t>> bk((size_t)nb);
+        for (int32_t i = 0; i < b->n; ++i)
+            bk[(size_t)tsu_host::bucket_of(std::max(b->pairs[(size_t)i].p_len, b->pairs[(size_t)i].t_len))].push_back(i);
+        int64_t runs_need = 0;
+        for (int bi = nb - 1; bi >= 0; --bi) {
+            std::vector<int32_t>& ids = bk[(size_t)bi];
+            if (ids.empty()) continue;
+            int32_t max_len = 0;
+            int64_t s_len = 0;
+            for (int32_t i : ids) {
+                const tsu_pair& pr = b->pairs[(size_t)i];
+                max_len = std::max(max_len, std::max(pr.p_len, pr.t_len));
+                s_len = std::max(s_len, score_bound(ctx, pr.p_len, pr.t_len));
+            }
+            const int32_t bound = tsu_host::bucket_bound(bi);
+            const tsu_tune_entry* te = tune_lookup(ctx, bound);
+            if (te && te->waves_per_pair > 1) continue;  // tuned for the LDS kernel
+            int32_t W = (te && te->capacity > 0) ? te->capacity : initial_capacity(ctx, std::min(bound, max_len));
+            W = std::min<int32_t>(W, capacity_full(max_len, max_len));
+            Shape sh{default_waves(W), max_len <= 32000, true, true, true};
+            sh.fixed_242 = true;
+            // the kernel's own window: 64 diagonals per chunk
+            int32_t win = 0;
+            const int c1 = tsu_host::reg_chunks(ctx, W);
+            if (c1 > 0 && max_len <= 16000) {
+                sh.nw = 1;
+                sh.reg_c = c1;
+                sh.lring = true;
+                win = 64 * c1;
+            } else if (int nwl = tsu_host::ml_waves(ctx, W, max_len)) {
+                sh.nw = nwl;
+                sh.reg_c = 16;
+                win = 1024 * nwl;
+            } else {
+                continue;  // the LDS kernel records per score's width: sized from scores
+            }
+            // a window holding every diagonal of the longest pair never overflows
+            int64_t s_cap = win >= capacity_full(max_len, max_len) ? s_len : std::min(s_len, score_for_capacity(ctx, win));
+            if (s_cap < 0 || (sh.nw == 1 && s_cap > 16000)) continue;
+            KArgs a = base_args(ctx);
+            a.W = W;
+            a.seq_words = seq_words_for(max_len);
+            int per_cu = 0;
+            if (tsu::occupancy(sh, tsu::lds_bytes(sh, a), &per_cu) != 0 || per_cu <= 0) continue;
+            // the launch's state (records of every resident pair) stays under
+            // the single-pass limit: fewer steps if needed, unless that leaves
+            // less than a third of the window's scores
+            const int64_t resident = std::min<int64_t>((int64_t)per_cu * ctx->cu_count, (int64_t)ids.size());
+            const int64_t step_bytes = (int64_t)win * 4 + 12;
+            const int64_t fit = std::min(budget, kSinglePassState) / (resident * step_bytes) - 2;
+            if (fit < s_cap) {
+                if (fit * 3 < s_cap) continue;
+                s_cap = fit;
+            }
+            const int32_t step_cap = (int32_t)(s_cap + 2);
+            const int64_t recs = (int64_t)step_cap * win;
+            if (sh.nw > 1) sh.dbl = max_len <= tsu::kMlDblMaxLen && s_cap <= tsu::kMlDblMaxScore;
+            const int64_t per_wg = recs * 4 + (int64_t)step_cap * 12;
+            for (int32_t i : ids) {
+                const tsu_pair& pr = b->pairs[(size_t)i];
+                runs_need += 2 * (std::min(s_cap + 1, score_bound(ctx, pr.p_len, pr.t_len)) / unit) + 2;
+            }
+            tsu_host::order_longest_first(ids, b->pairs.data());
+            plans.push_back(Plan{std::move(ids), sh, a, W, max_len, step_cap, recs, per_wg});
+        }
+        if (!plans.empty()) {
+            runs_cap = std::max<int64_t>(runs_need, 1);
+            HIPCHK(d_runs.alloc((size_t)runs_cap), "hipMalloc (runs)");
+            for (Plan& pl : plans) {
+                int r = launch_bt(pl.sh, pl.a, pl.W, pl.ids, pl.max_len, pl.recs, pl.step_cap, pl.per_wg);
+                if (r != TSU_OK) return r;
+            }
+            HIPCHK(hipMemcpy(sc2.data(), b->d_scores, (size_t)b->n * sizeof(int32_t), hipMemcpyDeviceToHost), "hipMemcpy D2H");
+            for (const Plan& pl : plans)
+                for (int32_t i : pl.ids) {
+                    const int32_t v = sc2[(size_t)i];
+                    if (v >= 0) { scores[i] = v; done[(size_t)i] = 1; }
+                    else if (v != TSU_EKOVERFLOW && v != TSU_ECIGAR)
+                        return fail(ctx, v, "pair " + std::to_string(i) + ": " + tsu_strerror(v));
+                }
+        }
+    }
+
+    // 2) The other pairs: scores first (batched form), which size the
+    // backtrace state exactly, then launches grouped by score.
+    std::vector<int32_t> order;
+    for (int32_t i = 0; i < b->n; ++i)
+        if (!done[(size_t)i]) order.push_back(i);
+    if (!order.empty()) {
+        int r = batched_core(ctx, b, order, scores, t, true);
+        if (r != TSU_OK) return r;
+        for (int32_t i : order)
+            if (scores[i] < 0) return fail(ctx, scores[i], "pair " + std::to_string(i) + ": " + tsu_strerror(scores[i]));
+        tsu_host::order_desc(order, [&](int32_t i) -> int64_t { return scores[i]; });
+        // their runs go after what the record launches above used
+        unsigned long long used0 = 0;
+        HIPCHK(hipMemcpy(&used0, d_used.p, sizeof(used0), hipMemcpyDeviceToHost), "hipMemcpy D2H");
+        int64_t need = (int64_t)used0;
+        for (int32_t i : order) need += 2ll * (scores[i] / ctx->g / unit) + 2;
+        if (need > runs_cap) {
+            DevBuf<uint32_t> grown;
+            HIPCHK(grown.alloc((size_t)need), "hipMalloc (runs)");
+            if (used0) HIPCHK(hipMemcpy(grown.p, d_runs.p, (size_t)used0 * 4, hipMemcpyDeviceToDevice), "hipMemcpy D2D");
+            std::swap(grown.p, d_runs.p);
+            std::swap(grown.n, d_runs.n);
+            runs_cap = need;
+        }
+        // group pairs into launches of similar score so buffers fit the widest
+        size_t pos = 0;
+        while (pos < order.size()) {
+            const int32_t s0 = scores[order[pos]] / ctx->g;  // widest in this group (sorted desc)
+            int32_t max_len = 0;
+            size_t end = pos;
+            // all pairs with score within this group share W and buffers
+            const int32_t W = std::max<int32_t>(capacity_for_score(ctx, s0), 16);
+            const int32_t steps = s0 + 2;
+            KArgs a = base_args(ctx);
+            for (end = pos; end < order.size(); ++end) {
+                const tsu_pair& pr = b->pairs[(size_t)order[end]];
+                max_len = std::max(max_len, std::max(pr.p_len, pr.t_len));
+                if (end - pos >= 65536) break;
+            }
+            a.W = W;
+            a.seq_words = seq_words_for(max_len);
+            const int64_t group_n = (int64_t)(end - pos);
+            Shape sh{default_waves(W), max_len <= 32000, true, true, true};
+            sh.fixed_242 = tsu_host::fixed_242(ctx);
+            int64_t recs = 0;
+            int32_t step_cap = steps;
+            const int c1 = tsu_host::reg_chunks(ctx, W);
+            if (c1 > 0 && max_len <= 16000 && s0 <= 16000) {
+                // single-wave kernel with the int16 LDS ring: records at a fixed
+                // place per step and chunk, (s * chunks + c) * 64 + lane
+                sh.nw = 1;
+                sh.reg_c = c1;
+                sh.lring = true;
+                recs = (int64_t)steps * c1 * 64;
+            } else if (int nwl = tsu_host::ml_waves(ctx, W, max_len)) {
+                // 16-chunk multi-wave kernel: records at a fixed place per step and
+                // chunk, (s * chunks + j) * 64 + lane
+                sh.nw = nwl;
+                sh.reg_c = 16;
+                sh.dbl = max_len <= tsu::kMlDblMaxLen && s0 <= tsu::kMlDblMaxScore;
+                recs = (int64_t)steps * nwl * 16 * 64;
+            } else {
+                for (int64_t s = 0; s <= s0; ++s)
+                    recs += std::min<int64_t>(W, 2 * (std::max<int64_t>(0, s - ctx->os) / ctx->es + 2) + 1);
+                recs += 64;
+                if (tsu::lds_bytes(sh, a) > ctx->max_lds) { sh.lds_ring = false; sh.nw = 16; }
+            }
+            int64_t per_wg = recs * 4 + (int64_t)steps * 12 + (sh.lds_ring || sh.reg_c ? 0 : tsu::ring_bytes(sh, W, ctx->rm, ctx->ri));
+            // checkpoints instead of records when records do not fit one pair per CU
+            const bool ck = ctx->cigar_mode == TSU_CIGAR_CHECKPOINTS ||
+                            (ctx->cigar_mode == TSU_CIGAR_AUTO && s0 > 0 &&
+                             per_wg * std::min<int64_t>(group_n, ctx->cu_count) > budget);
+            if (ck) {
+                sh = Shape{std::max(4, default_waves(W)), max_len <= 32000, true, true, true};
+                sh.fixed_242 = tsu_host::fixed_242(ctx);
+                sh.ck = true;
+                if (tsu::lds_bytes(sh, a) > ctx->max_lds) { sh.lds_ring = false; sh.nw = 16; }
+                const int32_t nslot = ctx->rm + 2 * ctx->ri;
+                const int32_t osz = sh.off16 ? 2 : 4;
+                // window K balancing snapshots (s0 / K of nslot * W cells) against
+                // records (K * W words)
+                int32_t K = ctx->cigar_ck_every;
+                if (K <= 0) K = (int32_t)std::max(8.0, std::round(std::sqrt((double)s0 * nslot * osz / 4.0)));
+                K = std::max<int32_t>(1, std::min<int32_t>(K, std::max<int32_t>(s0, 1)));
+                a.ck_every = K;
+                a.ck_cap = (std::max<int32_t>(s0, 1) - 1) / K + 1;
+                a.ck_head = (nslot * 8 + 15) / 16 * 16;
+                a.ck_bytes = a.ck_head + ((int64_t)nslot * W * osz + 15) / 16 * 16;
+                recs = (int64_t)K * W + 64;
+                step_cap = K + 1;
+                per_wg = recs * 4 + (int64_t)step_cap * 12 + (int64_t)a.ck_cap * a.ck_bytes +
+                         (sh.lds_ring ? 0 : tsu::ring_bytes(sh, W, ctx->rm, ctx->ri));
+            }
+            std::vector<int32_t> ids(order.begin() + (ptrdiff_t)pos, order.begin() + (ptrdiff_t)end);
+            r = launch_bt(sh, a, W, ids, max_len, recs, step_cap, per_wg);
+            if (r != TSU_OK) return r;
+            pos = end;
+        }
+    }
     HIPCHK(hipMemcpy(sc2.data(), b->d_scores, (size_t)b->n * sizeof(int32_t), hipMemcpyDeviceToHost), "hipMemcpy D2H");
     unsigned long long used = 0;
     HIPCHK(hipMemcpy(&used, d_used.p, sizeof(used), hipMemcpyDeviceToHost), "hipMemcpy D2H");
@@ -1056,7 +1222,8 @@ extern "C" int tsu_align_cigar(tsu_ctx* ctx, tsu_batch* b, int32_t* scores, tsu_
     }
     const double w1 = now_ms();
     if (t) { t->kernel_ms += kms_total; t->launches += launches; t->wall_ms = w1 - w0; t->pairs = b->n; t->cells = b->cells; }
-    for (int32_t i = 0; i < b->n; ++i) {
+    // pairs of the score-first path: the backtrace launch's score equals the score pass's
+    for (int32_t i : order) {
         if (sc2[(size_t)i] != scores[i]) {
             const int st = sc2[(size_t)i] < 0 ? sc2[(size_t)i] : TSU_EINTERNAL;
             return fail(ctx, st, "pair " + std::to_string(i) + ": backtrace pass returned " + std::to_string(sc2[(size_t)i]) +
@@ -1074,6 +1241,8 @@ extern "C" int tsu_set_cigar_memory(tsu_ctx* ctx, int32_t mode, int32_t checkpoi
     ctx->cigar_mode = mode;
     ctx->cigar_ck_every = checkpoint_every;
     ctx->cigar_budget = budget_bytes;
+    HIPCHK(hipSetDevice(ctx->device), "hipSetDevice");
+    bt_release(ctx);  // the next call sizes its state for the new setting
     return TSU_OK;
 }
 
