@@ -920,10 +920,14 @@ static int copy_to_host(tsu_ctx* ctx, void* dst, const void* src, size_t bytes) 
     return TSU_OK;
 }
 
-// Backtrace state of one single-pass CIGAR launch at most: freeing a much
-// larger device buffer takes about a second on MI355X (137 GiB: 1.04 s,
-// 68 GiB: under 1 ms), more than the score pass the single pass saves.
-constexpr int64_t kSinglePassState = 64ll << 30;
+// Kernel family of a CIGAR launch at capacity W for a pair of length len
+// and scaled score s: 1 the single-wave register kernel, 2 + waves the
+// 16-chunk kernel, 0 the LDS kernel.
+static int bt_family(const tsu_ctx* ctx, int32_t W, int32_t len, int64_t s) {
+    if (tsu_host::reg_chunks(ctx, W) > 0 && len <= 16000 && s <= 16000) return 1;
+    if (int nw = tsu_host::ml_waves(ctx, W, len)) return 2 + nw;
+    return 0;
+}
 
 // Highest scaled score a pair of lengths (p, t) can have: min(p, t)
 // mismatches and one gap of |p - t|.
@@ -1069,12 +1073,12 @@ extern "C" int tsu_align_cigar(tsu_ctx* ctx, tsu_batch* b, int32_t* scores, tsu_
             a.seq_words = seq_words_for(max_len);
             int per_cu = 0;
             if (tsu::occupancy(sh, tsu::lds_bytes(sh, a), &per_cu) != 0 || per_cu <= 0) continue;
-            // the launch's state (records of every resident pair) stays under
-            // the single-pass limit: fewer steps if needed, unless that leaves
-            // less than a third of the window's scores
+            // the launch's state (records of every resident pair) stays within
+            // the budget: fewer steps if needed, unless that leaves less than a
+            // third of the window's scores
             const int64_t resident = std::min<int64_t>((int64_t)per_cu * ctx->cu_count, (int64_t)ids.size());
             const int64_t step_bytes = (int64_t)win * 4 + 12;
-            const int64_t fit = std::min(budget, kSinglePassState) / (resident * step_bytes) - 2;
+            const int64_t fit = budget / (resident * step_bytes) - 2;
             if (fit < s_cap) {
                 if (fit * 3 < s_cap) continue;
                 s_cap = fit;
@@ -1142,9 +1146,18 @@ extern "C" int tsu_align_cigar(tsu_ctx* ctx, tsu_batch* b, int32_t* scores, tsu_
             const int32_t W = std::max<int32_t>(capacity_for_score(ctx, s0), 16);
             const int32_t steps = s0 + 2;
             KArgs a = base_args(ctx);
+            // the group ends where the kernel family of a pair's own capacity
+            // changes: a few pairs that need the LDS kernel do not take the
+            // register kernels' pairs with them
+            int family = -1;
             for (end = pos; end < order.size(); ++end) {
                 const tsu_pair& pr = b->pairs[(size_t)order[end]];
-                max_len = std::max(max_len, std::max(pr.p_len, pr.t_len));
+                const int32_t len = std::max(pr.p_len, pr.t_len);
+                const int32_t si = scores[order[end]] / ctx->g;
+                const int f = bt_family(ctx, std::max<int32_t>(capacity_for_score(ctx, si), 16), len, si);
+                if (family < 0) family = f;
+                else if (f != family) break;
+                max_len = std::max(max_len, len);
                 if (end - pos >= 65536) break;
             }
             a.W = W;
