@@ -125,6 +125,27 @@ def test_other_penalties_checkpointed(tsu, oracle):
             a.close()
 
 
+def test_single_pass_step_bound_from_budget(ck_aligner, tsu, oracle):
+    """A 150 MiB budget lowers the single pass's step bound below what the
+    4096-diagonal window holds (1,064 of 2,046 scaled scores for 9 resident
+    pairs): the 3-3.5 kbp pairs at 5% end within it, the unrelated pairs pass
+    it and are done score first, with checkpoints at that budget.  Scores and
+    CIGARs equal the oracle's."""
+    rng = np.random.default_rng(43)
+    pairs = tsu.Dataset.generate(6, seed=SEED + 41, len_min=3000, len_max=3500, err_min=0.05).pairs()
+    for n in (2500, 3000, 3500):
+        pairs.append(("".join(rng.choice(list("ACGT"), n)), "".join(rng.choice(list("ACGT"), n))))
+    say("single pass under a 150 MiB budget, 9 pairs")
+    ck_aligner.set_cigar_memory(ck_aligner.CIGAR_AUTO, 0, 150 << 20)
+    try:
+        got, cig, t = ck_aligner.align_cigar(ck_aligner.batch(pairs))
+    finally:
+        ck_aligner.set_cigar_memory(ck_aligner.CIGAR_AUTO)
+    _check_vs_oracle(got, cig, pairs, oracle)
+    assert t["record_launches"] >= 1 and t["checkpoint_launches"] >= 1
+    assert t["state_bytes"] <= 150 << 20
+
+
 def test_set_cigar_memory_rejects_bad_arguments(ck_aligner, tsu):
     for args in [(3, 0, 0), (-1, 0, 0), (0, -1, 0), (0, 0, -1)]:
         with pytest.raises(tsu.TsuError):
