@@ -1011,8 +1011,24 @@ extern "C" int tsu_align_cigar(tsu_ctx* ctx, tsu_batch* b, int32_t* scores, tsu_
         ++launches;
         return TSU_OK;
     };
+    // makes room in the run pool for `more` words after what was reserved so far
+    auto grow_runs = [&](int64_t more) -> int {
+        unsigned long long used0 = 0;
+        HIPCHK(hipMemcpy(&used0, d_used.p, sizeof(used0), hipMemcpyDeviceToHost), "hipMemcpy D2H");
+        const int64_t need = (int64_t)used0 + more;
+        if (need > runs_cap) {
+            DevBuf<uint32_t> grown;
+            HIPCHK(grown.alloc((size_t)need), "hipMalloc (runs)");
+            if (used0) HIPCHK(hipMemcpy(grown.p, d_runs.p, (size_t)used0 * 4, hipMemcpyDeviceToDevice), "hipMemcpy D2D");
+            std::swap(grown.p, d_runs.p);
+            std::swap(grown.n, d_runs.n);
+            runs_cap = need;
+        }
+        return TSU_OK;
+    };
     std::vector<int32_t> sc2((size_t)b->n);
     std::vector<uint8_t> done((size_t)b->n, 0);
+    std::vector<int32_t> wide;  // pairs that overflowed the widest register window
 
     // 1) Record launches without a score pass, per length bucket of the
     // batched form at its capacity, where a register kernel records: the
@@ -1024,8 +1040,9 @@ extern "C" int tsu_align_cigar(tsu_ctx* ctx, tsu_batch* b, int32_t* scores, tsu_
             std::vector<int32_t> ids;
             Shape sh;
             KArgs a;
-            int32_t W, max_len, step_cap;
+            int32_t W, max_len, step_cap, win;
             int64_t recs, per_wg;
+            bool at_window;  // the step bound is the window's: a pair past it needs a wider one
         };
         std::vector<Plan> plans;
         const int nb = tsu_host::bucket_count();
@@ -1079,6 +1096,7 @@ extern "C" int tsu_align_cigar(tsu_ctx* ctx, tsu_batch* b, int32_t* scores, tsu_
             const int64_t resident = std::min<int64_t>((int64_t)per_cu * ctx->cu_count, (int64_t)ids.size());
             const int64_t step_bytes = (int64_t)win * 4 + 12;
             const int64_t fit = budget / (resident * step_bytes) - 2;
+            const int64_t s_win = s_cap;
             if (fit < s_cap) {
                 if (fit * 3 < s_cap) continue;
                 s_cap = fit;
@@ -1092,7 +1110,8 @@ extern "C" int tsu_align_cigar(tsu_ctx* ctx, tsu_batch* b, int32_t* scores, tsu_
                 runs_need += 2 * (std::min(s_cap + 1, score_bound(ctx, pr.p_len, pr.t_len)) / unit) + 2;
             }
             tsu_host::order_longest_first(ids, b->pairs.data());
-            plans.push_back(Plan{std::move(ids), sh, a, W, max_len, step_cap, recs, per_wg});
+            plans.push_back(Plan{std::move(ids), sh, a, W, max_len, step_cap, win, recs, per_wg,
+                                 s_cap == s_win && s_win < s_len});
         }
         if (!plans.empty()) {
             runs_cap = std::max<int64_t>(runs_need, 1);
@@ -1106,13 +1125,57 @@ extern "C" int tsu_align_cigar(tsu_ctx* ctx, tsu_batch* b, int32_t* scores, tsu_
                 for (int32_t i : pl.ids) {
                     const int32_t v = sc2[(size_t)i];
                     if (v >= 0) { scores[i] = v; done[(size_t)i] = 1; }
+                    else if (pl.win >= 16 * 1024 && (v == TSU_EKOVERFLOW || (v == TSU_ECIGAR && pl.at_window)))
+                        wide.push_back(i);
                     else if (v != TSU_EKOVERFLOW && v != TSU_ECIGAR)
                         return fail(ctx, v, "pair " + std::to_string(i) + ": " + tsu_strerror(v));
                 }
         }
     }
 
-    // 2) The other pairs: scores first (batched form), which size the
+    // 2) Pairs wider than every register window: one LDS-kernel launch over
+    // the whole diagonal range of their lengths, records sized for the
+    // highest score the lengths allow, when that fits the budget (a few long
+    // divergent pairs, each alone on a CU: a score pass first would double
+    // their time).
+    if (!wide.empty()) {
+        int32_t max_len = 0;
+        int64_t s_len = 0, runs_more = 0;
+        for (int32_t i : wide) {
+            const tsu_pair& pr = b->pairs[(size_t)i];
+            max_len = std::max(max_len, std::max(pr.p_len, pr.t_len));
+            const int64_t sb = score_bound(ctx, pr.p_len, pr.t_len);
+            s_len = std::max(s_len, sb);
+            runs_more += 2 * (sb / unit) + 2;
+        }
+        const int32_t W = capacity_full(max_len, max_len);
+        const int32_t step_cap = (int32_t)std::min<int64_t>(s_len + 2, 1 << 30);
+        int64_t recs = 64;
+        for (int64_t s = 0; s <= s_len; ++s)
+            recs += std::min<int64_t>(W, 2 * (std::max<int64_t>(0, s - ctx->os) / ctx->es + 2) + 1);
+        KArgs a = base_args(ctx);
+        a.W = W;
+        a.seq_words = seq_words_for(max_len);
+        Shape sh{default_waves(W), max_len <= 32000, true, true, true};
+        sh.fixed_242 = tsu_host::fixed_242(ctx);
+        if (tsu::lds_bytes(sh, a) > ctx->max_lds) { sh.lds_ring = false; sh.nw = 16; }
+        const int64_t per_wg = recs * 4 + (int64_t)step_cap * 12 + (sh.lds_ring ? 0 : tsu::ring_bytes(sh, W, ctx->rm, ctx->ri));
+        if (per_wg * (int64_t)wide.size() <= budget) {
+            int r = grow_runs(runs_more);
+            if (r != TSU_OK) return r;
+            r = launch_bt(sh, a, W, wide, max_len, recs, step_cap, per_wg);
+            if (r != TSU_OK) return r;
+            HIPCHK(hipMemcpy(sc2.data(), b->d_scores, (size_t)b->n * sizeof(int32_t), hipMemcpyDeviceToHost), "hipMemcpy D2H");
+            for (int32_t i : wide) {
+                const int32_t v = sc2[(size_t)i];
+                if (v >= 0) { scores[i] = v; done[(size_t)i] = 1; }
+                else if (v != TSU_EKOVERFLOW && v != TSU_ECIGAR)
+                    return fail(ctx, v, "pair " + std::to_string(i) + ": " + tsu_strerror(v));
+            }
+        }
+    }
+
+    // 3) The other pairs: scores first (batched form), which size the
     // backtrace state exactly, then launches grouped by score.
     std::vector<int32_t> order;
     for (int32_t i = 0; i < b->n; ++i)
@@ -1124,18 +1187,10 @@ extern "C" int tsu_align_cigar(tsu_ctx* ctx, tsu_batch* b, int32_t* scores, tsu_
             if (scores[i] < 0) return fail(ctx, scores[i], "pair " + std::to_string(i) + ": " + tsu_strerror(scores[i]));
         tsu_host::order_desc(order, [&](int32_t i) -> int64_t { return scores[i]; });
         // their runs go after what the record launches above used
-        unsigned long long used0 = 0;
-        HIPCHK(hipMemcpy(&used0, d_used.p, sizeof(used0), hipMemcpyDeviceToHost), "hipMemcpy D2H");
-        int64_t need = (int64_t)used0;
-        for (int32_t i : order) need += 2ll * (scores[i] / ctx->g / unit) + 2;
-        if (need > runs_cap) {
-            DevBuf<uint32_t> grown;
-            HIPCHK(grown.alloc((size_t)need), "hipMalloc (runs)");
-            if (used0) HIPCHK(hipMemcpy(grown.p, d_runs.p, (size_t)used0 * 4, hipMemcpyDeviceToDevice), "hipMemcpy D2D");
-            std::swap(grown.p, d_runs.p);
-            std::swap(grown.n, d_runs.n);
-            runs_cap = need;
-        }
+        int64_t more = 0;
+        for (int32_t i : order) more += 2ll * (scores[i] / ctx->g / unit) + 2;
+        r = grow_runs(more);
+        if (r != TSU_OK) return r;
         // group pairs into launches of similar score so buffers fit the widest
         size_t pos = 0;
         while (pos < order.size()) {
