@@ -207,8 +207,10 @@ int tsu_tune_table(const tsu_ctx* ctx, tsu_tune_entry* entries, int32_t cap, int
 /* Scores and CIGARs.  Backtrace follows WFA2-lib's piggy-backed priority
  * (common.h:67-77).  Pairs whose length bucket runs on a register kernel are
  * recorded in one pass, with state sized for the highest score the kernel's
- * window holds; the others, and pairs beyond that bound, get their scores
- * first (the batched form), which size the state exactly.  Each CIGAR is a run list, run = (length << 2) | op with
+ * window holds; pairs wider than the widest register window are recorded in
+ * one launch over their whole diagonal range when that fits the budget; the
+ * others get their scores first (the batched form), which size the state
+ * exactly.  Each CIGAR is a run list, run = (length << 2) | op with
  * op 0 'M' match, 1 'X' mismatch, 2 'I' insertion (consumes text), 3 'D'
  * deletion (consumes pattern). */
 int tsu_align_cigar(tsu_ctx* ctx, tsu_batch* b, int32_t* scores, tsu_cigars** out, tsu_timing* t);
