@@ -149,7 +149,7 @@ extern "C" int tsu_create_2p(tsu_ctx** out, int device, int32_t mismatch, int32_
 
 // Frees the backtrace state the context keeps between tsu_align_cigar calls.
 static void bt_release(tsu_ctx* ctx) {
-    for (int i = 0; i < 4; ++i) {
+    for (int i = 0; i < 7; ++i) {
         if (ctx->bt_buf[i]) (void)hipFree(ctx->bt_buf[i]);
         ctx->bt_buf[i] = nullptr;
         ctx->bt_bytes[i] = 0;
@@ -977,7 +977,7 @@ extern "C" int tsu_align_cigar(tsu_ctx* ctx, tsu_batch* b, int32_t* scores, tsu_
     size_t free_b = 0, total_b = 0;
     HIPCHK(hipMemGetInfo(&free_b, &total_b), "hipMemGetInfo");
     // the state this context already holds counts as free
-    for (int i = 0; i < 4; ++i) free_b += ctx->bt_bytes[i];
+    for (int i = 0; i < 7; ++i) free_b += ctx->bt_bytes[i];
     const int64_t budget = ctx->cigar_budget > 0 ? ctx->cigar_budget : (int64_t)(0.5 * (double)free_b);
     // One backtrace launch of `ids` with `recs` record words and `step_cap`
     // scores of state per workgroup (per_wg bytes in all).
@@ -1029,6 +1029,101 @@ extern "C" int tsu_align_cigar(tsu_ctx* ctx, tsu_batch* b, int32_t* scores, tsu_
     std::vector<int32_t> sc2((size_t)b->n);
     std::vector<uint8_t> done((size_t)b->n, 0);
     std::vector<int32_t> wide;  // pairs that overflowed the widest register window
+
+    // Pairs wider than every register window (they overflowed the 16,384
+    // diagonals of the 16-chunk kernel): one LDS-kernel launch over the whole
+    // diagonal range of their lengths, records sized for the highest score
+    // the lengths allow, when that fits the budget (a few long divergent
+    // pairs, each alone on a CU: a score pass first would double their time).
+    struct WidePlan {
+        Shape sh;
+        KArgs a;
+        int32_t W, max_len, step_cap;
+        int64_t recs, per_wg, runs;
+    };
+    auto plan_wide = [&](const std::vector<int32_t>& ids) -> WidePlan {
+        WidePlan wp{};
+        int64_t s_len = 0;
+        wp.max_len = 0;
+        wp.runs = 0;
+        for (int32_t i : ids) {
+            const tsu_pair& pr = b->pairs[(size_t)i];
+            wp.max_len = std::max(wp.max_len, std::max(pr.p_len, pr.t_len));
+            const int64_t sb = score_bound(ctx, pr.p_len, pr.t_len);
+            s_len = std::max(s_len, sb);
+            wp.runs += 2 * (sb / unit) + 2;
+        }
+        wp.W = capacity_full(wp.max_len, wp.max_len);
+        wp.step_cap = (int32_t)std::min<int64_t>(s_len + 2, 1 << 30);
+        wp.recs = 64;
+        for (int64_t s = 0; s <= s_len; ++s)
+            wp.recs += std::min<int64_t>(wp.W, 2 * (std::max<int64_t>(0, s - ctx->os) / ctx->es + 2) + 1);
+        wp.a = base_args(ctx);
+        wp.a.W = wp.W;
+        wp.a.seq_words = seq_words_for(wp.max_len);
+        wp.sh = Shape{default_waves(wp.W), wp.max_len <= 32000, true, true, true};
+        wp.sh.fixed_242 = tsu_host::fixed_242(ctx);
+        if (tsu::lds_bytes(wp.sh, wp.a) > ctx->max_lds) { wp.sh.lds_ring = false; wp.sh.nw = 16; }
+        wp.per_wg = wp.recs * 4 + (int64_t)wp.step_cap * 12 +
+                    (wp.sh.lds_ring ? 0 : tsu::ring_bytes(wp.sh, wp.W, ctx->rm, ctx->ri));
+        return wp;
+    };
+    auto settle_wide = [&](const std::vector<int32_t>& ids) -> int {
+        for (int32_t i : ids) {
+            const int32_t v = sc2[(size_t)i];
+            if (v >= 0) { scores[i] = v; done[(size_t)i] = 1; }
+            else if (v != TSU_EKOVERFLOW && v != TSU_ECIGAR)
+                return fail(ctx, v, "pair " + std::to_string(i) + ": " + tsu_strerror(v));
+        }
+        return TSU_OK;
+    };
+    // The same launch on the second stream, beside the main stream's
+    // launches, with state of its own (context slots 4-6; the run pool was
+    // reserved for these pairs up front).  Leaves `launched` false when that
+    // state cannot be allocated now: the launch then runs afterwards.
+    struct WideAsync {
+        DevBuf<int32_t> d_ids;
+        DevBuf<uint8_t> ring;
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        bool launched = false;
+        hipStream_t s = nullptr;
+        ~WideAsync() {
+            if (s) (void)hipStreamSynchronize(s);
+            if (e0) (void)hipEventDestroy(e0);
+            if (e1) (void)hipEventDestroy(e1);
+        }
+    } wide_async;
+    auto launch_wide_async = [&](WidePlan& wp, const std::vector<int32_t>& ids) -> int {
+        int per_cu = 0;
+        if (tsu::occupancy(wp.sh, tsu::lds_bytes(wp.sh, wp.a), &per_cu) != 0 || per_cu <= 0) return TSU_OK;
+        int64_t grid = std::min<int64_t>((int64_t)per_cu * ctx->cu_count, (int64_t)ids.size());
+        grid = std::max<int64_t>(1, std::min<int64_t>(grid, budget / std::max<int64_t>(wp.per_wg, 1)));
+        void *rec = nullptr, *slo = nullptr, *soff = nullptr;
+        if (bt_reserve(ctx, 4, (size_t)(wp.recs * grid) * 4, &rec) != hipSuccess ||
+            bt_reserve(ctx, 5, (size_t)((int64_t)wp.step_cap * grid) * 4, &slo) != hipSuccess ||
+            bt_reserve(ctx, 6, (size_t)((int64_t)wp.step_cap * grid) * 8, &soff) != hipSuccess ||
+            wide_async.d_ids.alloc(ids.size()) != hipSuccess) {
+            (void)hipGetLastError();  // out of memory now: run it afterwards instead
+            return TSU_OK;
+        }
+        HIPCHK(hipEventCreate(&wide_async.e0), "hipEventCreate");
+        HIPCHK(hipEventCreate(&wide_async.e1), "hipEventCreate");
+        HIPCHK(hipMemcpyAsync(wide_async.d_ids.p, ids.data(), ids.size() * sizeof(int32_t), hipMemcpyHostToDevice, ctx->stream2),
+               "hipMemcpyAsync");
+        KArgs a = wp.a;
+        a.rec_g = static_cast<uint32_t*>(rec); a.rec_cap = wp.recs;
+        a.step_lo_g = static_cast<int32_t*>(slo); a.step_off_g = static_cast<int64_t*>(soff); a.step_cap = wp.step_cap;
+        a.runs = d_runs.p; a.runs_cap = std::max<int64_t>(runs_cap, 1); a.runs_used = d_used.p;
+        a.run_off = d_run_off.p; a.run_n = d_run_n.p;
+        wide_async.s = ctx->stream2;
+        int r = launch_shape(ctx, b, wp.sh, wp.W, wide_async.d_ids.p, (int32_t)ids.size(), wp.max_len, grid, &a, ctx->stream2,
+                             ctx->d_queue + 16, wide_async.e0, wide_async.e1, wide_async.ring, &wide_async.launched);
+        if (r != TSU_OK) return r;
+        cg->state_bytes = std::max(cg->state_bytes, wp.per_wg * grid);
+        ++cg->record_launches;
+        ++launches;
+        return TSU_OK;
+    };
 
     // 1) Record launches without a score pass, per length bucket of the
     // batched form at its capacity, where a register kernel records: the
@@ -1105,9 +1200,12 @@ extern "C" int tsu_align_cigar(tsu_ctx* ctx, tsu_batch* b, int32_t* scores, tsu_
             const int64_t recs = (int64_t)step_cap * win;
             if (sh.nw > 1) sh.dbl = max_len <= tsu::kMlDblMaxLen && s_cap <= tsu::kMlDblMaxScore;
             const int64_t per_wg = recs * 4 + (int64_t)step_cap * 12;
+            // pairs of the widest window may also be walked by the launch for
+            // wider pairs, beside later launches: room for their whole bound
             for (int32_t i : ids) {
                 const tsu_pair& pr = b->pairs[(size_t)i];
-                runs_need += 2 * (std::min(s_cap + 1, score_bound(ctx, pr.p_len, pr.t_len)) / unit) + 2;
+                const int64_t sb = score_bound(ctx, pr.p_len, pr.t_len);
+                runs_need += 2 * ((win >= 16 * 1024 ? sb : std::min(s_cap + 1, sb)) / unit) + 2;
             }
             tsu_host::order_longest_first(ids, b->pairs.data());
             plans.push_back(Plan{std::move(ids), sh, a, W, max_len, step_cap, win, recs, per_wg,
@@ -1116,12 +1214,7 @@ extern "C" int tsu_align_cigar(tsu_ctx* ctx, tsu_batch* b, int32_t* scores, tsu_
         if (!plans.empty()) {
             runs_cap = std::max<int64_t>(runs_need, 1);
             HIPCHK(d_runs.alloc((size_t)runs_cap), "hipMalloc (runs)");
-            for (Plan& pl : plans) {
-                int r = launch_bt(pl.sh, pl.a, pl.W, pl.ids, pl.max_len, pl.recs, pl.step_cap, pl.per_wg);
-                if (r != TSU_OK) return r;
-            }
-            HIPCHK(hipMemcpy(sc2.data(), b->d_scores, (size_t)b->n * sizeof(int32_t), hipMemcpyDeviceToHost), "hipMemcpy D2H");
-            for (const Plan& pl : plans)
+            auto settle_plan = [&](const Plan& pl) -> int {
                 for (int32_t i : pl.ids) {
                     const int32_t v = sc2[(size_t)i];
                     if (v >= 0) { scores[i] = v; done[(size_t)i] = 1; }
@@ -1130,48 +1223,66 @@ extern "C" int tsu_align_cigar(tsu_ctx* ctx, tsu_batch* b, int32_t* scores, tsu_
                     else if (v != TSU_EKOVERFLOW && v != TSU_ECIGAR)
                         return fail(ctx, v, "pair " + std::to_string(i) + ": " + tsu_strerror(v));
                 }
+                return TSU_OK;
+            };
+            // the widest windows first: their pairs that need more are then
+            // known, and run on the second stream beside the other launches
+            size_t k = 0;
+            for (; k < plans.size() && plans[k].win >= 16 * 1024; ++k) {
+                int r = launch_bt(plans[k].sh, plans[k].a, plans[k].W, plans[k].ids, plans[k].max_len, plans[k].recs,
+                                  plans[k].step_cap, plans[k].per_wg);
+                if (r != TSU_OK) return r;
+            }
+            if (k > 0) {
+                HIPCHK(hipMemcpy(sc2.data(), b->d_scores, (size_t)b->n * sizeof(int32_t), hipMemcpyDeviceToHost), "hipMemcpy D2H");
+                for (size_t j = 0; j < k; ++j) {
+                    int r = settle_plan(plans[j]);
+                    if (r != TSU_OK) return r;
+                }
+            }
+            if (!wide.empty() && k < plans.size()) {
+                WidePlan wp = plan_wide(wide);
+                if (wp.per_wg * (int64_t)wide.size() <= budget) {
+                    int r = launch_wide_async(wp, wide);
+                    if (r != TSU_OK) return r;
+                }
+            }
+            for (size_t j = k; j < plans.size(); ++j) {
+                int r = launch_bt(plans[j].sh, plans[j].a, plans[j].W, plans[j].ids, plans[j].max_len, plans[j].recs,
+                                  plans[j].step_cap, plans[j].per_wg);
+                if (r != TSU_OK) return r;
+            }
+            if (wide_async.launched) {
+                HIPCHK(hipStreamSynchronize(ctx->stream2), "hipStreamSynchronize");
+                float ms = 0.f;
+                HIPCHK(hipEventElapsedTime(&ms, wide_async.e0, wide_async.e1), "hipEventElapsedTime");
+                kms_total += ms;
+            }
+            HIPCHK(hipMemcpy(sc2.data(), b->d_scores, (size_t)b->n * sizeof(int32_t), hipMemcpyDeviceToHost), "hipMemcpy D2H");
+            for (size_t j = k; j < plans.size(); ++j) {
+                int r = settle_plan(plans[j]);
+                if (r != TSU_OK) return r;
+            }
+            if (wide_async.launched) {
+                int r = settle_wide(wide);
+                if (r != TSU_OK) return r;
+                wide.clear();
+            }
         }
     }
 
-    // 2) Pairs wider than every register window: one LDS-kernel launch over
-    // the whole diagonal range of their lengths, records sized for the
-    // highest score the lengths allow, when that fits the budget (a few long
-    // divergent pairs, each alone on a CU: a score pass first would double
-    // their time).
+    // 2) Pairs wider than every register window, when they did not run
+    // beside the other launches above.
     if (!wide.empty()) {
-        int32_t max_len = 0;
-        int64_t s_len = 0, runs_more = 0;
-        for (int32_t i : wide) {
-            const tsu_pair& pr = b->pairs[(size_t)i];
-            max_len = std::max(max_len, std::max(pr.p_len, pr.t_len));
-            const int64_t sb = score_bound(ctx, pr.p_len, pr.t_len);
-            s_len = std::max(s_len, sb);
-            runs_more += 2 * (sb / unit) + 2;
-        }
-        const int32_t W = capacity_full(max_len, max_len);
-        const int32_t step_cap = (int32_t)std::min<int64_t>(s_len + 2, 1 << 30);
-        int64_t recs = 64;
-        for (int64_t s = 0; s <= s_len; ++s)
-            recs += std::min<int64_t>(W, 2 * (std::max<int64_t>(0, s - ctx->os) / ctx->es + 2) + 1);
-        KArgs a = base_args(ctx);
-        a.W = W;
-        a.seq_words = seq_words_for(max_len);
-        Shape sh{default_waves(W), max_len <= 32000, true, true, true};
-        sh.fixed_242 = tsu_host::fixed_242(ctx);
-        if (tsu::lds_bytes(sh, a) > ctx->max_lds) { sh.lds_ring = false; sh.nw = 16; }
-        const int64_t per_wg = recs * 4 + (int64_t)step_cap * 12 + (sh.lds_ring ? 0 : tsu::ring_bytes(sh, W, ctx->rm, ctx->ri));
-        if (per_wg * (int64_t)wide.size() <= budget) {
-            int r = grow_runs(runs_more);
+        WidePlan wp = plan_wide(wide);
+        if (wp.per_wg * (int64_t)wide.size() <= budget) {
+            int r = grow_runs(wp.runs);
             if (r != TSU_OK) return r;
-            r = launch_bt(sh, a, W, wide, max_len, recs, step_cap, per_wg);
+            r = launch_bt(wp.sh, wp.a, wp.W, wide, wp.max_len, wp.recs, wp.step_cap, wp.per_wg);
             if (r != TSU_OK) return r;
             HIPCHK(hipMemcpy(sc2.data(), b->d_scores, (size_t)b->n * sizeof(int32_t), hipMemcpyDeviceToHost), "hipMemcpy D2H");
-            for (int32_t i : wide) {
-                const int32_t v = sc2[(size_t)i];
-                if (v >= 0) { scores[i] = v; done[(size_t)i] = 1; }
-                else if (v != TSU_EKOVERFLOW && v != TSU_ECIGAR)
-                    return fail(ctx, v, "pair " + std::to_string(i) + ": " + tsu_strerror(v));
-            }
+            r = settle_wide(wide);
+            if (r != TSU_OK) return r;
         }
     }
 

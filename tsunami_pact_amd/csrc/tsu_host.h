@@ -41,11 +41,12 @@ struct tsu_ctx {
     void* h_stage = nullptr;            // pinned staging for large device-to-host copies
     size_t h_stage_bytes = 0;
     // backtrace state of tsu_align_cigar (records, step bounds, step offsets,
-    // checkpoints), kept between calls and grown when a launch needs more:
-    // freeing tens of GB of written device memory can take a second.
-    // Released by tsu_set_cigar_memory and tsu_destroy.
-    void* bt_buf[4] = {nullptr, nullptr, nullptr, nullptr};
-    size_t bt_bytes[4] = {0, 0, 0, 0};
+    // checkpoints; then records, step bounds and step offsets of the launch
+    // on the second stream), kept between calls and grown when a launch
+    // needs more: freeing tens of GB of written device memory can take a
+    // second.  Released by tsu_set_cigar_memory and tsu_destroy.
+    void* bt_buf[7] = {};
+    size_t bt_bytes[7] = {};
 };
 
 struct tsu_batch {
