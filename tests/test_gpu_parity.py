@@ -425,22 +425,27 @@ def test_cigar_single_pass_falls_back(aligner, tsu, oracle):
     assert t["record_launches"] >= 2 and t["checkpoint_launches"] == 0
 
 
-def test_cigar_pairs_wider_than_register_windows(aligner, tsu, oracle):
+@pytest.mark.parametrize("beside", [False, True])
+def test_cigar_pairs_wider_than_register_windows(aligner, tsu, oracle, beside):
     """18-19 kbp pairs at 20-22% need about 20,500 diagonals, more than the
     16-chunk kernel's widest window: they overflow its single pass and are
     recorded in one LDS-kernel launch over their whole diagonal range, sized
     by their lengths (no score pass), beside 17-19 kbp pairs at 4% that the
-    single pass finishes.  Same scores and CIGARs as the oracle."""
+    single pass finishes.  With 3-4 kbp pairs too (beside=True) that launch
+    runs on the second stream while their single-pass launch runs.  Same
+    scores and CIGARs as the oracle."""
     pairs = (tsu.Dataset.generate(2, seed=SEED + 47, len_min=18000, len_max=20000, err_min=0.20, err_max=0.22).pairs() +
              tsu.Dataset.generate(2, seed=SEED + 53, len_min=17000, len_max=19000, err_min=0.04).pairs())
+    if beside:
+        pairs += tsu.Dataset.generate(40, seed=SEED + 59, len_min=3000, len_max=4000, err_min=0.02, err_max=0.08).pairs()
     want, ops, _ = oracle.wfa_batch(pairs, 4, 6, 2, cigar=True)
-    say("cigar wider than the register windows, 4 pairs")
+    say(f"cigar wider than the register windows, {len(pairs)} pairs")
     got, cig, t = aligner.align_cigar(aligner.batch(pairs))
-    assert (got == want).all(), (got, want)
+    assert (got == want).all(), np.nonzero(got != want)[0]
     for i, (p, t_) in enumerate(pairs):
         assert cig[i] == oracle.rle(ops[i]), i
         assert oracle.rescore(oracle.expand(cig[i]), p, t_, 4, 6, 2) == got[i]
-    assert t["record_launches"] >= 2 and t["checkpoint_launches"] == 0
+    assert t["record_launches"] >= (3 if beside else 2) and t["checkpoint_launches"] == 0
 
 
 
