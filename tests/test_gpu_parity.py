@@ -425,6 +425,25 @@ def test_cigar_single_pass_falls_back(aligner, tsu, oracle):
     assert t["record_launches"] >= 2 and t["checkpoint_launches"] == 0
 
 
+def test_cigars_staged_copy_several_slices(aligner, tsu, oracle):
+    """100,000 pairs of 1000 bp at 5%: the run pool (2 * (s / 2) + 2 words
+    per pair for scaled score s, about 50 MB) reaches the host in 32 MiB
+    staged slices, the next slice's copy overlapping the host copy of the
+    current one; 1,500 pairs spread over the batch equal the oracle's
+    scores and CIGARs."""
+    ds = tsu.Dataset.generate(100000, seed=SEED + 61, len_min=1000, err_min=0.05)
+    say("cigar 100000x1000, staged copy")
+    got, cig, _ = aligner.align_cigar(aligner.batch(ds))
+    pool = int((2 * (got.astype(np.int64) // 2 // 2) + 2).sum()) * 4
+    assert pool > 32 << 20, pool
+    idx = np.linspace(0, len(ds) - 1, 1500).astype(int)
+    pairs = [ds.pair(int(i)) for i in idx]
+    want, ops, _ = oracle.wfa_batch(pairs, 4, 6, 2, cigar=True)
+    assert (got[idx] == want).all()
+    for k, i in enumerate(idx):
+        assert cig[i] == oracle.rle(ops[k]), int(i)
+
+
 @pytest.mark.parametrize("beside", [False, True])
 def test_cigar_pairs_wider_than_register_windows(aligner, tsu, oracle, beside):
     """18-19 kbp pairs at 20-22% need about 20,500 diagonals, more than the

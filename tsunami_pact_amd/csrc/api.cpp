@@ -895,11 +895,13 @@ extern "C" int tsu_tune_table(const tsu_ctx* ctx, tsu_tune_entry* entries, int32
 // CIGAR form
 // ---------------------------------------------------------------------------
 // Device-to-host copy of a large buffer into pageable memory: slices go
-// through a pinned staging buffer kept by the context, and host threads move
-// each slice on (touching the destination's fresh pages in parallel).
+// through a pinned staging buffer of two halves kept by the context; while
+// host threads move one slice on (touching the destination's fresh pages in
+// parallel), the next slice's DMA fills the other half.
 static int copy_to_host(tsu_ctx* ctx, void* dst, const void* src, size_t bytes) {
-    constexpr size_t kSlice = 64u << 20;
-    const size_t want = std::min(bytes, kSlice);
+    constexpr size_t kSlice = 32u << 20;
+    if (bytes == 0) return TSU_OK;
+    const size_t want = 2 * kSlice;
     if (ctx->h_stage_bytes < want) {
         if (ctx->h_stage) (void)hipHostFree(ctx->h_stage);
         ctx->h_stage = nullptr;
@@ -907,14 +909,25 @@ static int copy_to_host(tsu_ctx* ctx, void* dst, const void* src, size_t bytes) 
         HIPCHK(hipHostMalloc(&ctx->h_stage, want, hipHostMallocDefault), "hipHostMalloc (staging)");
         ctx->h_stage_bytes = want;
     }
-    for (size_t off = 0; off < bytes; off += kSlice) {
+    uint8_t* half[2] = {static_cast<uint8_t*>(ctx->h_stage), static_cast<uint8_t*>(ctx->h_stage) + kSlice};
+    const uint8_t* from = static_cast<const uint8_t*>(src);
+    const size_t slices = (bytes + kSlice - 1) / kSlice;
+    HIPCHK(hipMemcpyAsync(half[0], from, std::min(kSlice, bytes), hipMemcpyDeviceToHost, ctx->stream), "hipMemcpyAsync D2H");
+    for (size_t k = 0; k < slices; ++k) {
+        HIPCHK(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");  // slice k is in half[k & 1]
+        if (k + 1 < slices) {
+            const size_t off1 = (k + 1) * kSlice;
+            HIPCHK(hipMemcpyAsync(half[(k + 1) & 1], from + off1, std::min(kSlice, bytes - off1), hipMemcpyDeviceToHost,
+                                  ctx->stream), "hipMemcpyAsync D2H");
+        }
+        const size_t off = k * kSlice;
         const size_t n = std::min(kSlice, bytes - off);
-        HIPCHK(hipMemcpy(ctx->h_stage, static_cast<const uint8_t*>(src) + off, n, hipMemcpyDeviceToHost), "hipMemcpy D2H");
+        const uint8_t* st = half[k & 1];
         const size_t part = (n + 15) / 16;
 #pragma omp parallel for schedule(static)
         for (int64_t i = 0; i < 16; ++i) {
             const size_t b0 = std::min(n, (size_t)i * part), b1 = std::min(n, b0 + part);
-            if (b1 > b0) std::memcpy(static_cast<uint8_t*>(dst) + off + b0, static_cast<const uint8_t*>(ctx->h_stage) + b0, b1 - b0);
+            if (b1 > b0) std::memcpy(static_cast<uint8_t*>(dst) + off + b0, st + b0, b1 - b0);
         }
     }
     return TSU_OK;
