@@ -23,8 +23,9 @@ The CPU baseline is the oracle's exact WFA (C, OpenMP over pairs, every host
 thread), timed on the first pairs of the same workload, and its scores double
 as a bit-exactness check of the timed GPU run.
 
-The batched form runs the autotuner on a sample first (untimed); with
---compare-plain it also times the plain form on the same pairs.
+The batched and CIGAR forms run the autotuner on a sample first (untimed);
+with --compare-plain the batched form also times the plain form on the same
+pairs.
 
     python bench.py [--steps K] [--warmup W] [--form plain|batched|cigar|served|refgrid|twopiece]
 """
@@ -351,7 +352,8 @@ def main():
         mode = {"auto": a.CIGAR_AUTO, "records": a.CIGAR_RECORDS, "checkpoints": a.CIGAR_CHECKPOINTS}[args.cigar_memory]
         a.set_cigar_memory(mode, args.checkpoint_every)
     tune = None
-    # the CIGAR form's score pass is the batched form: it uses the same table
+    # the CIGAR form sizes its record launches from the batched form's per-bucket
+    # capacities (and its score-first pairs run the batched form): same table
     if args.form in ("batched", "cigar") and not args.no_autotune:
         t0 = time.perf_counter()
         tune = a.autotune(b, args.tune_sample)
