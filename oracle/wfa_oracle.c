@@ -17,7 +17,7 @@
  *                    the neighbour-synchronised GPU kernel, which has no
  *                    whole-window view per score.  Tests pin its score and
  *                    CIGAR to orc_wfa's (and the score to Gotoh's).
- *   orc_wfa_batch()  OpenMP driver over many pairs (CPU baseline; mirrors the
+ *   orc_wfa_batch()  OpenMP loop over many pairs (CPU baseline; mirrors the
  *                    reference's check loop, wfa_gpu.cu:150-166, but reuses one
  *                    scratch state per thread).
  *
