@@ -249,6 +249,29 @@ def test_wfa_gpu_cli_check_mode(repo, tsu, tmp_path):
     assert "|ALIGNEMENT FINISHED CORRECTLY|" in r.stdout
 
 
+def test_wfa_gpu_cli_cigar_mode(repo, tsu, oracle, tmp_path):
+    """bin/wfa_gpu with TSUNAMI_MODE=cigar and TSUNAMI_CIGARS writes one
+    score<TAB>CIGAR line per pair of sequences.txt (300 pairs of 100-3000 bp
+    at 1-12%), equal to the oracle's on the pairs as read back from the
+    file; the reference's stdout lines stay the same."""
+    ds = tsu.Dataset.generate(300, seed=SEED + 67, len_min=100, len_max=3000, err_min=0.01, err_max=0.12)
+    ds.write(str(tmp_path / "sequences.txt"), 2)
+    env = dict(os.environ, TSUNAMI_MODE="cigar", TSUNAMI_CIGARS=str(tmp_path / "cigars.tsv"))
+    say("cli cigar mode 300 pairs")
+    r = subprocess.run([os.path.join(repo, "bin", "wfa_gpu"), "4", "6", "2", "sequences.txt", "42", "0"],
+                       cwd=tmp_path, env=env, capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert r.stdout.splitlines()[0].startswith("GPU Time: ")
+    pairs = tsu.Dataset.read(str(tmp_path / "sequences.txt")).pairs()
+    want, ops, _ = oracle.wfa_batch(pairs, 4, 6, 2, cigar=True)
+    lines = (tmp_path / "cigars.tsv").read_text().splitlines()
+    assert len(lines) == len(pairs)
+    for i, line in enumerate(lines):
+        sc, cig = line.split("\t")
+        assert int(sc) == want[i], i
+        assert cig == oracle.rle(ops[i]), i
+
+
 def test_register_and_lds_kernels_agree(aligner, tsu, oracle, fixtures):
     """4/6/2 pairs run through the single-wave register kernel (capacity
     <= 512) and the multi-wave one (capacity > 512) give the oracle's scores;
