@@ -105,13 +105,13 @@ int64_t tsu_packed_words(int32_t len);
 /* Packs len bases of seq into out[0, tsu_packed_words(len)).  Returns
  * TSU_EBADINPUT on a character outside {A,C,G,T}.  The reference compares
  * raw bytes (wfa_gpu.cu:735,276); its generator emits only ACGT
- * (generate_dataset.cpp:114,139). */
+ * (generate_dataset.cpp:25,50,63-65). */
 int tsu_pack(const char* seq, int32_t len, uint32_t* out);
 
 /* ---- host datasets: sequences.txt and the seeded generator --------------- */
 
 /* Reads a sequences.txt file: header "num len err" then per pair a pattern
- * token and a text token (wfa_gpu.cu:71-81, generate_dataset.cpp:128-158).
+ * token and a text token (wfa_gpu.cu:71-81, generate_dataset.cpp:39-41,68-69).
  * A token longer than the header length keeps its first len characters, as
  * the reference's fixed slots do; shorter tokens are kept whole (variable-
  * length files written by tsu_dataset_write).  err (may be NULL) receives a
@@ -126,8 +126,11 @@ int tsu_dataset_write(const tsu_dataset* ds, const char* path, int32_t error_rat
  * rate is err_min, or U(err_min, err_max); edits = round(rate * len) at
  * distinct pattern positions; each edit is a substitution (to a different
  * base), an insertion (uniform base) or a deletion with probability 1/3
- * (edit_mode 0), or a substitution only (edit_mode 1).  Pair i depends only
- * on (seed, i), so the result is independent of the thread count. */
+ * (edit_mode 0), or a substitution only (edit_mode 1).  edit_mode 2 is the
+ * reference generator's model: substitutions only, len * percent / 100
+ * rounded down of them, with percent = rate * 100 rounded to a whole number
+ * (generate_dataset.cpp:20-23,56-66).  Pair i depends only on (seed, i), so
+ * the result is independent of the thread count. */
 typedef struct {
     uint64_t seed;
     int32_t len_min;
@@ -176,8 +179,13 @@ void tsu_batch_free(tsu_batch* b);
  * reference's argv[5] (wfa_gpu.cu:63,217).  Replaces the launch of
  * wfa_dispatcher<<<N,32>>> and the score copy (wfa_gpu.cu:113-128,191-194).
  * scores[i] receives pair i's score; a pair whose diagonal range exceeds the
- * capacity gets TSU_EKOVERFLOW and the call returns TSU_EKOVERFLOW.  The
- * capacity used is at least wf_length (it may be rounded up to 64 diagonals). */
+ * capacity gets TSU_EKOVERFLOW and the call returns TSU_EKOVERFLOW.
+ * wf_length is the least capacity used: with penalties that divide to
+ * 2/3/1 (4/6/2 and its multiples) the register kernels round it up to their
+ * window, 64 * 2^j diagonals (wf_length 400 runs in 512), so a pair whose
+ * range is wider than wf_length but fits that window succeeds; with other
+ * penalties the capacity is wf_length itself.  The reference writes past its
+ * ring instead when a range exceeds wf_length (wfa_gpu.cu:217,618-626). */
 int tsu_align_plain(tsu_ctx* ctx, tsu_batch* b, int32_t wf_length, int32_t* scores, tsu_timing* t);
 
 /* Reference-shaped convenience: num_couples pairs of seq_len bases in the
@@ -253,6 +261,16 @@ int tsu_set_poll_limit(tsu_ctx* ctx, int32_t polls);
  * (device bytes) and the launches of each kind.  Any pointer may be NULL. */
 int tsu_cigars_stats(const tsu_cigars* c, int64_t* state_bytes, int32_t* record_launches,
                      int32_t* checkpoint_launches);
+/* Which kernels recorded the backtraces of the call that made c, as a mask
+ * of TSU_KERNEL_* bits.  No reference counterpart (the reference computes no
+ * CIGAR on the GPU); for callers and tests that check which path ran. */
+#define TSU_KERNEL_REG1    0x01u  /* one wave per pair, wavefronts in registers */
+#define TSU_KERNEL_ML4     0x02u  /* 16 chunks per wave, 4 waves per pair */
+#define TSU_KERNEL_ML8     0x04u  /* 16 chunks per wave, 8 waves per pair */
+#define TSU_KERNEL_ML16    0x08u  /* 16 chunks per wave, 16 waves per pair */
+#define TSU_KERNEL_LDS     0x10u  /* rings in LDS (or HBM), records for every score */
+#define TSU_KERNEL_LDS_CK  0x20u  /* rings in LDS (or HBM), ring checkpoints */
+int tsu_cigars_kernels(const tsu_cigars* c, uint32_t* kernels);
 
 /* ---- serving (no reference counterpart) ---------------------------------- */
 

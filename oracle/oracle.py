@@ -20,7 +20,8 @@ REF_GENERATOR = os.path.join(HERE, "_ref", "generate_dataset")
 
 class Counters(ctypes.Structure):
     _fields_ = [("cells", ctypes.c_int64), ("extend_words", ctypes.c_int64),
-                ("steps", ctypes.c_int64), ("max_width", ctypes.c_int64)]
+                ("steps", ctypes.c_int64), ("max_width", ctypes.c_int64),
+                ("min_lo", ctypes.c_int64), ("max_hi", ctypes.c_int64)]
 
     def as_dict(self) -> dict:
         return {f: int(getattr(self, f)) for f, _ in self._fields_}
@@ -58,6 +59,10 @@ def lib() -> ctypes.CDLL:
                                     ctypes.c_int, P(ctypes.c_int32), ctypes.c_int, ctypes.c_char_p,
                                     P(ctypes.c_int64), P(ctypes.c_int32), P(Counters)]
         l.orc_max_threads.restype = ctypes.c_int
+        l.orc_wfa_prio.restype = ctypes.c_int
+        l.orc_wfa_prio.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_char_p, ctypes.c_int,
+                                   ctypes.c_int, ctypes.c_int, ctypes.c_int, P(ctypes.c_int),
+                                   ctypes.c_char_p, ctypes.c_int, P(ctypes.c_int)]
         l.orc_gotoh2p.restype = ctypes.c_int64
         l.orc_gotoh2p.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_char_p, ctypes.c_int] + [ctypes.c_int] * 5
         l.orc_wfa2p.restype = ctypes.c_int
@@ -101,6 +106,24 @@ def wfa(p: str, t: str, x: int = 4, o: int = 6, e: int = 2, cigar: bool = False)
         raise RuntimeError(f"orc_wfa failed: {r}")
     ops = buf.raw[:n.value].decode() if cigar else None
     return r, ops, c.as_dict()
+
+
+# Backtrace priorities of WFA2-lib (common.h:67-77), in the order the oracle
+# takes them: M (mismatch), D_ext, D_open, I_ext, I_open.
+PRIORITY = (9, 6, 5, 2, 1)
+PRIORITY_KINDS = ("M", "D_ext", "D_open", "I_ext", "I_open")
+
+
+def wfa_prio(p: str, t: str, x: int, o: int, e: int, prio: Sequence[int]) -> Tuple[int, str]:
+    """(score, ops) with the backtrace priorities `prio` (order of PRIORITY_KINDS)."""
+    cap = len(p) + len(t) + 1
+    buf = ctypes.create_string_buffer(cap)
+    n = ctypes.c_int(0)
+    pr = (ctypes.c_int * 5)(*prio)
+    r = lib().orc_wfa_prio(p.encode(), len(p), t.encode(), len(t), x, o, e, pr, buf, cap, ctypes.byref(n))
+    if r < 0:
+        raise RuntimeError(f"orc_wfa_prio failed: {r}")
+    return r, buf.raw[:n.value].decode()
 
 
 def gotoh2p(p: str, t: str, x: int, o1: int, e1: int, o2: int, e2: int) -> int:

@@ -17,6 +17,8 @@
  *                    the neighbour-synchronised GPU kernel, which has no
  *                    whole-window view per score.  Tests pin its score and
  *                    CIGAR to orc_wfa's (and the score to Gotoh's).
+ *   orc_wfa_prio()   orc_wfa with caller-given backtrace priorities: lets the
+ *                    tests show that the priority order is pinned.
  *   orc_wfa_batch()  OpenMP loop over many pairs (CPU baseline; mirrors the
  *                    reference's check loop, wfa_gpu.cu:150-166, but reuses one
  *                    scratch state per thread).
@@ -53,12 +55,17 @@
  * down to that max, then emit the operation of the winning source and move
  * to it; finish with leading matches / deletions / insertions.
  *
- * Parity pins (see tests/test_oracle.py): Gotoh == brute-force enumeration
- * on tiny inputs; closed forms; WFA score == Gotoh on every fixture; CIGARs
- * re-score to the score and consume both sequences; the WFA2-lib README's
- * published example (score 24, "3M1X4M1D7M1I9M1X6M").  The choice among
- * co-optimal CIGARs is pinned only by that example: parity with WFA2-lib's
- * CIGARs is otherwise unpinned (WFA2-lib is not available offline).
+ * Parity pins (tests/test_oracle.py, tests/test_oracle_ties.py): Gotoh ==
+ * brute-force enumeration on tiny inputs; closed forms; WFA score == Gotoh
+ * on every fixture; CIGARs re-score to the score and consume both
+ * sequences; the WFA2-lib README's published example (score 24,
+ * "3M1X4M1D7M1I9M1X6M"); the choice among co-optimal CIGARs by twenty pairs
+ * whose CIGARs were derived by hand from the published backtrace, each
+ * decided by one comparison of the priority order, so that every other
+ * order of the five source types (orc_wfa_prio) changes at least one of
+ * them, and by gap placement in homopolymer runs.  What stays unpinned:
+ * agreement with a WFA2-lib build on details its published algorithm does
+ * not fix (WFA2-lib is not available offline).
  */
 #include <stdint.h>
 #include <stdlib.h>
@@ -74,11 +81,19 @@
 
 enum { ORC_OK = 0, ORC_EINVAL = -1, ORC_ENOMEM = -4, ORC_EINTERNAL = -9, ORC_ECIGAR = -10 };
 
+/* Backtrace priorities of the five source kinds, in the order
+ * { M (mismatch), D_ext, D_open, I_ext, I_open }: the values of WFA2-lib's
+ * backtrace_type that the reference restates in common.h:67-77
+ * (bt_M 9, bt_D1_ext 6, bt_D1_open 5, bt_I1_ext 2, bt_I1_open 1). */
+static const int ORC_PRIO[5] = { 9, 6, 5, 2, 1 };
+
 typedef struct {
     int64_t cells;        /* sum over computed (non-null) scores of (hi-lo+1) */
     int64_t extend_words; /* 32-base window compares in the extend */
     int64_t steps;        /* score steps executed (incl. null steps) */
     int64_t max_width;    /* widest computed range (hi-lo+1) */
+    int64_t min_lo;       /* lowest diagonal of any computed range (before trims) */
+    int64_t max_hi;       /* highest diagonal of any computed range (before trims) */
 } orc_counters;
 
 /* ------------------------------------------------------------------------ */
@@ -235,10 +250,11 @@ static void orc_reverse(char* a, int n)
  * Returns number of ops written to ops (forward order) or ORC_ECIGAR.
  */
 static int orc_backtrace(const orc_state* st, const char* p, int plen, const char* t, int tlen,
-                         int x, int o, int e, int score_end, char* ops, int ops_cap)
+                         int x, int o, int e, int score_end, char* ops, int ops_cap, const int* prio)
 {
     enum { T_M = 0, T_I = 1, T_D = 2 };
-    enum { BT_M = 9, BT_D_EXT = 6, BT_D_OPEN = 5, BT_I_EXT = 2, BT_I_OPEN = 1 };
+    /* priority of each source kind in the piggy-backed max (common.h:67-77) */
+    const int BT_M = prio[0], BT_D_EXT = prio[1], BT_D_OPEN = prio[2], BT_I_EXT = prio[3], BT_I_OPEN = prio[4];
     const int64_t PNULL = ((int64_t)ORC_NULL) * 16;
     int n = 0;
     int k = tlen - plen;
@@ -281,14 +297,12 @@ static int orc_backtrace(const orc_state* st, const char* p, int plen, const cha
             v = offset - k; h = offset;
             if (v <= 0 || h <= 0) break;
         }
-        switch (bt) {
-        case BT_M: score = s_mis; type = T_M; EMIT('X'); --offset; break;
-        case BT_I_OPEN: score = s_open; type = T_M; EMIT('I'); --k; --offset; break;
-        case BT_I_EXT: score = s_ext; type = T_I; EMIT('I'); --k; --offset; break;
-        case BT_D_OPEN: score = s_open; type = T_M; EMIT('D'); ++k; break;
-        case BT_D_EXT: score = s_ext; type = T_D; EMIT('D'); ++k; break;
-        default: return ORC_EINTERNAL;
-        }
+        if (bt == BT_M) { score = s_mis; type = T_M; EMIT('X'); --offset; }
+        else if (bt == BT_I_OPEN) { score = s_open; type = T_M; EMIT('I'); --k; --offset; }
+        else if (bt == BT_I_EXT) { score = s_ext; type = T_I; EMIT('I'); --k; --offset; }
+        else if (bt == BT_D_OPEN) { score = s_open; type = T_M; EMIT('D'); ++k; }
+        else if (bt == BT_D_EXT) { score = s_ext; type = T_D; EMIT('D'); ++k; }
+        else return ORC_EINTERNAL;
         v = offset - k; h = offset;
     }
     if (type == T_M) {
@@ -311,7 +325,7 @@ static int orc_backtrace(const orc_state* st, const char* p, int plen, const cha
 
 static int orc_wfa_impl(orc_state* st, const char* p, int plen, const char* t, int tlen,
                         int x, int o, int e, int want_cigar, char* ops, int ops_cap, int* ops_len,
-                        orc_counters* cnt, int null_oob)
+                        orc_counters* cnt, int null_oob, const int* prio)
 {
     if (plen < 0 || tlen < 0 || x <= 0 || o < 0 || e <= 0) return ORC_EINVAL;
     const int ring = ORC_MAX(x, o + e) + 1;          /* wfa_gpu.cu:83-84 */
@@ -378,6 +392,8 @@ static int orc_wfa_impl(orc_state* st, const char* p, int plen, const char* t, i
         if (cnt) {
             cnt->cells += hi - lo + 1;
             if (hi - lo + 1 > cnt->max_width) cnt->max_width = hi - lo + 1;
+            if (lo < cnt->min_lo) cnt->min_lo = lo;
+            if (hi > cnt->max_hi) cnt->max_hi = hi;
         }
         /* COMPUTE-NEXT (wfa_gpu.cu:610-631) */
         for (int k = lo; k <= hi; ++k) {
@@ -402,7 +418,7 @@ static int orc_wfa_impl(orc_state* st, const char* p, int plen, const char* t, i
         }
     }
     if (want_cigar) {
-        int n = orc_backtrace(st, p, plen, t, tlen, x, o, e, score, ops, ops_cap);
+        int n = orc_backtrace(st, p, plen, t, tlen, x, o, e, score, ops, ops_cap, prio ? prio : ORC_PRIO);
         if (n < 0) return n;
         if (ops_len) *ops_len = n;
     }
@@ -418,7 +434,7 @@ int orc_wfa(const char* p, int plen, const char* t, int tlen, int x, int o, int 
             int want_cigar, char* ops, int ops_cap, int* ops_len, orc_counters* cnt)
 {
     orc_state st = { NULL, NULL, NULL, 0 };
-    int r = orc_wfa_impl(&st, p, plen, t, tlen, x, o, e, want_cigar, ops, ops_cap, ops_len, cnt, 0);
+    int r = orc_wfa_impl(&st, p, plen, t, tlen, x, o, e, want_cigar, ops, ops_cap, ops_len, cnt, 0, NULL);
     state_destroy(&st);
     return r;
 }
@@ -432,7 +448,27 @@ int orc_wfa_nulled(const char* p, int plen, const char* t, int tlen, int x, int 
                    int want_cigar, char* ops, int ops_cap, int* ops_len)
 {
     orc_state st = { NULL, NULL, NULL, 0 };
-    int r = orc_wfa_impl(&st, p, plen, t, tlen, x, o, e, want_cigar, ops, ops_cap, ops_len, NULL, 1);
+    int r = orc_wfa_impl(&st, p, plen, t, tlen, x, o, e, want_cigar, ops, ops_cap, ops_len, NULL, 1, NULL);
+    state_destroy(&st);
+    return r;
+}
+
+/*
+ * One pair with the backtrace priorities given by the caller (prio[5] in the
+ * order M, D_ext, D_open, I_ext, I_open; five distinct positive values below
+ * 16).  Used only by the tests that check that every other priority order
+ * changes some hand-derived CIGAR.
+ */
+int orc_wfa_prio(const char* p, int plen, const char* t, int tlen, int x, int o, int e,
+                 const int* prio, char* ops, int ops_cap, int* ops_len)
+{
+    if (!prio) return ORC_EINVAL;
+    for (int i = 0; i < 5; ++i) {
+        if (prio[i] <= 0 || prio[i] >= 16) return ORC_EINVAL;
+        for (int j = 0; j < i; ++j) if (prio[i] == prio[j]) return ORC_EINVAL;
+    }
+    orc_state st = { NULL, NULL, NULL, 0 };
+    int r = orc_wfa_impl(&st, p, plen, t, tlen, x, o, e, 1, ops, ops_cap, ops_len, NULL, 0, prio);
     state_destroy(&st);
     return r;
 }
@@ -452,22 +488,22 @@ int orc_wfa_batch(const char* seqs, const int64_t* p_off, const int32_t* p_len,
                   orc_counters* total)
 {
     int failed = 0;
-    int64_t cells = 0, words = 0, steps = 0, maxw = 0;
+    int64_t cells = 0, words = 0, steps = 0, maxw = 0, minlo = 0, maxhi = 0;
 #ifdef _OPENMP
     if (threads > 0) omp_set_num_threads(threads);
 #else
     (void)threads;
 #endif
-#pragma omp parallel reduction(+:failed, cells, words, steps) reduction(max:maxw)
+#pragma omp parallel reduction(+:failed, cells, words, steps) reduction(max:maxw, maxhi) reduction(min:minlo)
     {
         orc_state st = { NULL, NULL, NULL, 0 };
-        orc_counters c = { 0, 0, 0, 0 };
+        orc_counters c = { 0, 0, 0, 0, 0, 0 };
 #pragma omp for schedule(dynamic, 16)
         for (int i = 0; i < n; ++i) {
             int olen = 0;
             int r = orc_wfa_impl(&st, seqs + p_off[i], p_len[i], seqs + t_off[i], t_len[i], x, o, e,
                                  want_cigar, want_cigar ? ops + ops_off[i] : NULL,
-                                 p_len[i] + t_len[i], &olen, total ? &c : NULL, 0);
+                                 p_len[i] + t_len[i], &olen, total ? &c : NULL, 0, NULL);
             scores[i] = r;
             if (want_cigar && ops_len) ops_len[i] = olen;
             if (r < 0) failed++;
@@ -477,8 +513,13 @@ int orc_wfa_batch(const char* seqs, const int64_t* p_off, const int32_t* p_len,
         state_destroy(&st);
         cells += c.cells; words += c.extend_words; steps += c.steps;
         if (c.max_width > maxw) maxw = c.max_width;
+        if (c.min_lo < minlo) minlo = c.min_lo;
+        if (c.max_hi > maxhi) maxhi = c.max_hi;
     }
-    if (total) { total->cells = cells; total->extend_words = words; total->steps = steps; total->max_width = maxw; }
+    if (total) {
+        total->cells = cells; total->extend_words = words; total->steps = steps; total->max_width = maxw;
+        total->min_lo = minlo; total->max_hi = maxhi;
+    }
     return failed;
 }
 

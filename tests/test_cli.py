@@ -52,6 +52,10 @@ def test_generate_dataset_usage(bins):
 
 
 def test_generate_dataset_seeded(bins, tmp_path, tsu):
+    """With the reference's three arguments (plus a seed) the file holds the
+    reference generator's model: equal lengths, substitutions only,
+    len * err / 100 rounded down of them per pair (generate_dataset.cpp:
+    20-23,45-66); "mixed" switches to sub/ins/del edits."""
     a = tmp_path / "a"; b = tmp_path / "b"
     a.mkdir(); b.mkdir()
     for d in (a, b):
@@ -60,12 +64,26 @@ def test_generate_dataset_seeded(bins, tmp_path, tsu):
     ta = (a / "sequences.txt").read_text()
     assert ta == (b / "sequences.txt").read_text()
     lines = ta.split("\n")
-    assert lines[0] == "100" and lines[2] == "2"
+    assert lines[0] == "100" and lines[1] == "128" and lines[2] == "2"
     ds = tsu.Dataset.read(str(a / "sequences.txt"))
     assert len(ds) == 100
-    ref = tsu.Dataset.generate(100, seed=42, len_min=128, err_min=0.02)
-    assert ds.pairs() == ref.pairs()
+    assert ds.pairs() == tsu.Dataset.generate(100, seed=42, len_min=128, err_min=0.02, edit_mode=2).pairs()
+    for p, t in ds.pairs():
+        assert len(p) == len(t) == 128
+        assert sum(x != y for x, y in zip(p, t)) == 2   # 128 * 2 // 100
+    # the default seed and mode, exactly the reference's argument list
+    r = run([bins[1], "10", "64", "5"], cwd=b)
+    assert r.returncode == 0
+    for p, t in tsu.Dataset.read(str(b / "sequences.txt")).pairs():
+        assert len(p) == len(t) == 64
+        assert sum(x != y for x, y in zip(p, t)) == 3   # 64 * 5 // 100
     r = run([bins[1], "10", "64", "5", "7", "sub"], cwd=a)
     assert r.returncode == 0
-    for p, t in tsu.Dataset.read(str(a / "sequences.txt")).pairs():
-        assert len(p) == len(t) == 64
+    assert tsu.Dataset.read(str(a / "sequences.txt")).pairs() == \
+        tsu.Dataset.generate(10, seed=7, len_min=64, err_min=0.05, edit_mode=2).pairs()
+    r = run([bins[1], "100", "128", "2", "42", "mixed"], cwd=a)
+    assert r.returncode == 0
+    assert tsu.Dataset.read(str(a / "sequences.txt")).pairs() == \
+        tsu.Dataset.generate(100, seed=42, len_min=128, err_min=0.02).pairs()
+    r = run([bins[1], "10", "64", "5", "7", "indels"], cwd=a)
+    assert r.returncode == 1

@@ -56,6 +56,14 @@ def test_generator_edit_model(tsu):
     for p, t in s.pairs():
         assert len(p) == len(t) == 200
         assert sum(a != b for a, b in zip(p, t)) == 10
+    # the reference's count, len * percent / 100 rounded down: 10240 at 2% is
+    # 204 substitutions where rounding would give 205 (generate_dataset.cpp:23)
+    r = tsu.Dataset.generate(20, seed=3, len_min=10240, err_min=0.02, edit_mode=2)
+    for p, t in r.pairs():
+        assert len(p) == len(t) == 10240
+        assert sum(a != b for a, b in zip(p, t)) == 204
+    assert sum(a != b for a, b in zip(*tsu.Dataset.generate(1, seed=3, len_min=10240, err_min=0.02,
+                                                             edit_mode=1).pair(0))) == 205
 
 
 def test_generator_length_and_rate_ranges(tsu):

@@ -36,6 +36,9 @@ TSU_ECIGAR = -10
 
 OPS = "MXID"
 
+# TSU_KERNEL_* bits (tsu_cigars_kernels): which kernels recorded backtraces
+KERNEL_BITS = {0x01: "reg1", 0x02: "ml4", 0x04: "ml8", 0x08: "ml16", 0x10: "lds", 0x20: "lds_ck"}
+
 # Every symbol include/tsunami_wfa.h declares.
 EXPORTS = (
     "tsu_create", "tsu_create_2p", "tsu_destroy", "tsu_last_error", "tsu_strerror", "tsu_device_info",
@@ -45,7 +48,7 @@ EXPORTS = (
     "tsu_batch_from_dataset", "tsu_batch_create", "tsu_batch_size", "tsu_batch_bytes", "tsu_batch_free",
     "tsu_align_plain", "tsu_align_fixed", "tsu_align_batched", "tsu_autotune", "tsu_tune_table",
     "tsu_align_cigar", "tsu_cigars_get", "tsu_cigars_string", "tsu_cigars_free",
-    "tsu_set_cigar_memory", "tsu_cigars_stats", "tsu_set_poll_limit",
+    "tsu_set_cigar_memory", "tsu_cigars_stats", "tsu_cigars_kernels", "tsu_set_poll_limit",
     "tsu_serve_open", "tsu_serve_submit", "tsu_serve_submit_packed", "tsu_serve_poll", "tsu_serve_wait",
     "tsu_serve_close",
     "tsu_calibrate_copy", "tsu_sync",
@@ -146,6 +149,7 @@ def library() -> ctypes.CDLL:
         "tsu_cigars_free": (None, [vp]),
         "tsu_set_cigar_memory": (ctypes.c_int, [vp, i32, i32, i64]),
         "tsu_cigars_stats": (ctypes.c_int, [vp, P(i64), P(i32), P(i32)]),
+        "tsu_cigars_kernels": (ctypes.c_int, [vp, P(ctypes.c_uint32)]),
         "tsu_set_poll_limit": (ctypes.c_int, [vp, i32]),
         "tsu_serve_open": (ctypes.c_int, [vp, P(ServeCfg), P(vp)]),
         "tsu_serve_submit": (ctypes.c_int, [vp, P(ctypes.c_char_p), P(i32), P(ctypes.c_char_p), P(i32), i32, P(ctypes.c_uint64)]),
@@ -425,7 +429,8 @@ class Aligner:
         computed and copied to the host by the library as usual but not
         turned into Python strings (None is returned in their place).  The
         timing dict also holds the backtrace state: state_bytes (largest of
-        one launch), record_launches and checkpoint_launches."""
+        one launch), record_launches, checkpoint_launches and kernels (the
+        set of KERNEL_BITS names of the kernels that recorded)."""
         n = len(batch)
         scores = np.zeros(max(n, 1), np.int32)
         t = Timing()
@@ -442,10 +447,13 @@ class Aligner:
             sb, rl, cl = ctypes.c_int64(), ctypes.c_int32(), ctypes.c_int32()
             self._check(self._lib.tsu_cigars_stats(h, ctypes.byref(sb), ctypes.byref(rl), ctypes.byref(cl)),
                         "tsu_cigars_stats")
+            km = ctypes.c_uint32()
+            self._check(self._lib.tsu_cigars_kernels(h, ctypes.byref(km)), "tsu_cigars_kernels")
         finally:
             self._lib.tsu_cigars_free(h)
         td = t.as_dict()
-        td.update(state_bytes=sb.value, record_launches=rl.value, checkpoint_launches=cl.value)
+        td.update(state_bytes=sb.value, record_launches=rl.value, checkpoint_launches=cl.value,
+                  kernels={name for bit, name in KERNEL_BITS.items() if km.value & bit})
         return scores[:n], cigars, td
 
     def calibrate_copy(self, nbytes: int) -> float:

@@ -1,15 +1,19 @@
 // generate_dataset -- writes ./sequences.txt like the reference's generator
-// (generate_dataset.cpp:11-75), but seeded and with indels.
+// (generate_dataset.cpp:11-75), but seeded.
 //
 //   generate_dataset num_couples sequence_length error_rate [seed] [mode]
 //
-// error_rate is a percentage as in the reference.  The edit count per pair is
-// round(error_rate/100 * length).  mode "mixed" (default): each edit is a
-// substitution, insertion or deletion with probability 1/3; mode "sub":
-// substitutions only, pairs of equal length as the reference writes them.
-// Seed defaults to 0x5453554E.  Set TSUNAMI_LEN_MAX / TSUNAMI_ERR_MAX to draw
-// lengths log-uniformly in [length, LEN_MAX] and rates uniformly in
-// [error_rate, ERR_MAX] percent.
+// With the reference's three arguments the pairs follow its model: equal
+// lengths, substitutions only, length * error_rate / 100 (rounded down) of
+// them per pair at distinct positions, each to a different base
+// (generate_dataset.cpp:20-23,45-66), so the reference's run scripts and
+// their wf_length values apply unchanged.  mode "mixed" makes each edit a
+// substitution, an insertion or a deletion with probability 1/3 and takes
+// round(error_rate / 100 * length) edits (the build's benchmark workloads);
+// mode "sub" is the default.  error_rate is a percentage as in the
+// reference.  Seed defaults to 0x5453554E.  Set TSUNAMI_LEN_MAX /
+// TSUNAMI_ERR_MAX to draw lengths log-uniformly in [length, LEN_MAX] and
+// rates uniformly in [error_rate, ERR_MAX] percent.
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -33,7 +37,16 @@ int main(int argc, char* argv[]) {
     cfg.len_max = len;
     cfg.err_min = rate / 100.0;
     cfg.err_max = rate / 100.0;
-    cfg.edit_mode = (argc > 5 && std::string(argv[5]) == "sub") ? 1 : 0;
+    cfg.edit_mode = 2;
+    if (argc > 5) {
+        const std::string mode = argv[5];
+        if (mode == "mixed") {
+            cfg.edit_mode = 0;
+        } else if (mode != "sub") {
+            std::cout << "generate_dataset: mode must be \"sub\" or \"mixed\"" << std::endl;
+            return 1;
+        }
+    }
     if (const char* v = std::getenv("TSUNAMI_LEN_MAX")) cfg.len_max = std::atoi(v);
     if (const char* v = std::getenv("TSUNAMI_ERR_MAX")) cfg.err_max = std::atof(v) / 100.0;
     tsu_dataset* ds = nullptr;

@@ -46,7 +46,7 @@ double now_ms() {
 }
 
 int hip_fail(tsu_ctx* ctx, hipError_t e, const char* what) {
-    return fail(ctx, TSU_EHIP, std::string(what) + ": " + hipGetErrorString(e));
+    return fail(ctx, e == hipErrorOutOfMemory ? TSU_ENOMEM : TSU_EHIP, std::string(what) + ": " + hipGetErrorString(e));
 }
 
 #define HIPCHK(call, what)                                   \
@@ -497,14 +497,17 @@ extern "C" int tsu_align_plain(tsu_ctx* ctx, tsu_batch* b, int32_t wf_length, in
         r = run_shape(ctx, b, sh, W, nullptr, b->n, b->max_len, 0, nullptr, &kms);
     }
     if (r != TSU_OK) return r;
-    const double w1 = now_ms();
+    double wall = now_ms() - w0;
+    double d0 = now_ms();
     HIPCHK(hipMemcpy(scores, b->d_scores, (size_t)b->n * sizeof(int32_t), hipMemcpyDeviceToHost), "hipMemcpy D2H (scores)");
-    const double w2 = now_ms();
+    double d2h = now_ms() - d0;
     int launches = 1;
     if (sh.reg_c == 16 || sh.lring) {
         // the kernels with an int16 LDS ring stop past their score limit with
         // an overflow status: such pairs get the LDS kernel at the same
-        // capacity, so the result depends on the capacity only
+        // capacity, so whether a pair fits depends on the capacity and not on
+        // that limit (the re-run's launch counts in the wall time)
+        const double r0 = now_ms();
         std::vector<int32_t> again;
         for (int32_t i = 0; i < b->n; ++i)
             if (scores[i] == TSU_EKOVERFLOW) again.push_back(i);
@@ -523,12 +526,15 @@ extern "C" int tsu_align_plain(tsu_ctx* ctx, tsu_batch* b, int32_t wf_length, in
             if (r != TSU_OK) return r;
             kms += kms2;
             ++launches;
+            wall += now_ms() - r0;
+            d0 = now_ms();
             HIPCHK(hipMemcpy(scores, b->d_scores, (size_t)b->n * sizeof(int32_t), hipMemcpyDeviceToHost), "hipMemcpy D2H (scores)");
+            d2h += now_ms() - d0;
         }
     }
     if (t) {
         std::memset(t, 0, sizeof(*t));
-        t->kernel_ms = kms; t->wall_ms = w1 - w0; t->d2h_ms = w2 - w1;
+        t->kernel_ms = kms; t->wall_ms = wall; t->d2h_ms = d2h;
         t->pairs = b->n; t->cells = b->cells; t->launches = launches;
     }
     for (int32_t i = 0; i < b->n; ++i) {
@@ -875,7 +881,11 @@ extern "C" int tsu_autotune(tsu_ctx* ctx, tsu_batch* b, int32_t sample_pairs) {
             tsu_timing tt;
             std::memset(&tt, 0, sizeof(tt));
             int r = batched_core(ctx, b, sample, scratch.data(), &tt, true);
-            if (r != TSU_OK) continue;
+            // a shape that cannot run here (no room in LDS, no occupancy, no
+            // device memory for its ring) is skipped; any other failure is a
+            // device or internal error and ends the call with no table
+            if (r == TSU_EINVAL || r == TSU_ENOMEM) { (void)hipGetLastError(); continue; }
+            if (r != TSU_OK) { ctx->tune.clear(); return r; }
             if (tt.kernel_ms < best) { best = tt.kernel_ms; pick = &probe; }
         }
         if (pick) table.push_back(*pick);
@@ -1003,6 +1013,7 @@ extern "C" int tsu_align_cigar(tsu_ctx* ctx, tsu_batch* b, int32_t* scores, tsu_
         grid = std::max<int64_t>(1, std::min<int64_t>(grid, budget / std::max<int64_t>(per_wg, 1)));
         cg->state_bytes = std::max(cg->state_bytes, per_wg * grid);
         if (sh.ck) ++cg->checkpoint_launches; else ++cg->record_launches;
+        cg->kernels |= tsu_host::kernel_bit(sh);
         void *rec = nullptr, *slo = nullptr, *soff = nullptr, *ckb = nullptr;
         HIPCHK(bt_reserve(ctx, 0, (size_t)(recs * grid) * 4, &rec), "hipMalloc (records)");
         HIPCHK(bt_reserve(ctx, 1, (size_t)((int64_t)step_cap * grid) * 4, &slo), "hipMalloc (steps)");
@@ -1134,6 +1145,7 @@ extern "C" int tsu_align_cigar(tsu_ctx* ctx, tsu_batch* b, int32_t* scores, tsu_
         if (r != TSU_OK) return r;
         cg->state_bytes = std::max(cg->state_bytes, wp.per_wg * grid);
         ++cg->record_launches;
+        cg->kernels |= tsu_host::kernel_bit(wp.sh);
         ++launches;
         return TSU_OK;
     };
@@ -1451,6 +1463,19 @@ extern "C" int tsu_cigars_stats(const tsu_cigars* c, int64_t* state_bytes, int32
     if (state_bytes) *state_bytes = c->state_bytes;
     if (record_launches) *record_launches = c->record_launches;
     if (checkpoint_launches) *checkpoint_launches = c->checkpoint_launches;
+    return TSU_OK;
+}
+
+uint32_t tsu_host::kernel_bit(const Shape& sh) {
+    if (sh.ck) return TSU_KERNEL_LDS_CK;
+    if (sh.reg_c == 16) return sh.nw <= 4 ? TSU_KERNEL_ML4 : sh.nw == 8 ? TSU_KERNEL_ML8 : TSU_KERNEL_ML16;
+    if (sh.reg_c > 0) return TSU_KERNEL_REG1;
+    return TSU_KERNEL_LDS;
+}
+
+extern "C" int tsu_cigars_kernels(const tsu_cigars* c, uint32_t* kernels) {
+    if (!c || !kernels) return TSU_EINVAL;
+    *kernels = c->kernels;
     return TSU_OK;
 }
 

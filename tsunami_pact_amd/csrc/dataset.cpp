@@ -1,13 +1,13 @@
 // dataset.cpp -- host-side inputs of the aligner: the sequences.txt format,
 // the seeded pair generator and the 2-bit packer.
 //
-// Format (reference: wfa_gpu.cu:71-81 reads it, generate_dataset.cpp:128-158
+// Format (reference: wfa_gpu.cu:71-81 reads it, generate_dataset.cpp:39-41,68-69
 // writes it): three integers "num len err", then for each pair a pattern
 // token and a text token separated by white space.  The reference reads each
 // token into a fixed slot of len characters; a token longer than len (its own
-// generator leaves trailing heap bytes when buffers are not NUL-terminated,
-// generate_dataset.cpp:115-116,157-158) therefore contributes its first len
-// characters.  That rule is kept.  Shorter tokens are kept whole, which makes
+// generator prints buffers that are not NUL-terminated, generate_dataset.cpp:
+// 26-27,68-69, so heap bytes can follow the sequence) therefore contributes
+// its first len characters.  That rule is kept.  Shorter tokens are kept whole, which makes
 // the same format carry pairs of unequal length (indels).
 #include "tsu_internal.h"
 
@@ -77,7 +77,10 @@ void generate_pair(const tsu_gen_cfg& cfg, uint64_t index, std::string& pat, std
     }
     double rate = cfg.err_min;
     if (cfg.err_max > cfg.err_min) rate = cfg.err_min + rng.unit() * (cfg.err_max - cfg.err_min);
-    int64_t edits = (int64_t)std::llround(rate * (double)L);
+    // edit_mode 2 counts like the reference's integer arithmetic,
+    // len * percent / 100 rounded down (generate_dataset.cpp:22-23)
+    int64_t edits = cfg.edit_mode == 2 ? (int64_t)L * (int64_t)std::llround(rate * 100.0) / 100
+                                       : (int64_t)std::llround(rate * (double)L);
     if (edits < 0) edits = 0;
     if (edits > L) edits = L;
     pat.resize((size_t)L);
@@ -93,7 +96,7 @@ void generate_pair(const tsu_gen_cfg& cfg, uint64_t index, std::string& pat, std
     for (int32_t j = 0; j < L; ++j) {
         const char c = pat[(size_t)j];
         if (!mark[(size_t)j]) { txt.push_back(c); continue; }
-        const int type = cfg.edit_mode == 1 ? 0 : (int)rng.below(3);
+        const int type = cfg.edit_mode != 0 ? 0 : (int)rng.below(3);
         if (type == 0) {                 // substitution to a different base
             const int cur = kBases.code[(unsigned char)c];
             txt.push_back(kAlphabet[(cur + 1 + (int)rng.below(3)) & 3]);
@@ -252,7 +255,7 @@ extern "C" int tsu_dataset_generate(const tsu_gen_cfg* cfg, int64_t n_pairs, tsu
     if (!cfg || !out || n_pairs < 0 || n_pairs > INT32_MAX) return TSU_EINVAL;
     if (cfg->len_min < 1 || cfg->len_max < cfg->len_min) return TSU_EINVAL;
     if (!(cfg->err_min >= 0.0) || !(cfg->err_max >= cfg->err_min) || cfg->err_max > 1.0) return TSU_EINVAL;
-    if (cfg->edit_mode != 0 && cfg->edit_mode != 1) return TSU_EINVAL;
+    if (cfg->edit_mode < 0 || cfg->edit_mode > 2) return TSU_EINVAL;
     *out = nullptr;
     tsu_dataset* ds = new (std::nothrow) tsu_dataset();
     if (!ds) return TSU_ENOMEM;

@@ -69,7 +69,13 @@ struct tsu_cigars {
     int64_t state_bytes = 0;       // largest backtrace state of one launch (device)
     int32_t record_launches = 0;   // launches with records for every score
     int32_t checkpoint_launches = 0;
+    uint32_t kernels = 0;          // TSU_KERNEL_* bits of the launches that recorded
 };
+
+namespace tsu_host {
+// TSU_KERNEL_* bit of a backtrace launch of this shape.
+uint32_t kernel_bit(const Shape& sh);
+}  // namespace tsu_host
 
 namespace tsu_host {
 
