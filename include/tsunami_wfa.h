@@ -168,6 +168,13 @@ int tsu_batch_from_dataset(tsu_ctx* ctx, const tsu_dataset* ds, int64_t first, i
 int tsu_batch_create(tsu_ctx* ctx, const uint32_t* words, int64_t n_words, const tsu_pair* pairs,
                      int32_t n_pairs, tsu_batch** out);
 int32_t tsu_batch_size(const tsu_batch* b);
+/* The batch's device arrays: packed words (n_words 32-bit words), pair
+ * descriptors and per-pair score slots (any pointer may be NULL).  They stay
+ * valid until tsu_batch_free.  The reference keeps its device buffers inside
+ * main() (wfa_gpu.cu:96-111); this hands a resident batch to callers' own
+ * device code or to tsu_align_batched_noalloc. */
+int tsu_batch_device(const tsu_batch* b, const uint32_t** d_words, int64_t* n_words,
+                     const tsu_pair** d_pairs, int32_t** d_scores);
 /* Bytes of HBM the batch holds (packed words + descriptors + score slots). */
 int64_t tsu_batch_bytes(const tsu_batch* b);
 /* May be called before or after tsu_destroy of the batch's context. */
@@ -207,6 +214,28 @@ typedef struct {
     int32_t reserved;
 } tsu_tune_entry;
 int tsu_align_batched(tsu_ctx* ctx, tsu_batch* b, int32_t* scores, tsu_timing* t);
+/* The batched form on the caller's device arrays with no device allocation.
+ * No counterpart in the reference: it is for callers that cannot allocate
+ * device memory during a call (serving loops with a fixed footprint,
+ * harnesses that forbid hipMalloc).  Inputs: d_words (n_words packed words)
+ * and d_pairs (n_pairs descriptors) on the context's device, and the same
+ * descriptors on the host (pairs), which bucket the work.  Outputs: d_scores
+ * (device, n_pairs slots) and scores (host, n_pairs slots, the same values).
+ * Scores equal tsu_align_batched's.  The work counters and the queue order
+ * live in one device array per GPU, allocated with the library's code and
+ * held until the process ends (TSU_NOALLOC_MAX_PAIRS * 4 + 256 bytes):
+ * more than TSU_NOALLOC_MAX_PAIRS pairs are refused with TSU_EINVAL.  Every
+ * call on a GPU shares that array, so calls that overlap on one GPU (from
+ * several threads or contexts) race and must be serialised by the caller.
+ * No hipMalloc or hipFree is made; every launch keeps its ring in LDS, so a
+ * pair whose capacity would need a ring in device memory (wider than
+ * 16,384 diagonals with 4/6/2, or than LDS holds with other penalties) gets
+ * TSU_EKOVERFLOW, as does the call (tsu_align_batched aligns such pairs).
+ * The call is synchronous. */
+#define TSU_NOALLOC_MAX_PAIRS 1048576
+int tsu_align_batched_noalloc(tsu_ctx* ctx, const uint32_t* d_words, int64_t n_words, const tsu_pair* d_pairs,
+                              const tsu_pair* pairs, int32_t n_pairs, int32_t* d_scores, int32_t* scores,
+                              tsu_timing* t);
 /* Runs the autotuner on a sample of b and stores the per-bucket table in the
  * context (also used by later tsu_align_batched calls).  Optional. */
 int tsu_autotune(tsu_ctx* ctx, tsu_batch* b, int32_t sample_pairs);

@@ -45,8 +45,10 @@ EXPORTS = (
     "tsu_packed_words", "tsu_pack",
     "tsu_dataset_read", "tsu_dataset_write", "tsu_dataset_generate", "tsu_dataset_from_strings",
     "tsu_dataset_size", "tsu_dataset_pair", "tsu_dataset_lengths", "tsu_dataset_header", "tsu_dataset_free",
-    "tsu_batch_from_dataset", "tsu_batch_create", "tsu_batch_size", "tsu_batch_bytes", "tsu_batch_free",
-    "tsu_align_plain", "tsu_align_fixed", "tsu_align_batched", "tsu_autotune", "tsu_tune_table",
+    "tsu_batch_from_dataset", "tsu_batch_create", "tsu_batch_size", "tsu_batch_device", "tsu_batch_bytes",
+    "tsu_batch_free",
+    "tsu_align_plain", "tsu_align_fixed", "tsu_align_batched", "tsu_align_batched_noalloc", "tsu_autotune",
+    "tsu_tune_table",
     "tsu_align_cigar", "tsu_cigars_get", "tsu_cigars_string", "tsu_cigars_free",
     "tsu_set_cigar_memory", "tsu_cigars_stats", "tsu_cigars_kernels", "tsu_set_poll_limit",
     "tsu_serve_open", "tsu_serve_submit", "tsu_serve_submit_packed", "tsu_serve_poll", "tsu_serve_wait",
@@ -136,11 +138,13 @@ def library() -> ctypes.CDLL:
         "tsu_batch_from_dataset": (ctypes.c_int, [vp, vp, i64, i64, P(vp)]),
         "tsu_batch_create": (ctypes.c_int, [vp, P(ctypes.c_uint32), i64, P(Pair), i32, P(vp)]),
         "tsu_batch_size": (i32, [vp]),
+        "tsu_batch_device": (ctypes.c_int, [vp, P(vp), P(i64), P(vp), P(vp)]),
         "tsu_batch_bytes": (i64, [vp]),
         "tsu_batch_free": (None, [vp]),
         "tsu_align_plain": (ctypes.c_int, [vp, vp, i32, P(i32), P(Timing)]),
         "tsu_align_fixed": (ctypes.c_int, [vp, ctypes.c_char_p, ctypes.c_char_p, i32, i32, i32, P(i32), P(Timing)]),
         "tsu_align_batched": (ctypes.c_int, [vp, vp, P(i32), P(Timing)]),
+        "tsu_align_batched_noalloc": (ctypes.c_int, [vp, vp, i64, vp, P(Pair), i32, vp, P(i32), P(Timing)]),
         "tsu_autotune": (ctypes.c_int, [vp, vp, i32]),
         "tsu_tune_table": (ctypes.c_int, [vp, P(TuneEntry), i32, P(i32)]),
         "tsu_align_cigar": (ctypes.c_int, [vp, vp, P(i32), P(vp), P(Timing)]),

@@ -113,6 +113,9 @@ extern "C" int tsu_create(tsu_ctx** out, int device, int32_t mismatch, int32_t g
     if ((e = hipSetDevice(device)) != hipSuccess) { delete ctx; return TSU_EHIP; }
     hipDeviceProp_t prop;
     if ((e = hipGetDeviceProperties(&prop, device)) != hipSuccess) { delete ctx; return TSU_EHIP; }
+    // the no-allocation entry point's scratch: its address on this device,
+    // resolved once per device with the device current
+    if (tsu::scratch_address(device, &ctx->scratch) != 0) { delete ctx; return TSU_EHIP; }
     ctx->cu_count = prop.multiProcessorCount;
     ctx->max_lds = (int64_t)prop.sharedMemPerBlock;
     std::snprintf(ctx->name, sizeof(ctx->name), "%.180s (%.60s)", prop.name, prop.gcnArchName);
@@ -267,6 +270,16 @@ extern "C" int tsu_batch_from_dataset(tsu_ctx* ctx, const tsu_dataset* ds, int64
 
 extern "C" int32_t tsu_batch_size(const tsu_batch* b) { return b ? b->n : 0; }
 
+extern "C" int tsu_batch_device(const tsu_batch* b, const uint32_t** d_words, int64_t* n_words,
+                                const tsu_pair** d_pairs, int32_t** d_scores) {
+    if (!b) return TSU_EINVAL;
+    if (d_words) *d_words = b->d_words;
+    if (n_words) *n_words = b->n_words;
+    if (d_pairs) *d_pairs = reinterpret_cast<const tsu_pair*>(b->d_pairs);
+    if (d_scores) *d_scores = b->d_scores;
+    return TSU_OK;
+}
+
 extern "C" int64_t tsu_batch_bytes(const tsu_batch* b) {
     if (!b) return 0;
     return b->n_words * 4 + (int64_t)b->n * (int64_t)(sizeof(KPair) + sizeof(int32_t));
@@ -327,21 +340,31 @@ int64_t tsu_host::resident_grid(const tsu_ctx* ctx, const Shape& sh, const KArgs
     return (int64_t)per_cu * ctx->cu_count;
 }
 
+// Device arrays a launch reads and writes: packed words, descriptors and
+// per-pair scores (a batch's own, or a caller's).
+struct DevIO {
+    const uint32_t* words;
+    const KPair* pairs;
+    int32_t* scores;
+};
+
+static DevIO io_of(const tsu_batch* b) { return DevIO{b->d_words, b->d_pairs, b->d_scores}; }
+
 // Launches one shape on `stream` (queue counter `queue`, events e0/e1 around
 // the kernel) without waiting; an HBM ring, when the shape needs one, is left
 // in `ring` and must outlive the kernel.  *launched is false when there was
 // nothing to launch.
-static int launch_shape(tsu_ctx* ctx, tsu_batch* b, Shape sh, int32_t W, const int32_t* d_order, int32_t n_entries,
+static int launch_shape(tsu_ctx* ctx, const DevIO& io, Shape sh, int32_t W, const int32_t* d_order, int32_t n_entries,
                         int32_t max_len, int64_t grid_cap, KArgs* extra, hipStream_t stream, int32_t* queue,
                         hipEvent_t e0, hipEvent_t e1, DevBuf<uint8_t>& ring, bool* launched) {
     *launched = false;
     KArgs a = extra ? *extra : base_args(ctx);
     a.x = ctx->xs; a.o = ctx->os; a.e = ctx->es; a.g = ctx->g; a.rm = ctx->rm; a.ri = ctx->ri;
-    a.words = b->d_words;
-    a.pairs = b->d_pairs;
+    a.words = io.words;
+    a.pairs = io.pairs;
     a.order = d_order;
     a.n = n_entries;
-    a.scores = b->d_scores;
+    a.scores = io.scores;
     a.W = W;
     a.seq_words = seq_words_for(max_len);
     a.queue = queue;
@@ -376,12 +399,12 @@ static int launch_shape(tsu_ctx* ctx, tsu_batch* b, Shape sh, int32_t W, const i
 // Runs one launch of `shape` over the queue entries `order` (or pairs [0,n)
 // when order is null and the launch is plain) on the context's stream and
 // waits for it.  Scores land in b->d_scores.
-static int run_shape(tsu_ctx* ctx, tsu_batch* b, Shape sh, int32_t W, const int32_t* d_order, int32_t n_entries,
+static int run_shape(tsu_ctx* ctx, const DevIO& io, Shape sh, int32_t W, const int32_t* d_order, int32_t n_entries,
                      int32_t max_len, int64_t grid_cap, KArgs* extra, double* kernel_ms) {
     DevBuf<uint8_t> ring;
     bool launched = false;
     if (kernel_ms) *kernel_ms = 0;
-    int r = launch_shape(ctx, b, sh, W, d_order, n_entries, max_len, grid_cap, extra, ctx->stream, ctx->d_queue,
+    int r = launch_shape(ctx, io, sh, W, d_order, n_entries, max_len, grid_cap, extra, ctx->stream, ctx->d_queue,
                          ctx->ev0, ctx->ev1, ring, &launched);
     if (r != TSU_OK || !launched) return r;
     HIPCHK(hipEventSynchronize(ctx->ev1), "hipEventSynchronize");
@@ -487,14 +510,14 @@ extern "C" int tsu_align_plain(tsu_ctx* ctx, tsu_batch* b, int32_t wf_length, in
     double kms = 0;
     int r;
     if (fits) {
-        r = run_shape(ctx, b, sh, W, nullptr, b->n, b->max_len, 0, nullptr, &kms);
+        r = run_shape(ctx, io_of(b), sh, W, nullptr, b->n, b->max_len, 0, nullptr, &kms);
     } else {
         // ring too wide for LDS: same one-workgroup-per-pair work, ring in HBM,
         // resident workgroups fed from the queue
         sh.queue = true;
         sh.lds_ring = false;
         sh.nw = 16;
-        r = run_shape(ctx, b, sh, W, nullptr, b->n, b->max_len, 0, nullptr, &kms);
+        r = run_shape(ctx, io_of(b), sh, W, nullptr, b->n, b->max_len, 0, nullptr, &kms);
     }
     if (r != TSU_OK) return r;
     double wall = now_ms() - w0;
@@ -522,7 +545,7 @@ extern "C" int tsu_align_plain(tsu_ctx* ctx, tsu_batch* b, int32_t wf_length, in
             HIPCHK(d_order.alloc(again.size()), "hipMalloc (order)");
             HIPCHK(hipMemcpy(d_order.p, again.data(), again.size() * sizeof(int32_t), hipMemcpyHostToDevice), "hipMemcpy (order)");
             double kms2 = 0;
-            r = run_shape(ctx, b, s2, W, d_order.p, (int32_t)again.size(), b->max_len, 0, nullptr, &kms2);
+            r = run_shape(ctx, io_of(b), s2, W, d_order.p, (int32_t)again.size(), b->max_len, 0, nullptr, &kms2);
             if (r != TSU_OK) return r;
             kms += kms2;
             ++launches;
@@ -701,7 +724,7 @@ static int batched_core(tsu_ctx* ctx, tsu_batch* b, const std::vector<int32_t>& 
             HIPCHK(hipMemcpyAsync(d_order.p, todo.data(), todo.size() * sizeof(int32_t), hipMemcpyHostToDevice, ctx->stream),
                    "hipMemcpyAsync (order)");
             double kms = 0;
-            int r = run_shape(ctx, b, sh, W, d_order.p, (int32_t)todo.size(), max_len, 0, nullptr, &kms);
+            int r = run_shape(ctx, io_of(b), sh, W, d_order.p, (int32_t)todo.size(), max_len, 0, nullptr, &kms);
             if (r != TSU_OK) return r;
             kms_total += kms;
             ++launches;
@@ -763,7 +786,7 @@ static int batched_core(tsu_ctx* ctx, tsu_batch* b, const std::vector<int32_t>& 
         HIPCHK(hipMemcpyAsync(d_order.p, first.data(), first.size() * sizeof(int32_t), hipMemcpyHostToDevice, ctx->stream),
                "hipMemcpyAsync (order)");
         double kms = 0;
-        int r = run_shape(ctx, b, sh, W, d_order.p, (int32_t)first.size(), max_len, 0, nullptr, &kms);
+        int r = run_shape(ctx, io_of(b), sh, W, d_order.p, (int32_t)first.size(), max_len, 0, nullptr, &kms);
         if (r != TSU_OK) return r;
         kms_total += kms;
         ++launches;
@@ -785,7 +808,7 @@ static int batched_core(tsu_ctx* ctx, tsu_batch* b, const std::vector<int32_t>& 
         HIPCHK(hipEventCreate(&d->e1), "hipEventCreate");
         HIPCHK(hipMemcpyAsync(d->d_ids.p, d->ids.data(), d->ids.size() * sizeof(int32_t), hipMemcpyHostToDevice, ctx->stream2),
                "hipMemcpyAsync (order)");
-        r = launch_shape(ctx, b, batched_shape(ctx, d->W, max_len, 0), d->W, d->d_ids.p, (int32_t)d->ids.size(), max_len, 0,
+        r = launch_shape(ctx, io_of(b), batched_shape(ctx, d->W, max_len, 0), d->W, d->d_ids.p, (int32_t)d->ids.size(), max_len, 0,
                          nullptr, ctx->stream2, ctx->d_queue + 16, d->e0, d->e1, d->ring, &d->launched);
         if (r != TSU_OK) { (void)hipStreamSynchronize(ctx->stream2); return r; }
         ++launches;
@@ -830,6 +853,122 @@ extern "C" int tsu_align_batched(tsu_ctx* ctx, tsu_batch* b, int32_t* scores, ts
     if (t) { t->wall_ms = w1 - w0; t->pairs = b->n; t->cells = b->cells; }
     if (r != TSU_OK) return r;
     for (int32_t i = 0; i < b->n; ++i)
+        if (scores[i] < 0) return fail(ctx, scores[i], "pair " + std::to_string(i) + ": " + tsu_strerror(scores[i]));
+    return TSU_OK;
+}
+
+// Batched form without device allocation: the caller's device arrays in and
+// out, the work counters and queue order in the device's scratch array
+// (tsu::scratch_address), every launch on the context's stream.  Buckets and
+// capacities as tsu_align_batched; overflowing pairs are re-run wider, one
+// launch after another on the same stream; a pair whose capacity would need
+// a ring in device memory is reported as overflowing.
+extern "C" int tsu_align_batched_noalloc(tsu_ctx* ctx, const uint32_t* d_words, int64_t n_words,
+                                         const tsu_pair* d_pairs, const tsu_pair* pairs, int32_t n_pairs,
+                                         int32_t* d_scores, int32_t* scores, tsu_timing* t) {
+    if (!ctx) return TSU_EINVAL;
+    if (n_pairs < 0 || n_words < 0 ||
+        (n_pairs > 0 && (!d_words || !d_pairs || !pairs || !d_scores || !scores || n_words == 0)))
+        return fail(ctx, TSU_EINVAL, "tsu_align_batched_noalloc: bad arguments");
+    if (n_pairs > tsu::kNoallocMaxPairs)
+        return fail(ctx, TSU_EINVAL, "tsu_align_batched_noalloc: " + std::to_string(n_pairs) + " pairs; at most " +
+                                         std::to_string(tsu::kNoallocMaxPairs) +
+                                         " fit its device scratch (one queue entry per pair)");
+    if (!ctx->scratch) return fail(ctx, TSU_EINTERNAL, "tsu_align_batched_noalloc: no scratch address");
+    int32_t max_all = 0;
+    double cells = 0;
+    for (int32_t i = 0; i < n_pairs; ++i) {
+        const tsu_pair& p = pairs[i];
+        if (p.p_len < 0 || p.t_len < 0 || p.p_off < 0 || p.t_off < 0 ||
+            p.p_off + tsu_packed_words(p.p_len) > n_words || p.t_off + tsu_packed_words(p.t_len) > n_words)
+            return fail(ctx, TSU_EINVAL, "pair " + std::to_string(i) + " lies outside the packed arena");
+        max_all = std::max(max_all, std::max(p.p_len, p.t_len));
+        cells += (double)p.p_len * (double)p.t_len;
+    }
+    if (max_all > tsu_host::max_seq_len(ctx))
+        return fail(ctx, TSU_EINVAL, "a sequence is longer than " + std::to_string(tsu_host::max_seq_len(ctx)) +
+                                         " bases (both sequences of a pair are staged in LDS)");
+    HIPCHK(hipSetDevice(ctx->device), "hipSetDevice");
+    if (t) std::memset(t, 0, sizeof(*t));
+    const double w0 = now_ms();
+    const DevIO io{d_words, reinterpret_cast<const KPair*>(d_pairs), d_scores};
+    int32_t* d_queue = static_cast<int32_t*>(ctx->scratch);
+    int32_t* d_order = reinterpret_cast<int32_t*>(static_cast<uint8_t*>(ctx->scratch) + tsu::kNoallocCounterBytes);
+    const int nb = tsu_host::bucket_count();
+    std::vector<std::vector<int32_t>> bk((size_t)nb);
+    for (int32_t i = 0; i < n_pairs; ++i)
+        bk[(size_t)tsu_host::bucket_of(std::max(pairs[i].p_len, pairs[i].t_len))].push_back(i);
+    // one region of the order array per bucket, longest pairs first
+    std::vector<int32_t> order;
+    order.reserve((size_t)n_pairs);
+    std::vector<size_t> base((size_t)nb, 0);
+    for (int bi = 0; bi < nb; ++bi) {
+        tsu_host::order_longest_first(bk[(size_t)bi], pairs);
+        base[(size_t)bi] = order.size();
+        order.insert(order.end(), bk[(size_t)bi].begin(), bk[(size_t)bi].end());
+    }
+    if (n_pairs > 0)
+        HIPCHK(hipMemcpyAsync(d_order, order.data(), order.size() * sizeof(int32_t), hipMemcpyHostToDevice, ctx->stream),
+               "hipMemcpyAsync (order)");
+    double kms_total = 0;
+    int launches = 0, retried = 0;
+    int32_t no_room = -1;  // a pair whose capacity needs a ring in device memory
+    DevBuf<uint8_t> unused_ring;  // stays empty: every launch here keeps its ring in LDS
+    for (int bi = nb - 1; bi >= 0; --bi) {
+        std::vector<int32_t> todo = bk[(size_t)bi];
+        if (todo.empty()) continue;
+        int32_t max_len = 0;
+        for (int32_t i : todo) max_len = std::max(max_len, std::max(pairs[i].p_len, pairs[i].t_len));
+        const tsu_tune_entry* te = tune_lookup(ctx, tsu_host::bucket_bound(bi));
+        int32_t W = (te && te->capacity > 0) ? te->capacity
+                                             : initial_capacity(ctx, std::min(tsu_host::bucket_bound(bi), max_len));
+        const int nw_hint = te ? te->waves_per_pair : 0;
+        int32_t* region = d_order + base[(size_t)bi];
+        for (int attempt = 0; !todo.empty(); ++attempt) {
+            W = std::min<int32_t>(W, capacity_full(max_len, max_len));
+            const Shape sh = batched_shape(ctx, W, max_len, attempt == 0 ? nw_hint : 0);
+            if (!sh.lds_ring) {
+                for (int32_t i : todo) scores[i] = TSU_EKOVERFLOW;
+                no_room = todo[0];
+                break;
+            }
+            if (attempt > 0)
+                HIPCHK(hipMemcpyAsync(region, todo.data(), todo.size() * sizeof(int32_t), hipMemcpyHostToDevice,
+                                      ctx->stream), "hipMemcpyAsync (order)");
+            bool launched = false;
+            int r = launch_shape(ctx, io, sh, W, region, (int32_t)todo.size(), max_len, 0, nullptr, ctx->stream, d_queue,
+                                 ctx->ev0, ctx->ev1, unused_ring, &launched);
+            if (r != TSU_OK) return r;
+            if (launched) {
+                HIPCHK(hipEventSynchronize(ctx->ev1), "hipEventSynchronize");
+                float ms = 0.f;
+                HIPCHK(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1), "hipEventElapsedTime");
+                kms_total += ms;
+                ++launches;
+            }
+            HIPCHK(hipMemcpy(scores, d_scores, (size_t)n_pairs * sizeof(int32_t), hipMemcpyDeviceToHost),
+                   "hipMemcpy D2H (scores)");
+            std::vector<int32_t> again;
+            for (int32_t i : todo)
+                if (scores[i] == TSU_EKOVERFLOW) again.push_back(i);
+            if (!again.empty()) {
+                if (W >= capacity_full(max_len, max_len)) return fail(ctx, TSU_EINTERNAL, "overflow at full capacity");
+                retried += (int)again.size();
+                W = (int32_t)std::min<int64_t>(2ll * W, capacity_full(max_len, max_len));
+            }
+            todo.swap(again);
+        }
+    }
+    const double w1 = now_ms();
+    if (t) {
+        t->kernel_ms = kms_total; t->wall_ms = w1 - w0; t->launches = launches; t->retried = retried;
+        t->pairs = n_pairs; t->cells = cells;
+    }
+    if (no_room >= 0)
+        return fail(ctx, TSU_EKOVERFLOW, "pair " + std::to_string(no_room) +
+                                             ": its capacity needs a ring in device memory, which this entry point "
+                                             "does not allocate (tsu_align_batched aligns it)");
+    for (int32_t i = 0; i < n_pairs; ++i)
         if (scores[i] < 0) return fail(ctx, scores[i], "pair " + std::to_string(i) + ": " + tsu_strerror(scores[i]));
     return TSU_OK;
 }
@@ -1029,7 +1168,7 @@ extern "C" int tsu_align_cigar(tsu_ctx* ctx, tsu_batch* b, int32_t* scores, tsu_
         a.runs = d_runs.p; a.runs_cap = std::max<int64_t>(runs_cap, 1); a.runs_used = d_used.p;
         a.run_off = d_run_off.p; a.run_n = d_run_n.p;
         double kms = 0;
-        int r = run_shape(ctx, b, sh, W, d_order.p, (int32_t)ids.size(), max_len, grid, &a, &kms);
+        int r = run_shape(ctx, io_of(b), sh, W, d_order.p, (int32_t)ids.size(), max_len, grid, &a, &kms);
         if (r != TSU_OK) return r;
         kms_total += kms;
         ++launches;
@@ -1140,7 +1279,7 @@ extern "C" int tsu_align_cigar(tsu_ctx* ctx, tsu_batch* b, int32_t* scores, tsu_
         a.runs = d_runs.p; a.runs_cap = std::max<int64_t>(runs_cap, 1); a.runs_used = d_used.p;
         a.run_off = d_run_off.p; a.run_n = d_run_n.p;
         wide_async.s = ctx->stream2;
-        int r = launch_shape(ctx, b, wp.sh, wp.W, wide_async.d_ids.p, (int32_t)ids.size(), wp.max_len, grid, &a, ctx->stream2,
+        int r = launch_shape(ctx, io_of(b), wp.sh, wp.W, wide_async.d_ids.p, (int32_t)ids.size(), wp.max_len, grid, &a, ctx->stream2,
                              ctx->d_queue + 16, wide_async.e0, wide_async.e1, wide_async.ring, &wide_async.launched);
         if (r != TSU_OK) return r;
         cg->state_bytes = std::max(cg->state_bytes, wp.per_wg * grid);

@@ -38,6 +38,7 @@ struct tsu_ctx {
     int32_t poll_limit = 0;             // neighbour-sync polls before a pair fails (0 = default)
     // scratch reused across calls
     int32_t* d_queue = nullptr;
+    void* scratch = nullptr;            // this device's no-allocation scratch (tsu::scratch_address)
     void* h_stage = nullptr;            // pinned staging for large device-to-host copies
     size_t h_stage_bytes = 0;
     // backtrace state of tsu_align_cigar (records, step bounds, step offsets,

@@ -91,6 +91,14 @@ struct Shape {
 constexpr int32_t kMlDblMaxLen = 16383;
 constexpr int32_t kMlDblMaxScore = 16383;
 
+// Device scratch of tsu_align_batched_noalloc (a file-scope array per GPU):
+// work counters, then one queue-order entry per pair.
+constexpr int32_t kNoallocMaxPairs = TSU_NOALLOC_MAX_PAIRS;
+constexpr size_t kNoallocCounterBytes = 256;
+constexpr size_t kNoallocScratchBytes = kNoallocCounterBytes + 4 * (size_t)kNoallocMaxPairs;
+// Device address of that array on `device` (current); hipError_t value.
+int scratch_address(int device, void** out);
+
 // LDS bytes a launch of this shape needs.
 int64_t lds_bytes(const Shape& sh, const KArgs& a);
 // Bytes of one ring (all slots) for W diagonals.
