@@ -27,11 +27,21 @@ The batched and CIGAR forms run the autotuner on a sample first (untimed);
 with --compare-plain the batched form also times the plain form on the same
 pairs.
 
+The default run (plain form, 1M x 1000 bp) also measures, each in a child
+process after the headline, and adds to the same JSON line:
+  batched_cfg4  500,000 pairs, 100-20,000 bp log-uniform, 1-15%: autotuned
+                batched form vs the plain form on the same pairs;
+  served_cfg5   Poisson arrivals at three rates, requests of U{1..256} pairs
+                of 100-10,000 bp at 1-15%: p50 / p99 latency and throughput;
+  cigar_cfg3    50,000 pairs of 10,000 bp at 10%: scores and CIGARs.
+(--headline-only skips them.)
+
     python bench.py [--steps K] [--warmup W] [--form plain|batched|cigar|served|refgrid|twopiece]
 """
 import argparse
 import json
 import os
+import subprocess
 import sys
 import time
 
@@ -66,6 +76,25 @@ def algorithmic_bytes(pl, tl):
     return int((8 * ((pl.astype(np.int64) + 31) // 32 + (tl.astype(np.int64) + 31) // 32) + 4).sum())
 
 
+def host_cpu():
+    """The host's CPU model and the cores this process may run on."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"nproc": len(os.sched_getaffinity(0)), "cpu_count": os.cpu_count(), "cpu_model": model}
+
+
+ORACLE_NOTE = ("the oracle: the C restatement of WFA2-lib's exact gap-affine WFA kept literal for parity "
+               "(unscaled penalties, byte-wise extend, per-score wavefront allocation), OpenMP dynamic over "
+               "pairs; WFA2-lib itself is absent offline, and no optimised CPU aligner stands in for it")
+
+
 def cpu_baseline(pairs_fn, n_total, target_cpu_s=20.0):
     """Oracle WFA on the first pairs of the workload, every host thread."""
     from oracle import oracle
@@ -86,7 +115,8 @@ def cpu_baseline(pairs_fn, n_total, target_cpu_s=20.0):
         n = n2
     return {"value": n / dt, "unit": "alignments/s", "cores": threads, "kind": "port",
             "sample": f"first {n} pairs of the workload, oracle exact WFA (C, OpenMP, {threads} threads), "
-                      f"{dt:.2f} s wall"}, scores, cnt
+                      f"{dt:.2f} s wall",
+            "omp_threads": threads, **host_cpu(), "note": ORACLE_NOTE}, scores, cnt
 
 
 def read_traffic(name):
@@ -241,6 +271,137 @@ def run_refgrid(a, tsu, args):
             "vs_reference_peak": best["gcups"] / REF_PEAK_GCUPS}
 
 
+def oracle_sample(ds, m, cigar=False):
+    """Oracle scores (and op strings) of the first m pairs of ds: the checker."""
+    from oracle import oracle
+    sample = [ds.pair(i) for i in range(min(m, len(ds)))]
+    scores, ops, _ = oracle.wfa_batch(sample, *PEN, threads=oracle.max_threads(), cigar=cigar)
+    return scores, ops
+
+
+def section_batched_cfg4(tsu, args):
+    """The mixed-length workload: autotuned batched form against the plain
+    form on the same resident pairs (one capacity for all, sized for the
+    score bound of the longest, most divergent pair)."""
+    n, lmin, lmax, emin, emax, soff = WORKLOADS["cfg4"]
+    if args.pairs:
+        n = min(n, args.pairs)
+    log(f"batched_cfg4: generating {n} pairs")
+    ds = tsu.Dataset.generate(n, seed=SEED + soff, len_min=lmin, len_max=lmax, err_min=emin, err_max=emax)
+    pl, tl = ds.lengths()
+    cells = float((pl.astype(np.float64) * tl).sum())
+    a = tsu.Aligner(*PEN)
+    b = a.batch(ds)
+    t0 = time.perf_counter()
+    tune = a.autotune(b, args.tune_sample)
+    tune_s = time.perf_counter() - t0
+    log(f"batched_cfg4: autotune {tune_s:.1f} s")
+    scores, _ = a.align_batched(b)                 # warmup
+    walls, kms = [], []
+    for i in range(2):
+        t0 = time.perf_counter()
+        scores, t = a.align_batched(b)
+        walls.append(time.perf_counter() - t0)
+        kms.append(t["kernel_ms"])
+        log(f"batched_cfg4: batched step {i}: {walls[-1] * 1e3:.1f} ms")
+    x, o, e = PEN
+    W = 2 * ((int(round(emax * lmax)) * (o + e) - o) // e) + 3
+    a.align_plain(b, W)                             # warmup
+    t0 = time.perf_counter()
+    ps, pt = a.align_plain(b, W)
+    pwall = time.perf_counter() - t0
+    log(f"batched_cfg4: plain step: {pwall * 1e3:.1f} ms")
+    bw = float(np.median(walls))
+    res = {"workload": f"cfg4: {n} pairs, length {lmin}-{lmax} log-uniform bp, {emin * 100:g}-{emax * 100:g}% mixed "
+                       "edits, penalties 4/6/2", "pairs": n, "seed": SEED + soff,
+           "batched": {"alignments_per_s": n / bw, "wall_ms": bw * 1e3, "kernel_ms": float(np.median(kms)),
+                       "gcups": cells / bw / 1e9, "launches": t["launches"], "pairs_widened": t["retried"],
+                       "steps": len(walls), "autotune_s": tune_s},
+           "plain_same_pairs": {"alignments_per_s": n / pwall, "wall_ms": pwall * 1e3, "kernel_ms": pt["kernel_ms"],
+                                "wf_length": W},
+           "speedup_batched_over_plain": pwall / bw,
+           "same_scores": bool((ps == scores).all()),
+           "tune": tune}
+    if not args.no_cpu:
+        want, _ = oracle_sample(ds, 2000)
+        res["bit_exact_sample"] = bool((scores[:len(want)] == want).all())
+        res["sample_pairs"] = int(len(want))
+    b.close()
+    a.close()
+    return res
+
+
+def section_served_cfg5(tsu, args):
+    """Poisson arrivals at three open-loop rates (run_served)."""
+    ns = argparse.Namespace(requests=args.requests, compare_plain=True, serve_budget=args.serve_budget,
+                            inflight=args.inflight, batch_pairs=args.batch_pairs)
+    a = tsu.Aligner(*PEN)
+    res = run_served(a, tsu, ns, None)
+    a.close()
+    return {"workload": "cfg5: Poisson arrivals at 0.3/0.6/0.9 of the batched capacity, requests of U{1..256} "
+                        "pairs, length 100-10000 bp log-uniform, 1-15% mixed edits, penalties 4/6/2",
+            "server": {"hbm_budget_mib": args.serve_budget, "max_inflight": args.inflight,
+                       "max_batch_pairs": args.batch_pairs}, **res}
+
+
+def section_cigar_cfg3(tsu, args):
+    """Long erroneous reads with CIGARs, at the configured 50,000 pairs."""
+    n, lmin, lmax, emin, emax, soff = WORKLOADS["cfg3"]
+    if args.pairs:
+        n = min(n, args.pairs)
+    log(f"cigar_cfg3: generating {n} pairs")
+    ds = tsu.Dataset.generate(n, seed=SEED + soff, len_min=lmin, len_max=lmax, err_min=emin, err_max=emax)
+    pl, tl = ds.lengths()
+    cells = float((pl.astype(np.float64) * tl).sum())
+    a = tsu.Aligner(*PEN)
+    b = a.batch(ds)
+    tune = a.autotune(b, args.tune_sample)
+    a.align_cigar(b, strings=False)                 # warmup
+    t0 = time.perf_counter()
+    m = 0 if args.no_cpu else 200
+    scores, cigars, t = a.align_cigar(b, strings=m)  # CIGAR strings of the first m pairs only
+    wall = time.perf_counter() - t0
+    log(f"cigar_cfg3: {wall * 1e3:.1f} ms")
+    res = {"workload": f"cfg3: {n} pairs, {lmin} bp, {emin * 100:g}% mixed edits, penalties 4/6/2, CIGAR form",
+           "pairs": n, "seed": SEED + soff, "alignments_per_s": n / wall, "wall_ms": wall * 1e3,
+           "kernel_ms": t["kernel_ms"], "gcups": cells / wall / 1e9, "launches": t["launches"],
+           "backtrace": {"state_bytes": t["state_bytes"], "record_launches": t["record_launches"],
+                         "checkpoint_launches": t["checkpoint_launches"], "kernels": sorted(t["kernels"])},
+           "tune": tune}
+    if m:
+        from oracle import oracle
+        want, ops = oracle_sample(ds, m, cigar=True)
+        res["bit_exact_sample"] = bool((scores[:m] == want).all()) and cigars == [oracle.rle(o) for o in ops]
+        res["sample_pairs"] = m
+    b.close()
+    a.close()
+    return res
+
+
+SECTIONS = {"batched_cfg4": section_batched_cfg4, "served_cfg5": section_served_cfg5,
+            "cigar_cfg3": section_cigar_cfg3}
+
+
+def run_section_child(name, args):
+    """One section in a child process (its own device context and memory);
+    its progress goes to stderr, its result is its last stdout line."""
+    cmd = [sys.executable, os.path.abspath(__file__), "--section", name, "--tune-sample", str(args.tune_sample),
+           "--requests", str(args.requests)]
+    if args.pairs:
+        cmd += ["--pairs", str(args.pairs)]
+    if args.no_cpu:
+        cmd.append("--no-cpu")
+    log(f"section {name}")
+    try:
+        r = subprocess.run(cmd, stdout=subprocess.PIPE, text=True, timeout=900)
+    except subprocess.TimeoutExpired:
+        return {"error": "timed out after 900 s"}
+    lines = r.stdout.strip().splitlines()
+    if r.returncode != 0 or not lines:
+        return {"error": f"exit status {r.returncode}"}
+    return json.loads(lines[-1])
+
+
 PEN_2P = (4, 6, 2, 24, 1)  # gap-affine 2-piece: gap of length L costs min(6 + 2L, 24 + L)
 
 
@@ -300,6 +461,9 @@ def main():
     ap.add_argument("--batch-pairs", type=int, default=2048, help="served: pairs merged per launch")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
                     help="only run the streaming copy kernel over BYTES (HBM counter calibration)")
+    ap.add_argument("--headline-only", action="store_true",
+                    help="default form: skip the batched_cfg4, served_cfg5 and cigar_cfg3 sections")
+    ap.add_argument("--section", choices=sorted(SECTIONS), help=argparse.SUPPRESS)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
@@ -308,6 +472,9 @@ def main():
 
     import tsunami_pact_amd as tsu
 
+    if args.section:
+        print(json.dumps(SECTIONS[args.section](tsu, args)), flush=True)
+        return
     if args.calibrate_copy:
         a = tsu.Aligner(*PEN)
         ms = a.calibrate_copy(args.calibrate_copy)
@@ -411,7 +578,7 @@ def main():
             ops_per_pair = (OPS_PER_CELL * cnt["cells"] + OPS_PER_EXTEND * cnt["extend_words"]) / m
             ach = ops_per_pair * n / (kernel_ms / 1e3)
             valu = {"bound": "valu", "achieved": ach / 1e12, "peak": VALU_PEAK_LANE_OPS / 1e12,
-                    "unit": "T int32 lane-ops/s", "frac": ach / VALU_PEAK_LANE_OPS,
+                    "unit": "TOP/s (int32 lane-ops)", "frac": ach / VALU_PEAK_LANE_OPS,
                     "ops_per_pair": ops_per_pair,
                     "model": f"{OPS_PER_CELL}/cell-triple + {OPS_PER_EXTEND}/extend compare, oracle counts "
                              f"over the CPU sample ({m} pairs)"}
@@ -439,12 +606,17 @@ def main():
         "launches_per_step": t.get("launches"),
         "pairs_widened_per_step": t.get("retried"),
         "bit_exact_sample": exact,
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic},
-        "roofline_valu": valu,
         "cpu_baseline": cpu,
         "device": info["name"],
     }
+    # the binding roof is integer VALU issue; HBM is reported beside it
+    hbm = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+           "frac": achieved / HBM_PEAK_GBS, "traffic": traffic}
+    if valu is not None:
+        out["roofline"] = {**valu, "traffic": traffic}
+        out["roofline_hbm"] = hbm
+    else:
+        out["roofline"] = hbm
     if args.form == "cigar":
         out["config"]["cigar_memory"] = args.cigar_memory
         out["backtrace"] = {"state_bytes": t.get("state_bytes"), "record_launches": t.get("record_launches"),
@@ -453,6 +625,12 @@ def main():
         out["tune"] = tune
     if plain is not None:
         out["plain_same_pairs"] = plain
+    if args.form == "plain" and args.workload == "cfg2" and not args.headline_only:
+        b.close()
+        ds.close()
+        for name in ("batched_cfg4", "served_cfg5", "cigar_cfg3"):
+            out[name] = run_section_child(name, args)
+            log(f"{name}: {json.dumps(out[name])[:400]}")
     print(json.dumps(out), flush=True)
 
 

@@ -428,10 +428,11 @@ class Aligner:
         the default."""
         self._check(self._lib.tsu_set_poll_limit(self._h, polls), "tsu_set_poll_limit")
 
-    def align_cigar(self, batch: Batch, strings: bool = True):
+    def align_cigar(self, batch: Batch, strings=True):
         """(scores, CIGAR strings, timing).  With strings=False the CIGARs are
         computed and copied to the host by the library as usual but not
-        turned into Python strings (None is returned in their place).  The
+        turned into Python strings (None is returned in their place); an
+        integer m returns the strings of the first m pairs only.  The
         timing dict also holds the backtrace state: state_bytes (largest of
         one launch), record_launches, checkpoint_launches and kernels (the
         set of KERNEL_BITS names of the kernels that recorded)."""
@@ -441,11 +442,12 @@ class Aligner:
         h = ctypes.c_void_p()
         self._check(self._lib.tsu_align_cigar(self._h, batch._h, _i32p(scores), ctypes.byref(h), ctypes.byref(t)),
                     "tsu_align_cigar")
-        cigars = [] if strings else None
+        count = n if strings is True else 0 if strings is False else min(int(strings), n)
+        cigars = [] if count or strings is True else None
         try:
             rp = ctypes.POINTER(ctypes.c_uint32)()
             nr = ctypes.c_int32()
-            for i in range(n if strings else 0):
+            for i in range(count):
                 self._check(self._lib.tsu_cigars_get(h, i, ctypes.byref(rp), ctypes.byref(nr)), "tsu_cigars_get")
                 cigars.append(cigar_string(rp[j] for j in range(nr.value)))
             sb, rl, cl = ctypes.c_int64(), ctypes.c_int32(), ctypes.c_int32()
