@@ -14,11 +14,13 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <map>
 #include <memory>
 #include <mutex>
 #include <new>
 #include <numeric>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "tsu_host.h"
@@ -121,6 +123,8 @@ extern "C" int tsu_create(tsu_ctx** out, int device, int32_t mismatch, int32_t g
     std::snprintf(ctx->name, sizeof(ctx->name), "%.180s (%.60s)", prop.name, prop.gcnArchName);
     if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess ||
         hipStreamCreateWithFlags(&ctx->stream2, hipStreamNonBlocking) != hipSuccess ||
+        hipStreamCreateWithFlags(&ctx->stream3, hipStreamNonBlocking) != hipSuccess ||
+        hipStreamCreateWithFlags(&ctx->stream_io, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreate(&ctx->ev0) != hipSuccess || hipEventCreate(&ctx->ev1) != hipSuccess ||
         hipMalloc(reinterpret_cast<void**>(&ctx->d_queue), 64 * sizeof(int32_t)) != hipSuccess) {
         tsu_destroy(ctx);
@@ -179,6 +183,8 @@ extern "C" void tsu_destroy(tsu_ctx* ctx) {
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     if (ctx->stream2) (void)hipStreamSynchronize(ctx->stream2);
+    if (ctx->stream3) (void)hipStreamSynchronize(ctx->stream3);
+    if (ctx->stream_io) (void)hipStreamSynchronize(ctx->stream_io);
     bt_release(ctx);
     if (ctx->d_queue) (void)hipFree(ctx->d_queue);
     if (ctx->h_stage) (void)hipHostFree(ctx->h_stage);
@@ -186,6 +192,8 @@ extern "C" void tsu_destroy(tsu_ctx* ctx) {
     if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     if (ctx->stream2) (void)hipStreamDestroy(ctx->stream2);
+    if (ctx->stream3) (void)hipStreamDestroy(ctx->stream3);
+    if (ctx->stream_io) (void)hipStreamDestroy(ctx->stream_io);
     delete ctx;
 }
 
@@ -689,9 +697,52 @@ static const tsu_tune_entry* tune_lookup(const tsu_ctx* ctx, int32_t bound) {
     return nullptr;
 }
 
-// Aligns the pairs listed in `sel` (pair indices) with per-bucket persistent
-// launches, widening the capacity of pairs that overflow.  Scores are left in
-// b->d_scores and copied to `scores` (full batch order) for the selected pairs.
+// Probe of the batched form: pairs whose bucket would run on a multi-wave
+// window are first aligned by the single-wave register kernel (512
+// diagonals) up to kProbeCap scaled scores.  Those that finish are done; for
+// the others, how far that score took them along both sequences (h + v)
+// predicts the final score, s_end ~ kProbeCap (p + t) / (h + v), and so the
+// diagonal range they need, about 2 s_end.  Each then runs in the narrowest
+// window predicted to hold it with kProbeMargin to spare, instead of its
+// bucket's widest: a low-error pair of 10 kbp runs in a window of 1,024
+// diagonals (several pairs per CU) rather than alone on a CU in 16,384.  The
+// prediction only places a pair; a pair that overflows is re-run wider, so
+// scores do not depend on it.  (On 400 pairs of 4-20 kbp at 1-15%, the ratio
+// of the true to the predicted score had median 1.01, 95th percentile 1.16,
+// maximum 1.29.)
+constexpr int32_t kProbeCap = 240;      // scaled scores (the window holds +-238 diagonals by then)
+constexpr double kProbeMargin = 1.2;
+
+// Capacity a probed pair gets: the predicted range with the margin, rounded
+// to the window of the kernel that will take it (64..512 one wave, 1024 two
+// waves, 4096 / 8192 / 16384 the 16-chunk kernel), so pairs fall into a few
+// launch groups.  Beyond 16,384 diagonals only the LDS kernel with its ring
+// in device memory remains, several times slower per diagonal: a pair goes
+// there directly only when even its unmargined prediction is well beyond
+// 16,384, and otherwise tries the 16-chunk kernel's widest window first.
+static int32_t probe_capacity(const tsu_ctx* ctx, const tsu_pair& pr, int32_t progress) {
+    const double s_end = (double)kProbeCap * (double)(pr.p_len + pr.t_len) / (double)std::max(progress, 1);
+    const double need = std::min(2.0 * (s_end - ctx->os) / ctx->es + 3.0, 1e9);
+    const int64_t w = (int64_t)std::ceil(kProbeMargin * need);
+    int64_t c;
+    if (w <= 512) {
+        for (c = 64; c < w; c *= 2) {}
+    } else if (w <= 1024) {
+        c = 1024;
+    } else if (w <= 16384 || need <= 1.25 * 16384) {
+        for (c = 4096; c < std::min<int64_t>(w, 16384); c *= 2) {}
+    } else {
+        c = (w + 16383) / 16384 * 16384;
+    }
+    // (a launch clamps it to what its longest pair can use)
+    return (int32_t)std::min<int64_t>(c, 1 << 26);
+}
+
+// Aligns the pairs listed in `sel` (pair indices) with persistent launches
+// per group of pairs (a length bucket, or a length bucket and predicted
+// window after the probe), widening the capacity of pairs that overflow.
+// Scores are left in b->d_scores and copied to `scores` (full batch order)
+// for the selected pairs.
 static int batched_core(tsu_ctx* ctx, tsu_batch* b, const std::vector<int32_t>& sel, int32_t* scores,
                         tsu_timing* t, bool use_tune) {
     const int nb = (int)(sizeof(kBucketBounds) / sizeof(kBucketBounds[0]));
@@ -708,134 +759,193 @@ static int batched_core(tsu_ctx* ctx, tsu_batch* b, const std::vector<int32_t>& 
     HIPCHK(d_order.alloc(std::max<size_t>(sel.size(), 1)), "hipMalloc (order)");
     double kms_total = 0;
     int launches = 0, retried = 0;
-    // every bucket ordered longest first, all ordered up front (one thread per
-    // bucket), so no sort sits between two launches
-#pragma omp parallel for schedule(dynamic, 1)
-    for (int bi = 0; bi < nb; ++bi) tsu_host::order_longest_first(buckets[(size_t)bi].idx, b->pairs.data());
     std::vector<int32_t> tmp((size_t)b->n);
-    // Launches `todo` at capacity W (widened while pairs overflow) and waits;
-    // scores go to `scores` at their pair index.
-    auto settle = [&](std::vector<int32_t> todo, int32_t W, int32_t max_len, int nw_hint) -> int {
-        for (int attempt = 0; !todo.empty(); ++attempt) {
-            W = std::min<int32_t>(W, capacity_full(max_len, max_len));
-            // the tuned shape is for the first attempt; wider re-runs take
-            // the default shape of their capacity
-            Shape sh = batched_shape(ctx, W, max_len, attempt == 0 ? nw_hint : 0);
-            HIPCHK(hipMemcpyAsync(d_order.p, todo.data(), todo.size() * sizeof(int32_t), hipMemcpyHostToDevice, ctx->stream),
-                   "hipMemcpyAsync (order)");
-            double kms = 0;
-            int r = run_shape(ctx, io_of(b), sh, W, d_order.p, (int32_t)todo.size(), max_len, 0, nullptr, &kms);
-            if (r != TSU_OK) return r;
-            kms_total += kms;
-            ++launches;
-            // scores live at their pair index
-            HIPCHK(hipMemcpy(tmp.data(), b->d_scores, (size_t)b->n * sizeof(int32_t), hipMemcpyDeviceToHost), "hipMemcpy D2H (scores)");
-            std::vector<int32_t> again;
-            for (int32_t i : todo) {
-                scores[i] = tmp[(size_t)i];
-                if (tmp[(size_t)i] == TSU_EKOVERFLOW) again.push_back(i);
-            }
-            if (!again.empty()) {
-                if (W >= capacity_full(max_len, max_len)) return fail(ctx, TSU_EINTERNAL, "overflow at full capacity");
-                retried += (int)again.size();
-                W = (int32_t)std::min<int64_t>(2ll * W, capacity_full(max_len, max_len));
-            }
-            todo.swap(again);
-        }
-        return TSU_OK;
+
+    // One launch group: pairs of one length bucket at one first capacity.
+    struct Group {
+        int32_t W;        // first capacity
+        int32_t max_len;
+        int nw_hint;      // tuned waves per pair (0: default shape)
+        std::vector<int32_t> idx;
     };
-    // The widened re-run of a bucket's overflowing pairs (a few long pairs,
-    // each on one CU for a long time) runs on the second stream beside the
-    // next bucket's launch instead of alone after it.
-    struct Deferred {
-        std::vector<int32_t> ids;
-        int32_t W, max_len;
-        DevBuf<int32_t> d_ids;
-        DevBuf<uint8_t> ring;
-        hipEvent_t e0 = nullptr, e1 = nullptr;
-        bool launched = false;
-        ~Deferred() { if (e0) (void)hipEventDestroy(e0); if (e1) (void)hipEventDestroy(e1); }
+    std::vector<Group> groups;
+    auto max_len_of = [&](const std::vector<int32_t>& ids) {
+        int32_t m = 0;
+        for (int32_t i : ids) m = std::max(m, std::max(b->pairs[(size_t)i].p_len, b->pairs[(size_t)i].t_len));
+        return m;
     };
-    std::vector<std::unique_ptr<Deferred>> deferred;
-    // on every return the second stream is drained before the re-runs' buffers are freed
-    struct Drain {
-        hipStream_t s;
-        ~Drain() { (void)hipStreamSynchronize(s); }
-    } drain{ctx->stream2};
-    int last = -1;
-    for (int bi = 0; bi < nb; ++bi)
-        if (!buckets[(size_t)bi].idx.empty()) { last = bi; break; }
-    // longest buckets first: their tails overlap nothing, so start them early
+    // buckets whose pairs are probed: the register kernels take them (2/3/1
+    // penalties), their first window would hold more than 1,024 diagonals and
+    // no capacity was tuned for them (a tuned capacity is used as given)
+    std::vector<int32_t> probe_ids;
+    std::vector<int> probed_bucket((size_t)nb, 0);
     for (int bi = nb - 1; bi >= 0; --bi) {
         Bucket& bk = buckets[(size_t)bi];
         if (bk.idx.empty()) continue;
-        int32_t max_len = 0;
-        for (int32_t i : bk.idx) max_len = std::max(max_len, std::max(b->pairs[(size_t)i].p_len, b->pairs[(size_t)i].t_len));
+        const int32_t max_len = max_len_of(bk.idx);
         const tsu_tune_entry* te = use_tune ? tune_lookup(ctx, bk.bound) : nullptr;
         int32_t W = (te && te->capacity > 0) ? te->capacity : initial_capacity(ctx, std::min(bk.bound, max_len));
         W = std::min<int32_t>(W, capacity_full(max_len, max_len));
         const int nw_hint = te ? te->waves_per_pair : 0;
-        if (bi == last) {  // nothing left to overlap with
-            int r = settle(bk.idx, W, max_len, nw_hint);
-            if (r != TSU_OK) return r;
+        if (tsu_host::fixed_242(ctx) && !(te && te->capacity > 0) && nw_hint <= 1 && W > 1024) {
+            probed_bucket[(size_t)bi] = 1;
+            probe_ids.insert(probe_ids.end(), bk.idx.begin(), bk.idx.end());
             continue;
         }
-        // first attempt, then its overflow re-run deferred to the second stream
-        std::vector<int32_t> first = bk.idx;
-        Shape sh = batched_shape(ctx, W, max_len, nw_hint);
-        HIPCHK(hipMemcpyAsync(d_order.p, first.data(), first.size() * sizeof(int32_t), hipMemcpyHostToDevice, ctx->stream),
-               "hipMemcpyAsync (order)");
+        groups.push_back(Group{W, max_len, nw_hint, bk.idx});
+    }
+    if (!probe_ids.empty()) {
+        tsu_host::order_longest_first(probe_ids, b->pairs.data());
+        const int32_t max_len = max_len_of(probe_ids);
+        DevBuf<int32_t> d_prog;
+        HIPCHK(d_prog.alloc(probe_ids.size() > 0 ? (size_t)b->n : 1), "hipMalloc (probe)");
+        KArgs pa = base_args(ctx);
+        pa.probe_cap = kProbeCap;
+        pa.probe_out = d_prog.p;
+        Shape ps{1, max_len <= 32000, true, false, true};
+        ps.fixed_242 = true;
+        ps.reg_c = 8;
+        ps.probe = true;
+        HIPCHK(hipMemcpyAsync(d_order.p, probe_ids.data(), probe_ids.size() * sizeof(int32_t), hipMemcpyHostToDevice,
+                              ctx->stream), "hipMemcpyAsync (order)");
         double kms = 0;
-        int r = run_shape(ctx, io_of(b), sh, W, d_order.p, (int32_t)first.size(), max_len, 0, nullptr, &kms);
+        int r = run_shape(ctx, io_of(b), ps, 512, d_order.p, (int32_t)probe_ids.size(), max_len, 0, &pa, &kms);
         if (r != TSU_OK) return r;
         kms_total += kms;
         ++launches;
+        std::vector<int32_t> prog((size_t)b->n, 0);
         HIPCHK(hipMemcpy(tmp.data(), b->d_scores, (size_t)b->n * sizeof(int32_t), hipMemcpyDeviceToHost), "hipMemcpy D2H (scores)");
-        std::vector<int32_t> again;
-        for (int32_t i : first) {
-            scores[i] = tmp[(size_t)i];
-            if (tmp[(size_t)i] == TSU_EKOVERFLOW) again.push_back(i);
+        HIPCHK(hipMemcpy(prog.data(), d_prog.p, (size_t)b->n * sizeof(int32_t), hipMemcpyDeviceToHost), "hipMemcpy D2H (probe)");
+        // the unfinished pairs, grouped by bucket and predicted capacity
+        std::map<std::pair<int, int32_t>, std::vector<int32_t>> by;
+        for (int bi = 0; bi < nb; ++bi) {
+            if (!probed_bucket[(size_t)bi]) continue;
+            for (int32_t i : buckets[(size_t)bi].idx) {
+                const int32_t v = tmp[(size_t)i];
+                if (v >= 0) { scores[i] = v; continue; }
+                if (v != tsu::kProbeStopped) {
+                    // the probe window overflowed: the bucket's own capacity
+                    const int32_t ml = std::max(b->pairs[(size_t)i].p_len, b->pairs[(size_t)i].t_len);
+                    by[{bi, std::min<int32_t>(initial_capacity(ctx, std::min(buckets[(size_t)bi].bound, ml)),
+                                              capacity_full(ml, ml))}].push_back(i);
+                    continue;
+                }
+                by[{bi, probe_capacity(ctx, b->pairs[(size_t)i], prog[(size_t)i])}].push_back(i);
+            }
         }
-        if (again.empty()) continue;
-        if (W >= capacity_full(max_len, max_len)) return fail(ctx, TSU_EINTERNAL, "overflow at full capacity");
-        retried += (int)again.size();
-        auto d = std::make_unique<Deferred>();
-        d->ids = std::move(again);
-        d->W = (int32_t)std::min<int64_t>(2ll * W, capacity_full(max_len, max_len));
-        d->max_len = max_len;
-        HIPCHK(d->d_ids.alloc(d->ids.size()), "hipMalloc (order)");
-        HIPCHK(hipEventCreate(&d->e0), "hipEventCreate");
-        HIPCHK(hipEventCreate(&d->e1), "hipEventCreate");
-        HIPCHK(hipMemcpyAsync(d->d_ids.p, d->ids.data(), d->ids.size() * sizeof(int32_t), hipMemcpyHostToDevice, ctx->stream2),
-               "hipMemcpyAsync (order)");
-        r = launch_shape(ctx, io_of(b), batched_shape(ctx, d->W, max_len, 0), d->W, d->d_ids.p, (int32_t)d->ids.size(), max_len, 0,
-                         nullptr, ctx->stream2, ctx->d_queue + 16, d->e0, d->e1, d->ring, &d->launched);
-        if (r != TSU_OK) { (void)hipStreamSynchronize(ctx->stream2); return r; }
-        ++launches;
-        deferred.push_back(std::move(d));
+        for (auto& kv : by) groups.push_back(Group{kv.first.second, max_len_of(kv.second), 0, std::move(kv.second)});
     }
-    // collect the deferred re-runs; pairs that overflow again widen further
-    HIPCHK(hipStreamSynchronize(ctx->stream2), "hipStreamSynchronize");
-    for (auto& d : deferred) {
-        if (d->launched) {
-            float ms = 0.f;
-            HIPCHK(hipEventElapsedTime(&ms, d->e0, d->e1), "hipEventElapsedTime");
-            kms_total += ms;
-        }
-    }
-    if (!deferred.empty())
-        HIPCHK(hipMemcpy(tmp.data(), b->d_scores, (size_t)b->n * sizeof(int32_t), hipMemcpyDeviceToHost), "hipMemcpy D2H (scores)");
-    for (auto& d : deferred) {
-        std::vector<int32_t> again;
-        for (int32_t i : d->ids) {
-            scores[i] = tmp[(size_t)i];
-            if (tmp[(size_t)i] == TSU_EKOVERFLOW) again.push_back(i);
-        }
-        if (again.empty()) continue;
-        if (d->W >= capacity_full(d->max_len, d->max_len)) return fail(ctx, TSU_EINTERNAL, "overflow at full capacity");
-        retried += (int)again.size();
-        int r = settle(again, (int32_t)std::min<int64_t>(2ll * d->W, capacity_full(d->max_len, d->max_len)), d->max_len, 0);
+    // widest capacity first (longest pairs first among equals): the long
+    // launches' tails overlap nothing, so start them early
+    std::stable_sort(groups.begin(), groups.end(), [](const Group& x, const Group& y) {
+        return x.W != y.W ? x.W > y.W : x.max_len > y.max_len;
+    });
+    // every group ordered longest first, all ordered up front (one thread per
+    // group), so no sort sits between two launches
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int gi = 0; gi < (int)groups.size(); ++gi) tsu_host::order_longest_first(groups[(size_t)gi].idx, b->pairs.data());
+    // Every group's first launch is issued at once, widest first, over three
+    // streams: a persistent launch holds the CUs until its queue drains, and
+    // the next launches' workgroups take the CUs it frees (or the room its
+    // workgroups leave on a CU), so the long launches' tails overlap the
+    // other groups' work instead of idling the chip between launches.  When
+    // a launch ends, its pairs that overflowed are re-launched at twice the
+    // capacity on the same stream while the others still run.
+    hipStream_t streams[3] = {ctx->stream, ctx->stream2, ctx->stream3};
+    struct Run {
+        std::vector<int32_t> ids;
+        int32_t W, max_len;
+        int nw_hint;
+        int32_t* order;      // this group's region of d_order
+        int32_t* counter;    // this group's work counter
+        hipStream_t stream;
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        DevBuf<uint8_t> ring;
+        bool done = false;
+        ~Run() { if (e0) (void)hipEventDestroy(e0); if (e1) (void)hipEventDestroy(e1); }
+    };
+    std::vector<std::unique_ptr<Run>> runs;
+    DevBuf<int32_t> d_counters;
+    HIPCHK(d_counters.alloc(std::max<size_t>(groups.size(), 1)), "hipMalloc (counters)");
+    // on every return the streams are drained before the launches' buffers are freed
+    struct Drain {
+        hipStream_t* s;
+        ~Drain() { for (int i = 0; i < 3; ++i) (void)hipStreamSynchronize(s[i]); }
+    } drain{streams};
+    std::vector<int32_t> all_ids;
+    all_ids.reserve(sel.size());
+    for (const Group& g : groups) all_ids.insert(all_ids.end(), g.idx.begin(), g.idx.end());
+    if (!all_ids.empty())
+        HIPCHK(hipMemcpy(d_order.p, all_ids.data(), all_ids.size() * sizeof(int32_t), hipMemcpyHostToDevice), "hipMemcpy (order)");
+    auto start_run = [&](std::unique_ptr<Run> rn) -> int {
+        HIPCHK(hipEventCreate(&rn->e0), "hipEventCreate");
+        HIPCHK(hipEventCreate(&rn->e1), "hipEventCreate");
+        bool launched = false;
+        const Shape sh = batched_shape(ctx, rn->W, rn->max_len, rn->nw_hint);
+        int r = launch_shape(ctx, io_of(b), sh, rn->W, rn->order, (int32_t)rn->ids.size(), rn->max_len, 0, nullptr,
+                             rn->stream, rn->counter, rn->e0, rn->e1, rn->ring, &launched);
         if (r != TSU_OK) return r;
+        if (!launched) return fail(ctx, TSU_EINTERNAL, "empty launch group");
+        ++launches;
+        runs.push_back(std::move(rn));
+        return TSU_OK;
+    };
+    size_t off = 0;
+    for (size_t gi = 0; gi < groups.size(); ++gi) {
+        Group& g = groups[gi];
+        auto rn = std::make_unique<Run>();
+        rn->max_len = g.max_len;
+        rn->W = std::min<int32_t>(g.W, capacity_full(g.max_len, g.max_len));
+        rn->nw_hint = g.nw_hint;
+        rn->order = d_order.p + off;
+        rn->counter = d_counters.p + gi;
+        rn->stream = streams[gi % 3];
+        off += g.idx.size();
+        rn->ids = std::move(g.idx);
+        int r = start_run(std::move(rn));
+        if (r != TSU_OK) return r;
+    }
+    for (size_t open = runs.size(); open > 0;) {
+        bool progressed = false;
+        for (size_t ri = 0; ri < runs.size(); ++ri) {
+            Run* rn = runs[ri].get();
+            if (rn->done) continue;
+            const hipError_t q = hipEventQuery(rn->e1);
+            if (q == hipErrorNotReady) continue;
+            if (q != hipSuccess) return hip_fail(ctx, q, "hipEventQuery");
+            rn->done = true;
+            --open;
+            progressed = true;
+            float ms = 0.f;
+            HIPCHK(hipEventElapsedTime(&ms, rn->e0, rn->e1), "hipEventElapsedTime");
+            kms_total += ms;
+            // scores live at their pair index
+            HIPCHK(hipMemcpyAsync(tmp.data(), b->d_scores, (size_t)b->n * sizeof(int32_t), hipMemcpyDeviceToHost,
+                                  ctx->stream_io), "hipMemcpyAsync D2H (scores)");
+            HIPCHK(hipStreamSynchronize(ctx->stream_io), "hipStreamSynchronize");
+            std::vector<int32_t> again;
+            for (int32_t i : rn->ids) {
+                scores[i] = tmp[(size_t)i];
+                if (tmp[(size_t)i] == TSU_EKOVERFLOW) again.push_back(i);
+            }
+            if (again.empty()) continue;
+            if (rn->W >= capacity_full(rn->max_len, rn->max_len)) return fail(ctx, TSU_EINTERNAL, "overflow at full capacity");
+            retried += (int)again.size();
+            auto nx = std::make_unique<Run>();
+            nx->W = (int32_t)std::min<int64_t>(2ll * rn->W, capacity_full(rn->max_len, rn->max_len));
+            nx->max_len = rn->max_len;
+            nx->nw_hint = 0;  // wider re-runs take the default shape of their capacity
+            nx->order = rn->order;      // the finished launch's region and counter
+            nx->counter = rn->counter;
+            nx->stream = rn->stream;
+            nx->ids = std::move(again);
+            HIPCHK(hipMemcpyAsync(nx->order, nx->ids.data(), nx->ids.size() * sizeof(int32_t), hipMemcpyHostToDevice,
+                                  nx->stream), "hipMemcpyAsync (order)");
+            int r = start_run(std::move(nx));
+            if (r != TSU_OK) return r;
+            ++open;
+        }
+        if (!progressed) std::this_thread::sleep_for(std::chrono::microseconds(50));
     }
     if (t) { t->kernel_ms += kms_total; t->launches += launches; t->retried += retried; }
     return TSU_OK;
@@ -1001,12 +1111,15 @@ extern "C" int tsu_autotune(tsu_ctx* ctx, tsu_batch* b, int32_t sample_pairs) {
         // Candidates: where the register kernels take the bucket's initial
         // capacity (4/6/2 penalties), that capacity and half of it (more
         // pairs per CU; pairs that overflow are re-run at the full one, and
-        // that time is counted); otherwise the LDS kernel with 1-16 waves.
+        // that time is counted), and, where that capacity needs several
+        // waves, capacity 0: each pair placed by the probe; otherwise the
+        // LDS kernel with 1-16 waves.
         int32_t max_len = 0;
         for (int32_t i : sample) max_len = std::max(max_len, std::max(b->pairs[(size_t)i].p_len, b->pairs[(size_t)i].t_len));
         const int32_t W0 = initial_capacity(ctx, std::min(kBucketBounds[bi], max_len));
         std::vector<tsu_tune_entry> cands;
         if (batched_shape(ctx, W0, max_len, 1).reg_c > 0) {
+            if (W0 > 1024) cands.push_back(tsu_tune_entry{1, 0, kBucketBounds[bi], 0});
             cands.push_back(tsu_tune_entry{1, W0, kBucketBounds[bi], 0});
             if (W0 / 2 >= 256) cands.push_back(tsu_tune_entry{1, W0 / 2, kBucketBounds[bi], 0});
         } else {
@@ -1019,13 +1132,17 @@ extern "C" int tsu_autotune(tsu_ctx* ctx, tsu_batch* b, int32_t sample_pairs) {
             ctx->tune.push_back(probe);
             tsu_timing tt;
             std::memset(&tt, 0, sizeof(tt));
+            // wall time: the launches of one call overlap, so the sum of
+            // their durations overstates a shape that splits its pairs
+            const double p0 = now_ms();
             int r = batched_core(ctx, b, sample, scratch.data(), &tt, true);
+            tt.wall_ms = now_ms() - p0;
             // a shape that cannot run here (no room in LDS, no occupancy, no
             // device memory for its ring) is skipped; any other failure is a
             // device or internal error and ends the call with no table
             if (r == TSU_EINVAL || r == TSU_ENOMEM) { (void)hipGetLastError(); continue; }
             if (r != TSU_OK) { ctx->tune.clear(); return r; }
-            if (tt.kernel_ms < best) { best = tt.kernel_ms; pick = &probe; }
+            if (tt.wall_ms < best) { best = tt.wall_ms; pick = &probe; }
         }
         if (pick) table.push_back(*pick);
     }

@@ -24,7 +24,9 @@ struct tsu_ctx {
     int32_t o2 = 0, e2 = 0;             // second gap piece as given
     int32_t o2s = 0, e2s = 0, r2 = 0;   // divided by gcd; I2/D2 ring depth
     hipStream_t stream = nullptr;
-    hipStream_t stream2 = nullptr;      // batched form: overflow re-runs beside the next launch
+    hipStream_t stream2 = nullptr;      // batched form: launches beside those on `stream`
+    hipStream_t stream3 = nullptr;      // batched form: a third stream of launches
+    hipStream_t stream_io = nullptr;    // batched form: score copies while launches run
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     int cu_count = 0;
     int64_t max_lds = 0;

@@ -69,7 +69,16 @@ struct KArgs {
     int32_t ck_cap;
     // neighbour-synchronised kernels: polls before a pair fails (0 = default)
     int32_t poll_limit;
+    // probe launches (single-wave kernel): stop after probe_cap scaled
+    // scores; a pair not finished by then gets kProbeStopped and, in
+    // probe_out, the largest h + v of its last scores
+    int32_t probe_cap;
+    int32_t* probe_out;
 };
+
+// Score slot of a pair a probe launch stopped before its end (internal: the
+// host reads it back and aligns the pair again; never returned to callers).
+constexpr int32_t kProbeStopped = -1000;
 
 struct Shape {
     int nw;          // waves per pair (1, 2, 4, 8, 16)
@@ -83,6 +92,7 @@ struct Shape {
     bool two_piece = false;  // gap-affine 2-piece kernel (one wave per pair, rings in LDS)
     bool ck = false;         // backtrace from ring checkpoints (bt and queue set, LDS kernel)
     bool dbl = false;        // 16-chunk kernel with doubled offsets (sequences up to kMlDblMaxLen bases)
+    bool probe = false;      // single-wave register kernel stopping at KArgs::probe_cap
 };
 
 // Doubled-offset 16-chunk kernel: longest sequence (2t fits int16) and the

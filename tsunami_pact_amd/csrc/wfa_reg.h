@@ -140,7 +140,11 @@ __device__ __forceinline__ int lcp_interleaved(const uint32_t* S, int v, int h, 
     return min(n2 >> 1, lim);
 }
 
-template <int C, int XS, int OS, int ES, bool QUEUE>
+// With PROBE the kernel stops after a.probe_cap scores: a pair that has not
+// reached its end cell by then gets kProbeStopped, and a.probe_out[pair]
+// receives the largest h + v over the valid M cells of its last RM scores
+// (how far along both sequences that score took it).
+template <int C, int XS, int OS, int ES, bool QUEUE, bool PROBE = false>
 __global__ __launch_bounds__(64) void wfa_reg_kernel(KArgs a) {
     static_assert(ES == 1, "in-place I/D needs e == 1 after gcd scaling");
     static_assert(C >= 1 && C <= 8, "window of 64..512 diagonals (8-bit chunk fields)");
@@ -221,6 +225,9 @@ __global__ __launch_bounds__(64) void wfa_reg_kernel(KArgs a) {
             constexpr int SO = (PH - OS - ES + 4 * RM) % RM;
             ++s;
             if (s > s_bound) return TSU_EINTERNAL;
+            if constexpr (PROBE) {
+                if (s > a.probe_cap) return kProbeStopped;
+            }
             // an opaque zero per step keeps the per-chunk bounds below from
             // being hoisted out of the score loop (16 VGPRs at C = 8): they
             // are recomputed, which costs less than the occupancy
@@ -332,6 +339,23 @@ __global__ __launch_bounds__(64) void wfa_reg_kernel(KArgs a) {
             }(std::make_integer_sequence<int, RM>{});
         }
         const int score = (r > 0 && s == 0) ? 0 : r;  // r > 0 is the score s of the last step
+        if constexpr (PROBE) {
+            if (r == kProbeStopped) {
+                // largest 2h - k = h + v of a valid M cell in the ring (the
+                // offsets are doubled: 2 M - k2 = 2 (h + v))
+                int best = -1;
+#pragma unroll
+                for (int c = 0; c < C; ++c) {
+                    const int k2 = 2 * (kmin + 64 * c) + lane2;
+#pragma unroll
+                    for (int q = 0; q < RM; ++q)
+                        if (Mr[q][c] >= 0) best = max(best, 2 * Mr[q][c] - k2);
+                }
+#pragma unroll
+                for (int d = 32; d >= 1; d >>= 1) best = max(best, __shfl_xor(best, d));
+                if (lane == 0) a.probe_out[pi] = best >> 1;
+            }
+        }
         if (lane == 0) a.scores[pi] = !lds_at_zero ? TSU_EINTERNAL : score < 0 ? score : score * a.g;
         __syncthreads();
     }

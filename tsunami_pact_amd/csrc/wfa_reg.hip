@@ -27,6 +27,7 @@ KernelFn pick_reg(const Shape& sh) {
         default: return nullptr;
         }
     }
+    if (sh.probe) return sh.reg_c == 8 && sh.queue ? wfa_reg_kernel<8, 2, 3, 1, true, true> : nullptr;
     switch (sh.reg_c) {
     case 1: return sh.queue ? wfa_reg_kernel<1, 2, 3, 1, true> : wfa_reg_kernel<1, 2, 3, 1, false>;
     case 2: return sh.queue ? wfa_reg_kernel<2, 2, 3, 1, true> : wfa_reg_kernel<2, 2, 3, 1, false>;
