@@ -710,7 +710,6 @@ static const tsu_tune_entry* tune_lookup(const tsu_ctx* ctx, int32_t bound) {
 // scores do not depend on it.  (On 400 pairs of 4-20 kbp at 1-15%, the ratio
 // of the true to the predicted score had median 1.01, 95th percentile 1.16,
 // maximum 1.29.)
-constexpr int32_t kProbeCap = 240;      // scaled scores (the window holds +-238 diagonals by then)
 constexpr double kProbeMargin = 1.2;
 
 // Capacity a probed pair gets: the predicted range with the margin, rounded
@@ -720,7 +719,7 @@ constexpr double kProbeMargin = 1.2;
 // in device memory remains, several times slower per diagonal: a pair goes
 // there directly only when even its unmargined prediction is well beyond
 // 16,384, and otherwise tries the 16-chunk kernel's widest window first.
-static int32_t probe_capacity(const tsu_ctx* ctx, const tsu_pair& pr, int32_t progress) {
+int32_t tsu_host::probe_capacity(const tsu_ctx* ctx, const tsu_pair& pr, int32_t progress) {
     const double s_end = (double)kProbeCap * (double)(pr.p_len + pr.t_len) / (double)std::max(progress, 1);
     const double need = std::min(2.0 * (s_end - ctx->os) / ctx->es + 3.0, 1e9);
     const int64_t w = (int64_t)std::ceil(kProbeMargin * need);
@@ -787,7 +786,7 @@ static int batched_core(tsu_ctx* ctx, tsu_batch* b, const std::vector<int32_t>& 
         int32_t W = (te && te->capacity > 0) ? te->capacity : initial_capacity(ctx, std::min(bk.bound, max_len));
         W = std::min<int32_t>(W, capacity_full(max_len, max_len));
         const int nw_hint = te ? te->waves_per_pair : 0;
-        if (tsu_host::fixed_242(ctx) && !(te && te->capacity > 0) && nw_hint <= 1 && W > 1024) {
+        if (tsu_host::probe_bucket(ctx, W, te ? te->capacity : 0, nw_hint)) {
             probed_bucket[(size_t)bi] = 1;
             probe_ids.insert(probe_ids.end(), bk.idx.begin(), bk.idx.end());
             continue;
@@ -800,7 +799,7 @@ static int batched_core(tsu_ctx* ctx, tsu_batch* b, const std::vector<int32_t>& 
         DevBuf<int32_t> d_prog;
         HIPCHK(d_prog.alloc(probe_ids.size() > 0 ? (size_t)b->n : 1), "hipMalloc (probe)");
         KArgs pa = base_args(ctx);
-        pa.probe_cap = kProbeCap;
+        pa.probe_cap = tsu_host::kProbeCap;
         pa.probe_out = d_prog.p;
         Shape ps{1, max_len <= 32000, true, false, true};
         ps.fixed_242 = true;
@@ -830,7 +829,7 @@ static int batched_core(tsu_ctx* ctx, tsu_batch* b, const std::vector<int32_t>& 
                                               capacity_full(ml, ml))}].push_back(i);
                     continue;
                 }
-                by[{bi, probe_capacity(ctx, b->pairs[(size_t)i], prog[(size_t)i])}].push_back(i);
+                by[{bi, tsu_host::probe_capacity(ctx, b->pairs[(size_t)i], prog[(size_t)i])}].push_back(i);
             }
         }
         for (auto& kv : by) groups.push_back(Group{kv.first.second, max_len_of(kv.second), 0, std::move(kv.second)});

@@ -1,24 +1,32 @@
 // serve.cpp -- continuous-batching alignment server on one GPU.
 //
 // Requests (1..N pairs each) are packed on the submitting thread; their pairs
-// join one FIFO queue per length bucket.  One worker thread owns the device
-// side:
-//   - a launch slot that frees up takes the bucket whose oldest waiting pair
-//     is the oldest, and merges up to max_batch_pairs waiting pairs of that
-//     bucket from any number of requests into one flight: their sequences
-//     are copied into one arena, one persistent queue-fed launch aligns them,
-//     the scores come back async;
+// join FIFO queues keyed by length bucket and capacity.  One worker thread
+// owns the device side:
+//   - pairs of long-pair buckets (whose first capacity needs several waves
+//     per pair) first go through the probe stage: the waiting ones are
+//     aligned together by the single-wave kernel up to a score cap on a
+//     stream of their own (tsu_host::probe_capacity); those that finish are
+//     done, the others join the queue of their bucket at the capacity the
+//     probe predicts, so a low-error long pair runs in a narrow window
+//     (several pairs per CU) rather than alone on a CU in the widest one;
+//   - a launch slot that frees up takes the queue whose oldest waiting pair
+//     is the oldest, and merges up to max_batch_pairs of its pairs from any
+//     number of requests into one flight: their sequences are copied into
+//     one arena, one persistent queue-fed launch aligns them, the scores come
+//     back async;
 //   - device memory is one pre-allocated region of hbm_budget_bytes carved
-//     by a first-fit allocator; a flight takes fewer pairs (or waits) when
-//     its bytes (sequences, descriptors, scores, order, ring spill) do not
-//     fit what is free;
-//   - up to max_inflight flights run at once on separate streams, so pairs
-//     that arrive while others are on the GPU are launched as soon as a slot
-//     and memory free up, and launches of different buckets overlap;
+//     by a first-fit allocator; a flight (or a probe batch) takes fewer pairs
+//     (or waits) when its bytes (sequences, descriptors, scores, order, ring
+//     spill) do not fit what is free;
+//   - up to max_inflight flights run at once on separate streams, beside the
+//     probe stage, so pairs that arrive while others are on the GPU are
+//     launched as soon as a slot and memory free up, and launches of
+//     different queues overlap;
 //   - a finished flight scatters its scores to the requests (a request is
 //     done when its last pair is) and releases its memory; pairs whose
-//     diagonal range overflowed are re-launched wider on the same stream
-//     first.
+//     diagonal range overflowed are re-launched at twice their capacity on
+//     the same stream first.
 // Each pair's result depends only on that pair, so a request's scores are the
 // same alone and in any mix.
 #include <algorithm>
@@ -89,11 +97,16 @@ struct Req {
     bool done = false;
 };
 
-// One waiting (or in-flight) pair: pair i of request r.
+// One waiting (or in-flight) pair: pair i of request r, to be aligned at
+// capacity W (0: its length bucket's first capacity).
 struct Entry {
     Req* r;
     int32_t i;
+    int32_t W;
 };
+
+// Launches one flight holds at once (length buckets, or probe windows).
+constexpr int kMaxLaunches = 16;
 
 struct LaunchPlan {
     int32_t first = 0, count = 0;  // range of the flight's order array
@@ -101,6 +114,7 @@ struct LaunchPlan {
     int32_t max_len = 0;
     Shape sh{};
     int64_t grid = 0;
+    bool probe = false;
 };
 
 struct Slot {
@@ -115,6 +129,8 @@ struct Slot {
     int32_t* d_scores = nullptr;
     int32_t* d_order = nullptr;
     int32_t* d_queue = nullptr;  // one counter per launch
+    int32_t* d_probe = nullptr;  // probe launch: furthest h + v per pair
+    bool probing = false;        // this slot runs the probe stage
     uint8_t* d_ring = nullptr;
     int64_t ring_bytes = 0;
     std::vector<LaunchPlan> plan;
@@ -125,6 +141,7 @@ struct Slot {
     tsu_pair* h_pairs = nullptr; int64_t h_pairs_cap = 0;
     int32_t* h_order = nullptr; int64_t h_order_cap = 0;
     int32_t* h_scores = nullptr; int64_t h_scores_cap = 0;
+    int32_t* h_probe = nullptr; int64_t h_probe_cap = 0;
 };
 
 template <typename T>
@@ -146,8 +163,10 @@ struct tsu_server {
     tsu_serve_cfg cfg{};
     std::mutex mu;
     std::condition_variable cv;
-    std::vector<std::deque<Entry>> pending;  // waiting pairs per length bucket, FIFO
-    int64_t n_pending = 0;
+    std::map<int64_t, std::deque<Entry>> pending;  // waiting pairs per (bucket, capacity), FIFO
+    std::deque<Entry> probe_wait;                  // long pairs waiting for the probe stage
+    int64_t n_pending = 0;                         // pairs in pending and probe_wait
+    Slot probe_slot;
     std::unordered_map<uint64_t, Req*> reqs;
     std::vector<Slot> slots;
     Pool pool;
@@ -160,59 +179,81 @@ struct tsu_server {
 
 namespace {
 
-// Plans the launches of a flight and returns its device bytes.
-int64_t plan_flight(tsu_server* s, Slot& sl, int* status) {
+// Queue key of a pair: its length bucket and capacity (0: the bucket's).
+int64_t queue_key(int bucket, int32_t W) { return ((int64_t)bucket << 32) | (uint32_t)W; }
+
+// First capacity and tuned waves of a length bucket for pairs up to max_len.
+int32_t bucket_capacity(const tsu_ctx* ctx, int bucket, int32_t max_len, int* waves) {
+    const int32_t bound = tsu_host::bucket_bound(bucket);
+    const int32_t tcap = tsu_host::tuned_capacity(ctx, bound);
+    if (waves) *waves = tsu_host::tuned_waves(ctx, bound);
+    return std::min(tcap > 0 ? tcap : tsu_host::initial_capacity(ctx, std::min(bound, max_len)),
+                    tsu_host::capacity_full(max_len, max_len));
+}
+
+// Pairs of this bucket go through the probe stage first.
+bool probed_bucket(const tsu_ctx* ctx, int bucket, int32_t max_len) {
+    const int32_t bound = tsu_host::bucket_bound(bucket);
+    int waves = 0;
+    const int32_t W = bucket_capacity(ctx, bucket, max_len, &waves);
+    return tsu_host::probe_bucket(ctx, W, tsu_host::tuned_capacity(ctx, bound), waves);
+}
+
+// Plans the flight's one launch (its pairs share one queue: one length
+// bucket, one capacity; or the probe) and returns its device bytes.  The
+// ring region covers the launch and every re-launch of overflowing pairs up
+// to the full capacity of the longest pair, so widening never fails.
+int64_t plan_flight(tsu_server* s, Slot& sl, int32_t W_entry, int* status) {
     tsu_ctx* ctx = s->ctx;
-    const int nb = tsu_host::bucket_count();
-    std::vector<std::vector<int32_t>> by((size_t)nb);
-    for (int32_t i = 0; i < sl.n; ++i) {
-        const tsu_pair& pr = sl.fpairs[(size_t)i];
-        by[(size_t)tsu_host::bucket_of(std::max(pr.p_len, pr.t_len))].push_back(i);
-    }
-    sl.order.clear();
+    sl.order.resize((size_t)sl.n);
+    for (int32_t i = 0; i < sl.n; ++i) sl.order[(size_t)i] = i;
+    tsu_host::order_longest_first(sl.order, sl.fpairs.data());
     sl.plan.clear();
-    int64_t ring = 0;
-    for (int bi = nb - 1; bi >= 0; --bi) {
-        std::vector<int32_t>& v = by[(size_t)bi];
-        if (v.empty()) continue;
-        tsu_host::order_longest_first(v, sl.fpairs.data());
-        LaunchPlan lp;
-        lp.first = (int32_t)sl.order.size();
-        lp.count = (int32_t)v.size();
-        for (int32_t i : v) lp.max_len = std::max(lp.max_len, std::max(sl.fpairs[(size_t)i].p_len, sl.fpairs[(size_t)i].t_len));
-        const int32_t bound = tsu_host::bucket_bound(bi);
-        const int32_t tcap = tsu_host::tuned_capacity(ctx, bound);
-        lp.W = std::min(tcap > 0 ? tcap : tsu_host::initial_capacity(ctx, std::min(bound, lp.max_len)),
-                        tsu_host::capacity_full(lp.max_len, lp.max_len));
-        lp.sh = tsu_host::batched_shape(ctx, lp.W, lp.max_len, tsu_host::tuned_waves(ctx, bound));
-        KArgs a = tsu_host::base_args(ctx);
-        a.W = lp.W;
-        a.seq_words = tsu_host::seq_words_for(lp.max_len);
-        lp.grid = std::min<int64_t>(tsu_host::resident_grid(ctx, lp.sh, a, status), lp.count);
-        if (*status != 0 || lp.grid <= 0) { *status = TSU_EINTERNAL; return -1; }
-        if (!lp.sh.lds_ring) ring = std::max(ring, tsu::ring_bytes(lp.sh, lp.W, ctx->rm, ctx->ri) * lp.grid);
-        // the first re-launch of pairs that overflow (retire: twice the
-        // capacity) may need a ring in HBM: reserve it with the flight
-        const int32_t W2 = std::min<int32_t>(2 * lp.W, tsu_host::capacity_full(lp.max_len, lp.max_len));
-        if (W2 > lp.W) {
-            const Shape sh2 = tsu_host::batched_shape(ctx, W2, lp.max_len, 0);
-            if (!sh2.lds_ring) {
-                KArgs a2 = a;
-                a2.W = W2;
-                const int64_t g2 = std::min<int64_t>(tsu_host::resident_grid(ctx, sh2, a2, status), lp.count);
-                if (*status != 0 || g2 <= 0) { *status = TSU_EINTERNAL; return -1; }
-                ring = std::max(ring, tsu::ring_bytes(sh2, W2, ctx->rm, ctx->ri) * g2);
-            }
-        }
-        sl.order.insert(sl.order.end(), v.begin(), v.end());
-        sl.plan.push_back(lp);
+    LaunchPlan lp;
+    lp.first = 0;
+    lp.count = sl.n;
+    for (const tsu_pair& pr : sl.fpairs) lp.max_len = std::max(lp.max_len, std::max(pr.p_len, pr.t_len));
+    const int bucket = tsu_host::bucket_of(lp.max_len);
+    int waves = 0;
+    const int32_t W0 = bucket_capacity(ctx, bucket, lp.max_len, &waves);
+    const int32_t full = tsu_host::capacity_full(lp.max_len, lp.max_len);
+    KArgs a = tsu_host::base_args(ctx);
+    a.seq_words = tsu_host::seq_words_for(lp.max_len);
+    if (sl.probing) {
+        lp.probe = true;
+        lp.W = 512;
+        lp.sh = Shape{1, lp.max_len <= 32000, true, false, true};
+        lp.sh.fixed_242 = true;
+        lp.sh.reg_c = 8;
+        lp.sh.probe = true;
+    } else {
+        lp.W = W_entry > 0 ? std::min(W_entry, full) : W0;
+        lp.sh = tsu_host::batched_shape(ctx, lp.W, lp.max_len, W_entry > 0 ? 0 : waves);
     }
+    a.W = lp.W;
+    lp.grid = std::min<int64_t>(tsu_host::resident_grid(ctx, lp.sh, a, status), lp.count);
+    if (*status != 0 || lp.grid <= 0) { *status = TSU_EINTERNAL; return -1; }
+    int64_t ring = 0;
+    if (!sl.probing) {
+        if (!lp.sh.lds_ring) ring = tsu::ring_bytes(lp.sh, lp.W, ctx->rm, ctx->ri) * lp.grid;
+        for (int32_t W2 = lp.W; W2 < full;) {
+            W2 = (int32_t)std::min<int64_t>(2ll * W2, full);
+            const Shape sh2 = tsu_host::batched_shape(ctx, W2, lp.max_len, 0);
+            if (sh2.lds_ring) continue;
+            KArgs a2 = a;
+            a2.W = W2;
+            const int64_t g2 = std::min<int64_t>(tsu_host::resident_grid(ctx, sh2, a2, status), lp.count);
+            if (*status != 0 || g2 <= 0) { *status = TSU_EINTERNAL; return -1; }
+            ring = std::max(ring, tsu::ring_bytes(sh2, W2, ctx->rm, ctx->ri) * g2);
+        }
+    }
+    sl.plan.push_back(lp);
     sl.ring_bytes = ring;
     int64_t words = 0;
     for (const tsu_pair& pr : sl.fpairs) words = std::max<int64_t>(words, std::max(pr.p_off + tsu_packed_words(pr.p_len), pr.t_off + tsu_packed_words(pr.t_len)));
     *status = 0;
-    return align256(words * 4) + align256((int64_t)sl.n * (int64_t)sizeof(KPair)) + 2 * align256((int64_t)sl.n * 4) +
-           align256((int64_t)sl.plan.size() * 4 + 4) + align256(ring);
+    return align256(words * 4) + align256((int64_t)sl.n * (int64_t)sizeof(KPair)) + 3 * align256((int64_t)sl.n * 4) +
+           align256((int64_t)kMaxLaunches * 4 + 4) + align256(ring);
 }
 
 int launch_plan(tsu_server* s, Slot& sl) {
@@ -228,6 +269,10 @@ int launch_plan(tsu_server* s, Slot& sl) {
         a.W = lp.W;
         a.seq_words = tsu_host::seq_words_for(lp.max_len);
         a.queue = sl.d_queue + li;
+        if (lp.probe) {
+            a.probe_cap = tsu_host::kProbeCap;
+            a.probe_out = sl.d_probe;
+        }
         if (!lp.sh.lds_ring) {
             a.ring_g = sl.d_ring;
             a.ring_stride = tsu::ring_bytes(lp.sh, lp.W, ctx->rm, ctx->ri);
@@ -261,21 +306,27 @@ void fail_flight(tsu_server* s, Slot& sl, int status) {
     s->cv.notify_all();
 }
 
-// Builds and launches a flight in slot sl: the bucket whose oldest waiting
-// pair is the oldest, up to max_batch_pairs of its waiting pairs, fewer when
-// the flight would not fit the free memory.  Returns true when something
-// was launched.
+// Builds and launches a flight in slot sl: from the queue whose oldest
+// waiting pair is the oldest (or, for the probe slot, from the pairs waiting
+// for the probe), up to max_batch_pairs of its pairs, fewer when the flight
+// would not fit the free memory.  Returns true when something was launched.
 bool admit(tsu_server* s, Slot& sl) {
     sl.ents.clear();
     sl.fpairs.clear();
-    int bsel = -1;
-    for (int b = 0; b < (int)s->pending.size(); ++b) {
-        if (s->pending[(size_t)b].empty()) continue;
-        if (bsel < 0 || s->pending[(size_t)b].front().r->submit_ns <= s->pending[(size_t)bsel].front().r->submit_ns) bsel = b;
+    std::deque<Entry>* qp = nullptr;
+    auto best = s->pending.end();
+    if (sl.probing) {
+        if (!s->probe_wait.empty()) qp = &s->probe_wait;
+    } else {
+        for (auto it = s->pending.begin(); it != s->pending.end(); ++it) {
+            if (it->second.empty()) continue;
+            if (!qp || it->second.front().r->submit_ns < qp->front().r->submit_ns) { qp = &it->second; best = it; }
+        }
     }
-    if (bsel < 0) return false;
-    std::deque<Entry>& q = s->pending[(size_t)bsel];
-    int32_t take = (int32_t)std::min<int64_t>((int64_t)q.size(), s->cfg.max_batch_pairs);
+    if (!qp) return false;
+    std::deque<Entry>& q = *qp;
+    const int32_t W_entry = q.front().W;
+    int32_t take = (int32_t)std::min<int64_t>((int64_t)q.size(), sl.probing ? 4 * s->cfg.max_batch_pairs : s->cfg.max_batch_pairs);
     int64_t words = 0, bytes = 0;
     for (;;) {
         // flight arena: each pair's pattern and text words, in order
@@ -293,7 +344,7 @@ bool admit(tsu_server* s, Slot& sl) {
         }
         sl.n = take;
         int st = 0;
-        bytes = plan_flight(s, sl, &st);
+        bytes = plan_flight(s, sl, W_entry, &st);
         if (st != 0 || bytes < 0) {
             s->fatal = TSU_EINTERNAL;
             return false;
@@ -309,6 +360,7 @@ bool admit(tsu_server* s, Slot& sl) {
     sl.ents.assign(q.begin(), q.begin() + take);
     q.erase(q.begin(), q.begin() + take);
     s->n_pending -= take;
+    if (q.empty() && best != s->pending.end()) s->pending.erase(best);
     const int32_t n = take;
     sl.bytes = bytes;
     sl.active = false;
@@ -318,12 +370,13 @@ bool admit(tsu_server* s, Slot& sl) {
     sl.d_pairs = reinterpret_cast<KPair*>(carve((int64_t)n * (int64_t)sizeof(KPair)));
     sl.d_scores = reinterpret_cast<int32_t*>(carve((int64_t)n * 4));
     sl.d_order = reinterpret_cast<int32_t*>(carve((int64_t)n * 4));
-    sl.d_queue = reinterpret_cast<int32_t*>(carve((int64_t)sl.plan.size() * 4 + 4));
+    sl.d_probe = reinterpret_cast<int32_t*>(carve((int64_t)n * 4));
+    sl.d_queue = reinterpret_cast<int32_t*>(carve((int64_t)kMaxLaunches * 4 + 4));
     sl.d_ring = sl.ring_bytes ? carve(sl.ring_bytes) : nullptr;
     // stage into pinned memory, copy, launch, copy back
     if (!grow_pinned(sl.h_words, sl.h_words_cap, std::max<int64_t>(words, 1)) ||
         !grow_pinned(sl.h_pairs, sl.h_pairs_cap, n) || !grow_pinned(sl.h_order, sl.h_order_cap, n) ||
-        !grow_pinned(sl.h_scores, sl.h_scores_cap, n)) {
+        !grow_pinned(sl.h_scores, sl.h_scores_cap, n) || !grow_pinned(sl.h_probe, sl.h_probe_cap, n)) {
         s->fatal = TSU_ENOMEM;
         fail_flight(s, sl, TSU_ENOMEM);
         return false;
@@ -344,8 +397,10 @@ bool admit(tsu_server* s, Slot& sl) {
               hipMemcpyAsync(sl.d_pairs, sl.h_pairs, (size_t)n * sizeof(KPair), hipMemcpyHostToDevice, sl.stream) == hipSuccess &&
               hipMemcpyAsync(sl.d_order, sl.h_order, (size_t)n * 4, hipMemcpyHostToDevice, sl.stream) == hipSuccess;
     ok = ok && launch_plan(s, sl) == TSU_OK;
-    ok = ok && hipMemcpyAsync(sl.h_scores, sl.d_scores, (size_t)n * 4, hipMemcpyDeviceToHost, sl.stream) == hipSuccess &&
-         hipEventRecord(sl.ev, sl.stream) == hipSuccess;
+    ok = ok && hipMemcpyAsync(sl.h_scores, sl.d_scores, (size_t)n * 4, hipMemcpyDeviceToHost, sl.stream) == hipSuccess;
+    if (sl.probing)
+        ok = ok && hipMemcpyAsync(sl.h_probe, sl.d_probe, (size_t)n * 4, hipMemcpyDeviceToHost, sl.stream) == hipSuccess;
+    ok = ok && hipEventRecord(sl.ev, sl.stream) == hipSuccess;
     if (!ok) {
         // nothing of this flight may still run when its memory is reused
         (void)hipStreamSynchronize(sl.stream);
@@ -357,36 +412,60 @@ bool admit(tsu_server* s, Slot& sl) {
     return true;
 }
 
-// A flight finished: hand out scores or re-launch overflowed pairs wider.
+// The probe batch finished: pairs it aligned are done; the others join the
+// queue of their bucket at the predicted capacity (at the bucket's first
+// capacity when the probe window overflowed).
+void retire_probe(tsu_server* s, Slot& sl) {
+    tsu_ctx* ctx = s->ctx;
+    const int64_t now = mono_ns();
+    for (int32_t k = 0; k < sl.n; ++k) {
+        const Entry& en = sl.ents[(size_t)k];
+        const int32_t v = sl.h_scores[k];
+        if (v >= 0) { finish_entry(en.r, en.i, v, now); continue; }
+        const tsu_pair& pr = sl.fpairs[(size_t)k];
+        const int bucket = tsu_host::bucket_of(std::max(pr.p_len, pr.t_len));
+        const int32_t W = v == tsu::kProbeStopped ? tsu_host::probe_capacity(ctx, pr, sl.h_probe[k]) : 0;
+        s->pending[queue_key(bucket, W)].push_back(Entry{en.r, en.i, W});
+        ++s->n_pending;
+    }
+    s->pool.release(sl.off, sl.bytes);
+    sl.off = -1;
+    sl.ents.clear();
+    sl.active = false;
+    s->cv.notify_all();
+}
+
+// A flight's launch finished: pairs that overflowed go again at twice the
+// capacity they had, on the same stream; when none is left, the scores are
+// handed out and the memory released.
 void retire(tsu_server* s, Slot& sl) {
+    if (sl.probing) { retire_probe(s, sl); return; }
     tsu_ctx* ctx = s->ctx;
     std::vector<int32_t> again;
-    for (int32_t i = 0; i < sl.n; ++i)
-        if (sl.h_scores[i] == TSU_EKOVERFLOW) again.push_back(i);
-    if (!again.empty()) {
-        // one wider launch over the overflowed pairs, ring in LDS or in the
-        // flight's ring region; fall back to failing them if neither fits
-        int32_t max_len = 0;
-        int32_t W = 0;
-        for (int32_t i : again) max_len = std::max(max_len, std::max(sl.fpairs[(size_t)i].p_len, sl.fpairs[(size_t)i].t_len));
-        for (const LaunchPlan& lp : sl.plan) W = std::max(W, lp.W);
-        const bool can_widen = W < tsu_host::capacity_full(max_len, max_len);
-        W = std::min<int32_t>(2 * W, tsu_host::capacity_full(max_len, max_len));
+    int32_t max_len = 0;
+    for (int32_t i = 0; i < sl.n; ++i) {
+        if (sl.h_scores[i] != TSU_EKOVERFLOW) continue;
+        again.push_back(i);
+        max_len = std::max(max_len, std::max(sl.fpairs[(size_t)i].p_len, sl.fpairs[(size_t)i].t_len));
+    }
+    const int32_t W = sl.plan.empty() ? 0 : sl.plan[0].W;
+    if (!again.empty() && W < tsu_host::capacity_full(max_len, max_len)) {
         LaunchPlan lp;
         lp.first = 0;
         lp.count = (int32_t)again.size();
-        lp.W = W;
+        lp.W = (int32_t)std::min<int64_t>(2ll * W, tsu_host::capacity_full(max_len, max_len));
         lp.max_len = max_len;
-        lp.sh = tsu_host::batched_shape(ctx, W, max_len, 0);
+        lp.sh = tsu_host::batched_shape(ctx, lp.W, max_len, 0);
         KArgs a = tsu_host::base_args(ctx);
-        a.W = W;
+        a.W = lp.W;
         a.seq_words = tsu_host::seq_words_for(max_len);
         int st = 0;
         lp.grid = std::min<int64_t>(tsu_host::resident_grid(ctx, lp.sh, a, &st), lp.count);
+        // the flight reserved the ring of every widening up to full capacity
         if (!lp.sh.lds_ring && st == 0)
-            lp.grid = std::min<int64_t>(lp.grid, sl.ring_bytes / std::max<int64_t>(1, tsu::ring_bytes(lp.sh, W, ctx->rm, ctx->ri)));
-        const bool widened = can_widen && st == 0 && lp.grid > 0;
-        if (widened) {
+            lp.grid = std::min<int64_t>(lp.grid, sl.ring_bytes / std::max<int64_t>(1, tsu::ring_bytes(lp.sh, lp.W, ctx->rm, ctx->ri)));
+        if (st == 0 && lp.grid > 0) {
+            sl.order = again;
             std::memcpy(sl.h_order, again.data(), again.size() * 4);
             sl.plan.assign(1, lp);
             bool ok = hipMemcpyAsync(sl.d_order, sl.h_order, again.size() * 4, hipMemcpyHostToDevice, sl.stream) == hipSuccess;
@@ -396,6 +475,8 @@ void retire(tsu_server* s, Slot& sl) {
             if (ok) return;  // still in flight
             (void)hipStreamSynchronize(sl.stream);  // before the memory is reused
             s->fatal = TSU_EHIP;
+        } else if (st != 0) {
+            s->fatal = TSU_EINTERNAL;
         }
     }
     const int64_t now = mono_ns();
@@ -407,14 +488,22 @@ void retire(tsu_server* s, Slot& sl) {
     s->cv.notify_all();
 }
 
+// The flight slots and the probe slot.
+std::vector<Slot*> all_slots(tsu_server* s) {
+    std::vector<Slot*> v;
+    for (Slot& sl : s->slots) v.push_back(&sl);
+    v.push_back(&s->probe_slot);
+    return v;
+}
+
 void worker_loop(tsu_server* s) {
     (void)hipSetDevice(s->ctx->device);
     std::unique_lock<std::mutex> lk(s->mu);
     for (;;) {
         bool progressed = false;
         bool any_active = false;
-        for (Slot& sl : s->slots) {
-            if (!sl.active) continue;
+        auto poll = [&](Slot& sl) {
+            if (!sl.active) return;
             const hipError_t q = hipEventQuery(sl.ev);
             if (q == hipSuccess) { retire(s, sl); progressed = true; }
             else if (q != hipErrorNotReady) {
@@ -424,10 +513,17 @@ void worker_loop(tsu_server* s) {
                 progressed = true;
             }
             if (sl.active) any_active = true;
-        }
+        };
+        poll(s->probe_slot);
+        for (Slot& sl : s->slots) poll(sl);
         if (s->fatal == TSU_OK) {
+            // the probe stage first: its pairs feed the queues the slots take from
+            if (!s->probe_slot.active && !s->probe_wait.empty() && admit(s, s->probe_slot)) {
+                progressed = true;
+                any_active = true;
+            }
             for (Slot& sl : s->slots) {
-                if (sl.active || s->n_pending == 0) continue;
+                if (sl.active || s->pending.empty()) continue;
                 if (admit(s, sl)) { progressed = true; any_active = true; }
                 else break;
             }
@@ -435,10 +531,12 @@ void worker_loop(tsu_server* s) {
         if (s->fatal != TSU_OK) {
             // fail everything still queued; in-flight work is waited for below
             const int64_t now = mono_ns();
-            for (std::deque<Entry>& q : s->pending) {
-                for (const Entry& en : q) finish_entry(en.r, en.i, s->fatal, now);
-                q.clear();
+            for (auto& kv : s->pending) {
+                for (const Entry& en : kv.second) finish_entry(en.r, en.i, s->fatal, now);
             }
+            s->pending.clear();
+            for (const Entry& en : s->probe_wait) finish_entry(en.r, en.i, s->fatal, now);
+            s->probe_wait.clear();
             s->n_pending = 0;
             s->cv.notify_all();
         }
@@ -468,11 +566,11 @@ extern "C" int tsu_serve_open(tsu_ctx* ctx, const tsu_serve_cfg* cfg, tsu_server
         return tsu_host::fail(ctx, TSU_ENOMEM, "tsu_serve_open: cannot reserve the HBM budget");
     }
     s->pool.init(s->d_base, cfg->hbm_budget_bytes);
-    s->pending.resize((size_t)tsu_host::bucket_count());
     s->slots.resize((size_t)cfg->max_inflight);
-    for (Slot& sl : s->slots) {
-        if (hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking) != hipSuccess ||
-            hipEventCreateWithFlags(&sl.ev, hipEventDisableTiming) != hipSuccess) {
+    s->probe_slot.probing = true;
+    for (Slot* sl : all_slots(s)) {
+        if (hipStreamCreateWithFlags(&sl->stream, hipStreamNonBlocking) != hipSuccess ||
+            hipEventCreateWithFlags(&sl->ev, hipEventDisableTiming) != hipSuccess) {
             tsu_serve_close(s);
             return tsu_host::fail(ctx, TSU_EHIP, "tsu_serve_open: stream creation");
         }
@@ -553,12 +651,13 @@ static int enqueue(tsu_server* s, Req* r, uint64_t* ticket) {
         r->done = true;
         r->admit_ns = r->done_ns = mono_ns();
     } else {
-        // a request that can never fit the budget is refused up front
-        Slot probe;
-        probe.fpairs = r->pairs;
-        probe.n = n;
+        // a request that can never fit the budget is refused up front: a
+        // flight of all its pairs at once must fit
+        Slot trial;
+        trial.fpairs = r->pairs;
+        trial.n = n;
         int st = 0;
-        const int64_t need = plan_flight(s, probe, &st);
+        const int64_t need = plan_flight(s, trial, 0, &st);
         if (st != 0 || need > s->pool.size) {
             s->reqs.erase(r->ticket);
             delete r;
@@ -568,7 +667,10 @@ static int enqueue(tsu_server* s, Req* r, uint64_t* ticket) {
         r->remaining = n;
         for (int32_t i = 0; i < n; ++i) {
             const tsu_pair& pr = r->pairs[(size_t)i];
-            s->pending[(size_t)tsu_host::bucket_of(std::max(pr.p_len, pr.t_len))].push_back(Entry{r, i});
+            const int32_t ml = std::max(pr.p_len, pr.t_len);
+            const int bucket = tsu_host::bucket_of(ml);
+            if (probed_bucket(s->ctx, bucket, ml)) s->probe_wait.push_back(Entry{r, i, 0});
+            else s->pending[queue_key(bucket, 0)].push_back(Entry{r, i, 0});
         }
         s->n_pending += n;
     }
@@ -619,14 +721,15 @@ extern "C" int tsu_serve_close(tsu_server* s) {
     s->cv.notify_all();
     if (s->worker.joinable()) s->worker.join();
     (void)hipSetDevice(s->ctx->device);
-    for (Slot& sl : s->slots) {
-        if (sl.stream) (void)hipStreamSynchronize(sl.stream);
-        if (sl.ev) (void)hipEventDestroy(sl.ev);
-        if (sl.stream) (void)hipStreamDestroy(sl.stream);
-        if (sl.h_words) (void)hipHostFree(sl.h_words);
-        if (sl.h_pairs) (void)hipHostFree(sl.h_pairs);
-        if (sl.h_order) (void)hipHostFree(sl.h_order);
-        if (sl.h_scores) (void)hipHostFree(sl.h_scores);
+    for (Slot* sl : all_slots(s)) {
+        if (sl->stream) (void)hipStreamSynchronize(sl->stream);
+        if (sl->ev) (void)hipEventDestroy(sl->ev);
+        if (sl->stream) (void)hipStreamDestroy(sl->stream);
+        if (sl->h_words) (void)hipHostFree(sl->h_words);
+        if (sl->h_pairs) (void)hipHostFree(sl->h_pairs);
+        if (sl->h_order) (void)hipHostFree(sl->h_order);
+        if (sl->h_scores) (void)hipHostFree(sl->h_scores);
+        if (sl->h_probe) (void)hipHostFree(sl->h_probe);
     }
     if (s->d_base) (void)hipFree(s->d_base);
     const int status = s->fatal;

@@ -115,6 +115,18 @@ int reg_waves(const tsu_ctx* ctx, int32_t W);
 // ring) for capacity W and sequences up to max_len (0: not eligible).
 int ml_waves(const tsu_ctx* ctx, int32_t W, int32_t max_len);
 
+// Probe of the batched and served forms: the single-wave register kernel
+// runs a pair up to kProbeCap scaled scores (its 512-diagonal window holds
+// the range that far), and the furthest h + v it reached gives the capacity
+// the pair is placed at (probe_capacity).
+constexpr int32_t kProbeCap = 240;
+int32_t probe_capacity(const tsu_ctx* ctx, const tsu_pair& pr, int32_t progress);
+// Pairs of a length bucket are probed when the register kernels take them
+// and the bucket's first capacity (W) needs several waves.
+inline bool probe_bucket(const tsu_ctx* ctx, int32_t W, int32_t tuned_cap, int tuned_waves) {
+    return fixed_242(ctx) && tuned_cap <= 0 && tuned_waves <= 1 && W > 1024;
+}
+
 // Reorders idx by key(i) descending, keeping the order of equal keys (a
 // counting sort over the key range, O(n + range), or a comparison sort when
 // the range is much wider than n).  Keys must be non-negative.
