@@ -766,7 +766,12 @@ static int batched_core(tsu_ctx* ctx, tsu_batch* b, const std::vector<int32_t>& 
         int32_t max_len;
         int nw_hint;      // tuned waves per pair (0: default shape)
         std::vector<int32_t> idx;
+        double cost = 0;  // estimated wave-steps: sum over pairs of score x waves
+        int stream = 0;
     };
+    // waves a pair takes at capacity W (windows of 1024 diagonals per wave
+    // above one wave's 512), as a weight of its estimated score
+    auto waves_of = [](int32_t W) { return std::max(0.25, (double)W / 1024.0); };
     std::vector<Group> groups;
     auto max_len_of = [&](const std::vector<int32_t>& ids) {
         int32_t m = 0;
@@ -791,7 +796,10 @@ static int batched_core(tsu_ctx* ctx, tsu_batch* b, const std::vector<int32_t>& 
             probe_ids.insert(probe_ids.end(), bk.idx.begin(), bk.idx.end());
             continue;
         }
-        groups.push_back(Group{W, max_len, nw_hint, bk.idx});
+        Group g{W, max_len, nw_hint, bk.idx};
+        // scores unknown: about a quarter of the length (a few % divergence)
+        for (int32_t i : g.idx) g.cost += 0.25 * (b->pairs[(size_t)i].p_len + b->pairs[(size_t)i].t_len) / 2 * waves_of(W);
+        groups.push_back(std::move(g));
     }
     if (!probe_ids.empty()) {
         tsu_host::order_longest_first(probe_ids, b->pairs.data());
@@ -817,6 +825,7 @@ static int batched_core(tsu_ctx* ctx, tsu_batch* b, const std::vector<int32_t>& 
         HIPCHK(hipMemcpy(prog.data(), d_prog.p, (size_t)b->n * sizeof(int32_t), hipMemcpyDeviceToHost), "hipMemcpy D2H (probe)");
         // the unfinished pairs, grouped by bucket and predicted capacity
         std::map<std::pair<int, int32_t>, std::vector<int32_t>> by;
+        std::map<std::pair<int, int32_t>, double> by_cost;
         for (int bi = 0; bi < nb; ++bi) {
             if (!probed_bucket[(size_t)bi]) continue;
             for (int32_t i : buckets[(size_t)bi].idx) {
@@ -825,26 +834,48 @@ static int batched_core(tsu_ctx* ctx, tsu_batch* b, const std::vector<int32_t>& 
                 if (v != tsu::kProbeStopped) {
                     // the probe window overflowed: the bucket's own capacity
                     const int32_t ml = std::max(b->pairs[(size_t)i].p_len, b->pairs[(size_t)i].t_len);
-                    by[{bi, std::min<int32_t>(initial_capacity(ctx, std::min(buckets[(size_t)bi].bound, ml)),
-                                              capacity_full(ml, ml))}].push_back(i);
+                    const std::pair<int, int32_t> key{bi, std::min<int32_t>(
+                        initial_capacity(ctx, std::min(buckets[(size_t)bi].bound, ml)), capacity_full(ml, ml))};
+                    by[key].push_back(i);
+                    by_cost[key] += 0.25 * ml * waves_of(key.second);
                     continue;
                 }
-                by[{bi, tsu_host::probe_capacity(ctx, b->pairs[(size_t)i], prog[(size_t)i])}].push_back(i);
+                const tsu_pair& pr = b->pairs[(size_t)i];
+                const std::pair<int, int32_t> key{bi, tsu_host::probe_capacity(ctx, pr, prog[(size_t)i])};
+                by[key].push_back(i);
+                by_cost[key] += (double)tsu_host::kProbeCap * (pr.p_len + pr.t_len) / std::max(prog[(size_t)i], 1) *
+                                waves_of(key.second);
             }
         }
-        for (auto& kv : by) groups.push_back(Group{kv.first.second, max_len_of(kv.second), 0, std::move(kv.second)});
+        for (auto& kv : by) {
+            Group g{kv.first.second, max_len_of(kv.second), 0, std::move(kv.second)};
+            g.cost = by_cost[kv.first];
+            groups.push_back(std::move(g));
+        }
     }
-    // widest capacity first (longest pairs first among equals): the long
-    // launches' tails overlap nothing, so start them early
+    // widest capacity first (longest pairs first among equals): the widest
+    // launches hold the longest single pairs, whose tails overlap nothing,
+    // so they start first; each group goes to the stream with the least
+    // estimated work so far (launches of one stream run one after another)
     std::stable_sort(groups.begin(), groups.end(), [](const Group& x, const Group& y) {
         return x.W != y.W ? x.W > y.W : x.max_len > y.max_len;
     });
+    {
+        double load[3] = {0, 0, 0};
+        for (Group& g : groups) {
+            int best = 0;
+            for (int k = 1; k < 3; ++k)
+                if (load[k] < load[best]) best = k;
+            g.stream = best;
+            load[best] += g.cost;
+        }
+    }
     // every group ordered longest first, all ordered up front (one thread per
     // group), so no sort sits between two launches
 #pragma omp parallel for schedule(dynamic, 1)
     for (int gi = 0; gi < (int)groups.size(); ++gi) tsu_host::order_longest_first(groups[(size_t)gi].idx, b->pairs.data());
-    // Every group's first launch is issued at once, widest first, over three
-    // streams: a persistent launch holds the CUs until its queue drains, and
+    // Every group's first launch is issued at once, over three streams: a
+    // persistent launch holds the CUs until its queue drains, and
     // the next launches' workgroups take the CUs it frees (or the room its
     // workgroups leave on a CU), so the long launches' tails overlap the
     // other groups' work instead of idling the chip between launches.  When
@@ -898,7 +929,7 @@ static int batched_core(tsu_ctx* ctx, tsu_batch* b, const std::vector<int32_t>& 
         rn->nw_hint = g.nw_hint;
         rn->order = d_order.p + off;
         rn->counter = d_counters.p + gi;
-        rn->stream = streams[gi % 3];
+        rn->stream = streams[g.stream];
         off += g.idx.size();
         rn->ids = std::move(g.idx);
         int r = start_run(std::move(rn));
