@@ -94,8 +94,11 @@ __device__ __forceinline__ int extend2(uint32_t v2, uint32_t h2, int lim2) {
     const uint32_t p0 = pp[0], p1 = pp[2], t0 = tt[1], t1 = tt[3];
     const uint32_t ap = __builtin_amdgcn_alignbit(p1, p0, v2), at = __builtin_amdgcn_alignbit(t1, t0, h2);
     const uint32_t x = ap ^ at;
-    int n2 = x ? (int)__builtin_ctz(x) : 32;  // matched bits (2 per base)
-    if (n2 == 32 && lim2 > 32) {
+    // matched bits (2 per base), clamped to lim2 at once: 32 then means 16
+    // bases matched and lim2 >= 32 (at lim2 == 32 the next round reads one
+    // word more than needed, still below the bound above, and is clamped)
+    int n2 = (int)min(x ? (uint32_t)__builtin_ctz(x) : 32u, (uint32_t)lim2);
+    if (n2 == 32) {
         const uint32_t xh = __builtin_amdgcn_alignbit(pp[4], p1, v2) ^ __builtin_amdgcn_alignbit(tt[5], t1, h2);
         int m = xh ? (int)__builtin_ctz(xh) : 32;
         n2 += m;
@@ -108,8 +111,9 @@ __device__ __forceinline__ int extend2(uint32_t v2, uint32_t h2, int lim2) {
                 bt2 += 16;
             } while (m == 64 && n2 < lim2);
         }
+        n2 = min(n2, lim2);
     }
-    return min(n2 & ~1, lim2);
+    return n2 & ~1;  // lim2 is even
 }
 
 // The same on plain offsets: bases pattern[v..] and text[h..] agree on, at
@@ -267,6 +271,13 @@ __global__ __launch_bounds__(64) void wfa_reg_kernel(KArgs a) {
                     const int32_t D = max(mor, der);
                     int32_t M = max(max(D, Mr[SX][c] + 2), I);
                     const bool valid = (uint32_t)M < (uint32_t)mt2;
+                    // lanes where M, I or D may be non-NULL (M before the
+                    // bounds test is their maximum, or a mismatch out of
+                    // bounds); those failing the bounds test include every
+                    // out-of-bounds I or D, so trimming on them is exact (a
+                    // trim that finds nothing out of bounds changes nothing)
+                    const uint64_t b = vote(M >= 0);
+                    oob |= b & ~vote(valid);
                     M = valid ? M : NUL;
                     const int lim2 = mt2 - 2 - M;  // 2 min(p - v, t - M) for a valid M
                     // extend from the packed words (v >= 0 for every valid M)
@@ -279,13 +290,10 @@ __global__ __launch_bounds__(64) void wfa_reg_kernel(KArgs a) {
                     // branches (two vector ops per flag).  The fields sit one
                     // bit up: a select of 1 : 0 becomes a zero-extension, which
                     // is also lowered through a VGPR.
-                    const uint64_t b = vote(max(max(M, I), D) >= 0);
                     uint32_t fl = (b ? 2u : 0u) | ((uint32_t)(b & 1ull) << 9) | ((uint32_t)(b >> 63) << 17);
                     asm volatile("" : "+s"(fl));
                     acc |= fl << c;
                     if (c == ctp) mend = __builtin_amdgcn_readlane(M, ltp);
-                    // I and D are out of bounds iff >= mt2 (D <= t by construction)
-                    oob |= vote(max(I, D) >= mt2);
                 } else {
                     carryI = NUL;  // an idle chunk holds only NULL
                 }

@@ -2,8 +2,11 @@
 // int16 LDS ring.
 //
 // The same one-wave-per-pair kernel as wfa_reg.h (windows of 64..512
-// diagonals, offsets kept doubled, chunk activity masks, fused extend), for
-// the scaled penalties 2/3/1, with the M ring split as in the 16-chunk
+// diagonals, offsets kept doubled, fused extend), for the scaled penalties
+// 2/3/1, with chunks chosen by the wavefront limits instead of per-chunk
+// activity masks (scalar arithmetic per score instead of per chunk; the
+// kernel issues as many scalar as vector instructions), and the M ring
+// split as in the 16-chunk
 // multi-wave kernel (wfa_regml.h): a lane keeps M of the two newest scores
 // (two VGPRs per chunk, by the parity of s) and I and D; M[s-2], once read
 // for the mismatch, goes to an int16 LDS ring of three slots, from which the
@@ -36,11 +39,10 @@ constexpr int kMl1MaxScore = 16000;
 
 template <int C, bool QUEUE, bool BT = false>
 __global__ __launch_bounds__(64) void wfa_reg1l_kernel(KArgs a) {
-    static_assert(C >= 1 && C <= 8, "window of 64..512 diagonals (8-bit chunk fields)");
+    static_assert(C >= 1 && C <= 8, "window of 64..512 diagonals");
     constexpr int32_t NUL = kRegNull;
     constexpr int W = 64 * C;
     constexpr int MRS = W + 2;  // int16 entries per ring slot, NULL guards at both ends
-    constexpr uint32_t ALL = (1u << C) - 1u;
     extern __shared__ __align__(16) uint32_t smem[];
     const int lane = threadIdx.x;
     const int lane2 = 2 * lane;
@@ -93,13 +95,39 @@ __global__ __launch_bounds__(64) void wfa_reg1l_kernel(KArgs a) {
             Ir[c] = NUL;
             Dr[c] = NUL;
         }
-        // step masks of scores s-1 .. s-5 (fields as in wfa_reg.h)
-        uint32_t F1 = 0, F2 = 0, F3 = 0, F4 = 0, F5 = 0;
         // per chunk: one past the last doubled offset in bounds,
-        // min(2(k + p + 1), 2(t + 1)), kept across the score loop
-        int32_t MT2[C];
+        // min(2(k + p + 1), 2(t + 1)), and two less (floored at 0: the only
+        // diagonal with no cell in bounds, k = -p-1, holds 0 in both), kept
+        // across the score loop.  A valid M below MT2m2 has bases left to
+        // extend over, MT2m2 - M doubled.
+        int32_t MT2[C], MT2m2[C];
 #pragma unroll
-        for (int c = 0; c < C; ++c) MT2[c] = min(2 * (kmin + 64 * c) + lane2 + 2 * (p + 1), 2 * (t + 1));
+        for (int c = 0; c < C; ++c) {
+            MT2[c] = min(2 * (kmin + 64 * c) + lane2 + 2 * (p + 1), 2 * (t + 1));
+            MT2m2[c] = max(MT2[c] - 2, 0);
+        }
+        // Window edges that are not bounds of the alignment: a non-NULL cell
+        // in the window's first (last) lane then means the range may leave
+        // the window.  Diagonals -p-1 and t+1 hold no cell in bounds.
+        const bool edge_lo = kmin > -p - 1;
+        const bool edge_hi = kmax < t + 1;
+
+        // Ranges of window diagonals (j = k - kmin) that may hold a non-NULL
+        // cell, as WFA2-lib's wavefront limits (wfa_gpu.cu:426-468) for the
+        // scaled penalties 2/3/1: I/D at s within [min(lo M[s-4], lo I/D[s-1])
+        // - 1, max(hi M[s-4], hi I/D[s-1]) + 1], M at s the union of that
+        // and M at s-2.  The ranges only grow (trimmed cells are NULLed but
+        // the limits stay), so the range at s covers every chunk that held a
+        // non-NULL cell at s-5 .. s-1 too, and its ring entries get
+        // overwritten.  mt/mb hold the M limits of the last two scores of
+        // each parity ([par][0] newest), it/ib those of I/D at s-1.
+        constexpr int RNONE = 1 << 20;  // limits of an empty range: top -RNONE, bottom +RNONE
+        int mt[2][2], mb[2][2];
+        mt[0][0] = -kmin; mb[0][0] = -kmin;  // score 0: M[0][0] only
+        mt[0][1] = -RNONE; mb[0][1] = RNONE;
+        mt[1][0] = -RNONE; mb[1][0] = RNONE;
+        mt[1][1] = -RNONE; mb[1][1] = RNONE;
+        int it = -RNONE, ib = RNONE;
 
         // score 0: M[0][0] = 0, extended (every lane computes it)
         const int ext0 = lcp_interleaved(sS, 0, 0, min(p, t));
@@ -108,7 +136,6 @@ __global__ __launch_bounds__(64) void wfa_reg1l_kernel(KArgs a) {
 #pragma unroll
             for (int c = 0; c < C; ++c)
                 if (c == c0) Mr[0][c] = lane == l0 ? 2 * ext0 : NUL;
-            F1 = (1u << c0) | (l0 == 0 ? 1u << (8 + c0) : 0u) | (l0 == 63 ? 1u << (16 + c0) : 0u);
         }
         int s = 0;
         int l3 = 0;  // s mod 3
@@ -131,23 +158,32 @@ __global__ __launch_bounds__(64) void wfa_reg1l_kernel(KArgs a) {
             // hoisted out of the score loop into VGPRs
             int zero_s;
             asm volatile("s_mov_b32 %0, 0" : "=s"(zero_s));
-            // chunk c computes when it may hold a non-NULL source (M of s-2
-            // and s-4, I/D of s-1), a ring value to overwrite (s-5), or when
-            // a neighbour chunk's edge lane feeds it
-            const uint32_t src = F1 | F4;
-            const uint32_t act = ((src | F2 | F5) | ((src >> 16) << 1) | (((src >> 8) & 0xffu) >> 1)) & ALL;
+            // the doubled pattern offset of a lane in chunk c is
+            // 2v = M - 2k = M + nl2 + (nkb - 128 c); nl2 is opaque per step
+            // too, so no per-chunk sum of it is held in a VGPR
+            int nl2 = -lane2;
+            asm volatile("" : "+v"(nl2));
+            const int nkb = zero_s - 2 * kmin;
+            // limits at s; chunk c computes when it meets the M range
+            it = max(mt[PAR][1], it) + 1;
+            ib = min(mb[PAR][1], ib) - 1;
+            mt[PAR][1] = mt[PAR][0];
+            mb[PAR][1] = mb[PAR][0];
+            mt[PAR][0] = max(mt[PAR][0], it);
+            mb[PAR][0] = min(mb[PAR][0], ib);
+            const int jlo = max(mb[PAR][0], 0), jhi = min(mt[PAR][0], W - 1);
+            const uint32_t act = jlo <= jhi ? (2u << (jhi >> 6)) - (1u << (jlo >> 6)) : 0u;
             const int16_t* mrd = MR + rd * MRS + lane;       // entry of k-1
             int16_t* mwr = MR + wrs * MRS + lane + 1;        // entry of k
             uint32_t* recs = nullptr;                        // records of this step
             if constexpr (BT) recs = rec + (int64_t)s * C * 64;
-            uint32_t acc = 0;
             uint64_t oob = 0;
+            uint32_t edge = 0;  // a non-NULL cell in an edge lane of the window
             int32_t mend = NUL;
             int32_t carryI = NUL;  // old I of the previous chunk's lane 63
 #pragma unroll
             for (int c = 0; c < C; ++c) {
                 if (act & (1u << c)) {
-                    const int k2 = 2 * (kmin + zero_s + 64 * c) + lane2;
                     const int mt2 = MT2[c];
                     const int32_t mx = Mr[PAR][c];
                     mwr[64 * c] = (int16_t)max(mx, -32768);
@@ -161,6 +197,15 @@ __global__ __launch_bounds__(64) void wfa_reg1l_kernel(KArgs a) {
                     const int32_t X = mx + 2;
                     int32_t M = max(max(D, X), I);
                     const bool valid = (uint32_t)M < (uint32_t)mt2;
+                    // M before the bounds test is out of bounds where I or D
+                    // is, or the mismatch (then trimming finds nothing and
+                    // changes nothing)
+                    oob |= vote(M >= mt2);
+                    // M before the bounds test is non-NULL where M, I or D
+                    // is, or a mismatch from a non-NULL M[s-2] (an edge hit
+                    // two scores earlier)
+                    if (c == 0 && edge_lo) edge |= (uint32_t)(vote(M >= 0) & 1ull);
+                    if (c == C - 1 && edge_hi) edge |= (uint32_t)(vote(M >= 0) >> 63);
                     if constexpr (BT) {
                         // origin of M (ties M > D > I), gap extension bits, M before extension
                         const uint32_t rb = (iel >= mol ? 4u : 0u) | (der >= mor ? 8u : 0u);
@@ -169,25 +214,20 @@ __global__ __launch_bounds__(64) void wfa_reg1l_kernel(KArgs a) {
                                                           : 0u);
                     }
                     M = valid ? M : NUL;
-                    const int lim2 = mt2 - 2 - M;
-                    if (valid && lim2 > 0) M += extend2((uint32_t)(M - k2), (uint32_t)M, lim2);
+                    const int mt2m2 = MT2m2[c];
+                    if ((uint32_t)M < (uint32_t)mt2m2)  // valid, with bases left
+                        M += extend2((uint32_t)M + (uint32_t)nl2 + (uint32_t)(nkb - 128 * c), (uint32_t)M, mt2m2 - M);
                     Mr[PAR][c] = M;
                     Ir[c] = I;
                     Dr[c] = D;
-                    const uint64_t b = vote(max(max(M, I), D) >= 0);
-                    uint32_t fl = (b ? 2u : 0u) | ((uint32_t)(b & 1ull) << 9) | ((uint32_t)(b >> 63) << 17);
-                    asm volatile("" : "+s"(fl));
-                    acc |= fl << c;
                     if (c == ctp) mend = __builtin_amdgcn_readlane(M, ltp);
-                    oob |= vote(max(I, D) >= mt2);
                 } else {
                     carryI = NUL;  // an idle chunk holds only NULL
                 }
             }
-            uint32_t Fs = acc >> 1;
             if (oob) {
                 // exact trim (wfa_gpu.cu:633-719): NULL I/D cells outside
-                // [first, last] in-bounds cell, then rebuild the step mask
+                // [first, last] in-bounds cell, then test the edge lanes again
                 auto trim = [&](int32_t (&R)[C]) __attribute__((always_inline)) {
                     int f = 0x3fffffff, l = -1;
 #pragma unroll
@@ -208,19 +248,13 @@ __global__ __launch_bounds__(64) void wfa_reg1l_kernel(KArgs a) {
                 };
                 trim(Ir);
                 trim(Dr);
-                uint32_t m2 = 0;
-#pragma unroll
-                for (int c = 0; c < C; ++c) {
-                    const uint64_t b = vote(max(max(Mr[PAR][c], Ir[c]), Dr[c]) >= 0);
-                    m2 |= ((b ? 2u : 0u) | ((uint32_t)(b & 1ull) << 9) | ((uint32_t)(b >> 63) << 17)) << c;
-                }
-                Fs = m2 >> 1;
+                edge = 0;
+                if (edge_lo) edge |= (uint32_t)(vote(max(max(Mr[PAR][0], Ir[0]), Dr[0]) >= 0) & 1ull);
+                if (edge_hi) edge |= (uint32_t)(vote(max(max(Mr[PAR][C - 1], Ir[C - 1]), Dr[C - 1]) >= 0) >> 63);
             }
-            F5 = F4; F4 = F3; F3 = F2; F2 = F1; F1 = Fs;
             if (mend >= 2 * t) return s;
             // a non-NULL cell at a window edge could extend the range beyond it
-            if ((Fs >> 8) & 1u) return TSU_EKOVERFLOW;
-            if ((Fs >> (16 + C - 1)) & 1u) return TSU_EKOVERFLOW;
+            if (edge) return TSU_EKOVERFLOW;
             return 0;
         };
 

@@ -312,6 +312,10 @@ __global__ __launch_bounds__(NW * 64) void wfa_regml_kernel(KArgs a) {
                     const int32_t X = mx + U;
                     int32_t M = max(max(D, X), I);
                     const bool valid = (uint32_t)M < (uint32_t)mt1;
+                    // lanes that may hold a non-NULL M, I or D (M before the
+                    // bounds test is their maximum; a mismatch source out of
+                    // bounds only adds lanes whose M at s-2 was non-NULL)
+                    const uint64_t b = vote(M >= 0);
                     if constexpr (BT) {
                         const uint32_t r = rb | (valid ? (((uint32_t)M >> (U - 1)) << 4) | ((X >= D && X >= I) ? 1u : (D >= I ? 2u : 3u)) : 0u);
                         recs[(uint32_t)(64 * NW * c) + rl] = r;
@@ -328,7 +332,6 @@ __global__ __launch_bounds__(NW * 64) void wfa_regml_kernel(KArgs a) {
                     Ir[c] = I;
                     Dr[c] = D;
                     epub[c * NW] = lane == 0 ? D : I;
-                    const uint64_t b = vote(max(max(M, I), D) >= 0);
                     uint32_t fl = b ? 2u << c : 0u;
                     asm volatile("" : "+s"(fl));
                     acc |= fl;

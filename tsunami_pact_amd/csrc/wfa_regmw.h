@@ -219,13 +219,15 @@ __global__ __launch_bounds__(NW * 64) void wfa_regmw_kernel(KArgs a) {
                     const int32_t D = max(mor, der);
                     int32_t M = max(max(D, Mr[SX][c] + 1), I);
                     const bool valid = (uint32_t)M < (uint32_t)mt1;
+                    // lanes that may hold a non-NULL M, I or D (M before the
+                    // bounds test is their maximum)
+                    const uint64_t b = vote(M >= 0);
                     M = valid ? M : NUL;
                     const int lim = mt1 - 1 - M;
                     if (valid && lim > 0) M += extend_len((uint32_t)(M - k), (uint32_t)M, lim);
                     Mr[PH][c] = M;
                     Ir[c] = I;
                     Dr[c] = D;
-                    const uint64_t b = vote(max(max(M, I), D) >= 0);
                     uint32_t fl = (b ? 2u : 0u) | ((uint32_t)(b & 1ull) << 9) | ((uint32_t)(b >> 63) << 17);
                     asm volatile("" : "+s"(fl));
                     acc |= fl << c;
