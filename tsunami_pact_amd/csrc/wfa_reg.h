@@ -82,11 +82,12 @@ __device__ __forceinline__ int cmp32(uint32_t bp, uint32_t bt, uint32_t sa, uint
 // Extension on doubled offsets (2 x bases, the unit the register kernels
 // keep offsets in): v2 = 2v, h2 = 2h, lim2 = 2 x the most bases allowed
 // (>= 2).  Returns 2 x the bases pattern[v..] and text[h..] agree on, at
-// most lim2.  16 bases first (most diagonals stop there), then the rest of
-// the 32-base window, then rounds of 32; a doubled offset is also the bit
+// most lim2.  16 bases first (most diagonals stop there), then rounds of
+// 16 over the next words; a doubled offset is also the bit
 // position of its base in the packed words, so it is the funnel shift
 // (alignbit takes it modulo 32).  Reads stay below word (v+lim-1)/16 + 2 of
 // each sequence, inside the two padding words.
+__device__ __forceinline__ lds_cu32* pp2(uint32_t b) { return reinterpret_cast<lds_cu32*>((uintptr_t)b); }
 __device__ __forceinline__ int extend2(uint32_t v2, uint32_t h2, int lim2) {
     const uint32_t bp = (v2 >> 5) << 3, bt = (h2 >> 5) << 3;
     lds_cu32* pp = reinterpret_cast<lds_cu32*>((uintptr_t)bp);
@@ -103,13 +104,17 @@ __device__ __forceinline__ int extend2(uint32_t v2, uint32_t h2, int lim2) {
         int m = xh ? (int)__builtin_ctz(xh) : 32;
         n2 += m;
         if (m == 32 && n2 < lim2) {
-            uint32_t bp2 = bp + 16, bt2 = bt + 16;
+            // 16 bases per round from here (long runs are rare; fewer live
+            // registers than a 32-base round)
+            uint32_t bp2 = bp + 16, bt2 = bt + 16;  // words w+2 (w: the word of base v / h)
             do {
-                m = cmp32(bp2, bt2, v2, h2);
+                const uint32_t y = __builtin_amdgcn_alignbit(pp2(bp2 + 8)[0], pp2(bp2)[0], v2) ^
+                                   __builtin_amdgcn_alignbit(pp2(bt2 + 8)[1], pp2(bt2)[1], h2);
+                m = y ? (int)__builtin_ctz(y) : 32;
                 n2 += m;
-                bp2 += 16;
-                bt2 += 16;
-            } while (m == 64 && n2 < lim2);
+                bp2 += 8;
+                bt2 += 8;
+            } while (m == 32 && n2 < lim2);
         }
         n2 = min(n2, lim2);
     }

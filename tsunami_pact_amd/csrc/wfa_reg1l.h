@@ -18,8 +18,8 @@
 //
 // Doubled offsets up to 2t must fit int16 (sequences of at most 16000
 // bases; the host uses wfa_reg.h beyond).  NULL is -32768 in the ring and
-// -2^30 in registers (an inline constant: the kernel is bound by VALU issue,
-// and a literal NULL costs more than clamping M on its way to the ring); a
+// -2^30 + 2^15 in registers, whose low half is the ring's NULL, so an M
+// (exactly NULL or an offset below 32768) goes to the ring as it is; a
 // doubled value fed only by the ring's NULL grows by two per score, so the
 // kernel gives up past kMl1MaxScore scaled scores with TSU_EKOVERFLOW.
 //
@@ -40,7 +40,7 @@ constexpr int kMl1MaxScore = 16000;
 template <int C, bool QUEUE, bool BT = false>
 __global__ __launch_bounds__(64) void wfa_reg1l_kernel(KArgs a) {
     static_assert(C >= 1 && C <= 8, "window of 64..512 diagonals");
-    constexpr int32_t NUL = kRegNull;
+    constexpr int32_t NUL = kRegNull + 0x8000;  // low 16 bits: -32768
     constexpr int W = 64 * C;
     constexpr int MRS = W + 2;  // int16 entries per ring slot, NULL guards at both ends
     extern __shared__ __align__(16) uint32_t smem[];
@@ -84,6 +84,7 @@ __global__ __launch_bounds__(64) void wfa_reg1l_kernel(KArgs a) {
         const int tp = t - p;
         const int ctp = (tp - kmin) >= 0 ? (tp - kmin) >> 6 : -1;
         const int ltp = (tp - kmin) & 63;
+        const uint32_t endm = ctp >= 0 && ctp < C ? 1u << ctp : 0u;  // chunk of the end cell
         const int64_t bound_u = 2ll * min(p, t) + (p != t ? 3ll + (int64_t)abs(p - t) : 0);
         const int s_bound = (int)min<int64_t>(bound_u, 0x3fffffff);
 
@@ -109,8 +110,7 @@ __global__ __launch_bounds__(64) void wfa_reg1l_kernel(KArgs a) {
         // Window edges that are not bounds of the alignment: a non-NULL cell
         // in the window's first (last) lane then means the range may leave
         // the window.  Diagonals -p-1 and t+1 hold no cell in bounds.
-        const bool edge_lo = kmin > -p - 1;
-        const bool edge_hi = kmax < t + 1;
+        const uint32_t edges = __builtin_amdgcn_readfirstlane((kmin > -p - 1 ? 1u : 0u) | (kmax < t + 1 ? 2u : 0u));
 
         // Ranges of window diagonals (j = k - kmin) that may hold a non-NULL
         // cell, as WFA2-lib's wavefront limits (wfa_gpu.cu:426-468) for the
@@ -159,11 +159,15 @@ __global__ __launch_bounds__(64) void wfa_reg1l_kernel(KArgs a) {
             int zero_s;
             asm volatile("s_mov_b32 %0, 0" : "=s"(zero_s));
             // the doubled pattern offset of a lane in chunk c is
-            // 2v = M - 2k = M + nl2 + (nkb - 128 c); nl2 is opaque per step
-            // too, so no per-chunk sum of it is held in a VGPR
-            int nl2 = -lane2;
-            asm volatile("" : "+v"(nl2));
+            // 2v = M - 2k = M + nl2 + (nkb - 128 c); nl2 is recomputed per
+            // step, so no per-chunk sum of it is held in a VGPR
+            const int nl2 = zero_s - lane2;
             const int nkb = zero_s - 2 * kmin;
+            // opaque per step: scalar bit tests per chunk
+            uint32_t em = __builtin_amdgcn_readfirstlane(endm);
+            asm volatile("" : "+s"(em));
+            uint32_t eg = edges;
+            asm volatile("" : "+s"(eg));
             // limits at s; chunk c computes when it meets the M range
             it = max(mt[PAR][1], it) + 1;
             ib = min(mb[PAR][1], ib) - 1;
@@ -186,7 +190,7 @@ __global__ __launch_bounds__(64) void wfa_reg1l_kernel(KArgs a) {
                 if (act & (1u << c)) {
                     const int mt2 = MT2[c];
                     const int32_t mx = Mr[PAR][c];
-                    mwr[64 * c] = (int16_t)max(mx, -32768);
+                    mwr[64 * c] = (int16_t)mx;  // exactly NULL or an offset below 32768
                     const int32_t mol = mrd[64 * c];
                     const int32_t mor = mrd[64 * c + 2];
                     const int32_t iel = dpp_shr1(carryI, Ir[c]);
@@ -195,32 +199,32 @@ __global__ __launch_bounds__(64) void wfa_reg1l_kernel(KArgs a) {
                     const int32_t I = max(mol, iel) + 2;
                     const int32_t D = max(mor, der);
                     const int32_t X = mx + 2;
-                    int32_t M = max(max(D, X), I);
-                    const bool valid = (uint32_t)M < (uint32_t)mt2;
-                    // M before the bounds test is out of bounds where I or D
-                    // is, or the mismatch (then trimming finds nothing and
-                    // changes nothing)
-                    oob |= vote(M >= mt2);
-                    // M before the bounds test is non-NULL where M, I or D
-                    // is, or a mismatch from a non-NULL M[s-2] (an edge hit
-                    // two scores earlier)
-                    if (c == 0 && edge_lo) edge |= (uint32_t)(vote(M >= 0) & 1ull);
-                    if (c == C - 1 && edge_hi) edge |= (uint32_t)(vote(M >= 0) >> 63);
+                    const int32_t Mp = max(max(D, X), I);  // M before the bounds test
+                    const bool valid = (uint32_t)Mp < (uint32_t)mt2;
+                    int32_t M = valid ? Mp : NUL;
+                    // Mp is out of bounds where I or D is, or the mismatch
+                    // (then trimming finds nothing and changes nothing)
+                    oob |= vote(Mp >= mt2);
+                    // Mp is non-NULL where M, I or D is, or a mismatch from a
+                    // non-NULL M[s-2] (an edge hit two scores earlier)
+                    if (c == 0) edge |= (uint32_t)(vote(Mp >= 0) & 1ull) & eg;
+                    if (c == C - 1) edge |= (uint32_t)(vote(Mp >= 0) >> 63) & (eg >> 1);
                     if constexpr (BT) {
                         // origin of M (ties M > D > I), gap extension bits, M before extension
                         const uint32_t rb = (iel >= mol ? 4u : 0u) | (der >= mor ? 8u : 0u);
-                        recs[64 * c + lane] = rb | (valid ? (((uint32_t)M >> 1) << 4) |
+                        recs[64 * c + lane] = rb | (valid ? (((uint32_t)Mp >> 1) << 4) |
                                                                 ((X >= D && X >= I) ? 1u : (D >= I ? 2u : 3u))
                                                           : 0u);
                     }
-                    M = valid ? M : NUL;
                     const int mt2m2 = MT2m2[c];
+                    int sc = nkb - 128 * c;  // one scalar operand: a three-way add
+                    asm volatile("" : "+s"(sc));
                     if ((uint32_t)M < (uint32_t)mt2m2)  // valid, with bases left
-                        M += extend2((uint32_t)M + (uint32_t)nl2 + (uint32_t)(nkb - 128 * c), (uint32_t)M, mt2m2 - M);
+                        M += extend2((uint32_t)M + (uint32_t)nl2 + (uint32_t)sc, (uint32_t)M, mt2m2 - M);
                     Mr[PAR][c] = M;
                     Ir[c] = I;
                     Dr[c] = D;
-                    if (c == ctp) mend = __builtin_amdgcn_readlane(M, ltp);
+                    if (em & (1u << c)) mend = __builtin_amdgcn_readlane(M, ltp);
                 } else {
                     carryI = NUL;  // an idle chunk holds only NULL
                 }
@@ -249,8 +253,8 @@ __global__ __launch_bounds__(64) void wfa_reg1l_kernel(KArgs a) {
                 trim(Ir);
                 trim(Dr);
                 edge = 0;
-                if (edge_lo) edge |= (uint32_t)(vote(max(max(Mr[PAR][0], Ir[0]), Dr[0]) >= 0) & 1ull);
-                if (edge_hi) edge |= (uint32_t)(vote(max(max(Mr[PAR][C - 1], Ir[C - 1]), Dr[C - 1]) >= 0) >> 63);
+                edge |= (uint32_t)(vote(max(max(Mr[PAR][0], Ir[0]), Dr[0]) >= 0) & 1ull) & eg;
+                edge |= (uint32_t)(vote(max(max(Mr[PAR][C - 1], Ir[C - 1]), Dr[C - 1]) >= 0) >> 63) & (eg >> 1);
             }
             if (mend >= 2 * t) return s;
             // a non-NULL cell at a window edge could extend the range beyond it
