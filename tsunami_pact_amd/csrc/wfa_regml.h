@@ -18,11 +18,12 @@
 // Global chunk j belongs to wave j mod NW, register slot j / NW, as in the
 // multi-wave register kernel (wfa_regmw.h).  I and D cross chunk edges
 // through LDS: each computed chunk publishes lane 63 of I and lane 0 of D
-// for its parity of s, and each wave publishes its step mask every step.
-// Chunks whose sources are all NULL are skipped; the argument that the
-// skipped registers, edge words and LDS slots then hold NULL values is the
-// same as for the other register kernels: a chunk with a non-NULL value at
-// score s is computed again at s+1, s+2, s+4 and s+5.
+// for its parity of s.  The chunks computed at score s are those that meet
+// the wavefront limits of s (WFA2-lib's lo / hi, wfa_gpu.cu:426-468), which
+// every wave derives from the same scalar recurrence, so the waves exchange
+// no activity masks.  The limits only grow, so a chunk, once computed, is
+// computed at every later score; the registers, edge words and ring slots of
+// a chunk not yet computed hold NULL.
 //
 // NULL is kRingNull in registers and -32768 in the int16 ring: its low half
 // is the ring's NULL, so an M register (a valid offset or exactly NULL) goes
@@ -78,21 +79,14 @@ struct MlLayout {
     static constexpr int WORDS = MR + (3 * MR_STRIDE + 1) / 2;
 };
 
-// Step masks (one bit per chunk): bit c is set when chunk c holds a
-// non-NULL M, I or D value.  A neighbour chunk counts as a source whenever
-// it holds any non-NULL value, which computes a few chunks more at the
-// wavefront's ends than edge-lane masks would, for a third of the scalar
-// work per chunk.
-
 // Neighbour synchronisation.  Wave w's step s reads, besides its own
 // registers, only what the waves owning its neighbour chunks (w-1 and w+1,
-// cyclically) wrote at steps s-1 and s-3: their edge lanes, their ring
-// entries and their step masks.  So instead of a workgroup barrier per score,
-// each wave publishes the last score it finished and starts score s+1 once
-// both neighbours have finished s.  Neighbours are then never more than one
-// score apart, which is what the double-buffered edges, the 3-slot M ring
-// and the 5-slot masks need: a wave at s+1 overwrites only data of scores its
-// neighbours have finished reading.  What needs the whole window is turned
+// cyclically) wrote at steps s-1 and s-3: their edge lanes and their ring
+// entries.  So instead of a workgroup barrier per score, each wave publishes
+// the last score it finished and starts score s+1 once both neighbours have
+// finished s.  Neighbours are then never more than one score apart, which is
+// what the double-buffered edges and the 3-slot M ring need: a wave at s+1
+// overwrites only data of scores its neighbours have finished reading.  What needs the whole window is turned
 // into per-wave events kept as a minimum (score * 4 + kind): the end cell
 // (kind 0), a non-NULL cell at the window edge (1), the score limit (2) and
 // an internal error (3).  Every wave goes on until it has finished the
@@ -120,7 +114,6 @@ __global__ __launch_bounds__(NW * 64) void wfa_regml_kernel(KArgs a) {
     constexpr int NC = NW * C;
     constexpr int W = 64 * NC;
     constexpr int NT = NW * 64;
-    constexpr uint32_t ALL = (1u << C) - 1u;
     using L = MlLayout<NW, C>;
     extern __shared__ __align__(16) uint32_t smem[];
     const int tid = threadIdx.x;
@@ -129,7 +122,6 @@ __global__ __launch_bounds__(NW * 64) void wfa_regml_kernel(KArgs a) {
     uint32_t* sS = smem;  // interleaved pattern / text words
     int32_t* X = reinterpret_cast<int32_t*>(smem + 2 * a.seq_words);
     int16_t* MR = reinterpret_cast<int16_t*>(X + L::MR);
-    uint32_t* FWp = reinterpret_cast<uint32_t*>(X + L::FW);
     const bool lds_at_zero = (uint32_t)(uintptr_t)sS == 0u;
     const int wl = w == 0 ? NW - 1 : w - 1;  // owner of chunk j-1 (slot r-1 when w == 0)
     const int wr = w == NW - 1 ? 0 : w + 1;  // owner of chunk j+1 (slot r+1 when w == NW-1)
@@ -190,8 +182,20 @@ __global__ __launch_bounds__(NW * 64) void wfa_regml_kernel(KArgs a) {
             Ir[c] = NUL;
             Dr[c] = NUL;
         }
-        // this wave's step masks of scores s-1 .. s-5
-        uint32_t F1 = 0, F2 = 0, F3 = 0, F4 = 0, F5 = 0;
+        // Window diagonal limits (j = k - kmin) of the cells that may be
+        // non-NULL, for the scaled penalties 2/3/1: I/D at s within
+        // [min(lo M[s-4], lo I/D[s-1]) - 1, max(hi M[s-4], hi I/D[s-1]) + 1],
+        // M at s the union of that and M at s-2; mt/mb hold the M limits of
+        // the last two scores of each parity ([par][0] newest), it/ib those
+        // of I/D at s-1.
+        constexpr int RNONE = 1 << 20;  // limits of an empty range: top -RNONE, bottom +RNONE
+        int mt[2][2], mb[2][2];
+        mt[0][0] = -kmin; mb[0][0] = -kmin;  // score 0: M[0][0] only
+        mt[0][1] = -RNONE; mb[0][1] = RNONE;
+        mt[1][0] = -RNONE; mb[1][0] = RNONE;
+        mt[1][1] = -RNONE; mb[1][1] = RNONE;
+        int it = -RNONE, ib = RNONE;
+        const uint32_t w0 = w == 0 ? 1u : 0u, wlast = w == NW - 1 ? 1u : 0u;
 
         // score 0: M[0][0] = 0, extended (every lane computes it)
         const int ext0 = lcp_interleaved(sS, 0, 0, min(p, t));
@@ -202,18 +206,11 @@ __global__ __launch_bounds__(NW * 64) void wfa_regml_kernel(KArgs a) {
 #pragma unroll
                 for (int r = 0; r < C; ++r)
                     if (r == r0) Mr[0][r] = lane == l0 ? U * ext0 : NUL;
-                F1 = 1u << r0;
             }
-            if (lane == 0) FWp[0 * NW + w] = F1;
         }
         __syncthreads();
         int s = 0;
-        int l5 = 0;  // s mod 5
         int l3 = 0;  // s mod 3
-        // neighbour waves' masks of scores s and s-3 (the sources of the
-        // next step), read after each neighbour wait
-        uint32_t nbL = (uint32_t)__builtin_amdgcn_readfirstlane(FWp[wl]);
-        uint32_t nbR = (uint32_t)__builtin_amdgcn_readfirstlane(FWp[wr]);
 
         // One score step; PAR = s & 1 selects the register slot holding
         // M[s-2], which receives M[s].  Returns 0 to go on, 1 once an event
@@ -241,9 +238,6 @@ __global__ __launch_bounds__(NW * 64) void wfa_regml_kernel(KArgs a) {
                     __builtin_amdgcn_s_sleep(2);  // about 128 clocks: 0.5% faster than 64 on a mixed batch; 256 within noise of it, 512 slower
                 }
                 __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-                const int n3 = l5 >= 3 ? l5 - 3 : l5 + 2;  // slot of s-3
-                nbL = (uint32_t)__builtin_amdgcn_readfirstlane(FWp[l5 * NW + wl] | FWp[n3 * NW + wl]);
-                nbR = (uint32_t)__builtin_amdgcn_readfirstlane(FWp[l5 * NW + wr] | FWp[n3 * NW + wr]);
             }
             ++s;
             // every wave reaches the score limits at the same score, so with
@@ -253,7 +247,6 @@ __global__ __launch_bounds__(NW * 64) void wfa_regml_kernel(KArgs a) {
             if constexpr (BT) {
                 if (s >= a.step_cap) return TSU_ECIGAR;
             }
-            l5 = l5 == 4 ? 0 : l5 + 1;
             l3 = l3 == 2 ? 0 : l3 + 1;
             const int rd = l3 == 0 ? 2 : l3 - 1;   // ring slot of s-4
             const int wrs = l3 == 2 ? 0 : l3 + 1;  // ring slot of s-2 (formerly s-5)
@@ -265,11 +258,21 @@ __global__ __launch_bounds__(NW * 64) void wfa_regml_kernel(KArgs a) {
             asm volatile("" : "+s"(pb));
             asm volatile("" : "+v"(ln));
             const int kl = U * (kmin + pb + ln);  // diagonal of this lane in chunk 0 (in offset units)
-            uint32_t hiL = nbL & ALL;
-            uint32_t loR = nbR & ALL;
-            if (w == 0) hiL <<= 1;
-            if (w == NW - 1) loR >>= 1;
-            const uint32_t act = (F1 | F2 | F4 | F5 | hiL | loR) & ALL;
+            // limits at s; this wave computes its chunks r (global chunk
+            // r NW + w) that meet the M range
+            it = max(mt[PAR][1], it) + 1;
+            ib = min(mb[PAR][1], ib) - 1;
+            mt[PAR][1] = mt[PAR][0];
+            mb[PAR][1] = mb[PAR][0];
+            mt[PAR][0] = max(mt[PAR][0], it);
+            mb[PAR][0] = min(mb[PAR][0], ib);
+            uint32_t act = 0;
+            {
+                const int jlo = max(mb[PAR][0], 0) >> 6, jhi = min(mt[PAR][0], W - 1) >> 6;
+                const int rlo = (jlo + NW - 1 - w) / NW;  // >= 0: jlo >= 0, w < NW
+                const int rhi = jhi >= w ? (jhi - w) / NW : -1;
+                if (mb[PAR][0] <= mt[PAR][0] && rlo <= rhi) act = (2u << rhi) - (1u << rlo);
+            }
             // edges of score s-1, read through one VGPR address per step
             int ea = 4 * (2 * a.seq_words + L::EI + (PAR ^ 1) * (NC + 2) + w);
             asm volatile("" : "+v"(ea));
@@ -285,7 +288,6 @@ __global__ __launch_bounds__(NW * 64) void wfa_regml_kernel(KArgs a) {
             if constexpr (BT) recs = rec + (int64_t)s * NC * 64;
             const int16_t* mrd = MR + rd * L::MR_STRIDE + pb + ln;        // entry of k-1
             int16_t* mwr = MR + wrs * L::MR_STRIDE + pb + ln + 1;         // entry of k
-            uint32_t acc = 0;  // step mask, one bit up
             uint32_t edge = 0;
             int32_t mend = NUL;
 #pragma unroll
@@ -310,17 +312,15 @@ __global__ __launch_bounds__(NW * 64) void wfa_regml_kernel(KArgs a) {
                     I = (uint32_t)I < (uint32_t)mt1 ? I : NUL;
                     D = (uint32_t)D < (uint32_t)mt1 ? D : NUL;
                     const int32_t X = mx + U;
-                    int32_t M = max(max(D, X), I);
-                    const bool valid = (uint32_t)M < (uint32_t)mt1;
-                    // lanes that may hold a non-NULL M, I or D (M before the
-                    // bounds test is their maximum; a mismatch source out of
-                    // bounds only adds lanes whose M at s-2 was non-NULL)
-                    const uint64_t b = vote(M >= 0);
+                    // M before the bounds test: non-NULL where M, I or D is,
+                    // or a mismatch from a non-NULL M[s-2]
+                    const int32_t Mp = max(max(D, X), I);
+                    const bool valid = (uint32_t)Mp < (uint32_t)mt1;
                     if constexpr (BT) {
-                        const uint32_t r = rb | (valid ? (((uint32_t)M >> (U - 1)) << 4) | ((X >= D && X >= I) ? 1u : (D >= I ? 2u : 3u)) : 0u);
+                        const uint32_t r = rb | (valid ? (((uint32_t)Mp >> (U - 1)) << 4) | ((X >= D && X >= I) ? 1u : (D >= I ? 2u : 3u)) : 0u);
                         recs[(uint32_t)(64 * NW * c) + rl] = r;
                     }
-                    M = valid ? M : NUL;
+                    int32_t M = valid ? Mp : NUL;
                     if constexpr (DBL) {
                         const int lim2 = mt1 - 2 - M;  // 2 x bases left
                         if (valid && lim2 > 0) M += extend2((uint32_t)(M - kl - 128 * NW * c), (uint32_t)M, lim2);
@@ -332,22 +332,16 @@ __global__ __launch_bounds__(NW * 64) void wfa_regml_kernel(KArgs a) {
                     Ir[c] = I;
                     Dr[c] = D;
                     epub[c * NW] = lane == 0 ? D : I;
-                    uint32_t fl = b ? 2u << c : 0u;
-                    asm volatile("" : "+s"(fl));
-                    acc |= fl;
-                    if (c == 0 && w == 0) edge |= (uint32_t)(b & 1ull);
-                    if (c == C - 1 && w == NW - 1) edge |= (uint32_t)(b >> 63);
+                    if (c == 0) edge |= (uint32_t)(vote(Mp >= 0) & 1ull) & w0;
+                    if (c == C - 1) edge |= (uint32_t)(vote(Mp >= 0) >> 63) & wlast;
                     if (em & (1u << c)) mend = __builtin_amdgcn_readlane(M, ltp);
                 }
             }
-            const uint32_t Fs = acc >> 1;
             if (mend >= U * t) atomicMin(X + L::EV, (s << 2) | kEvEnd);  // uniform conditions and values
             if (edge) atomicMin(X + L::EV, (s << 2) | kEvEdge);
-            if (lane == 0) FWp[l5 * NW + w] = Fs;
-            // this step's ring entries, edges and mask before the progress word
+            // this step's ring entries and edges before the progress word
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
             if (lane == 0) __hip_atomic_store(X + L::PG + w, s, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-            F5 = F4; F4 = F3; F3 = F2; F2 = F1; F1 = Fs;
             return 0;
         };
 
