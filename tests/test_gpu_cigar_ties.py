@@ -50,6 +50,7 @@ CASES_462 = TIES_462 + HOMOPOLYMER_462
 PATHS_462 = [
     ("one wave, 1 chunk", 0, "reg1"),
     ("one wave, 8 chunks", 300, "reg1"),
+    ("one wave, 16 chunks", 700, "reg1"),
     ("16 chunks x 4 waves", 2000, "ml4"),
     ("16 chunks x 8 waves", 5000, "ml8"),
     ("16 chunks x 16 waves, doubled offsets", 10000, "ml16"),

@@ -1229,25 +1229,35 @@ static int copy_to_host(tsu_ctx* ctx, void* dst, const void* src, size_t bytes) 
     return TSU_OK;
 }
 
+namespace {
+
+// Chunks of the single-wave recording kernel for capacity W (0: none): the
+// register windows of up to 512 diagonals, and 1024 (16 chunks), which only
+// the recording instance takes.
+int bt_reg_chunks(const tsu_ctx* ctx, int32_t W) {
+    if (int c = tsu_host::reg_chunks(ctx, W)) return c;
+    return tsu_host::fixed_242(ctx) && W <= 1024 ? 16 : 0;
+}
+
 // Kernel family of a CIGAR launch at capacity W for a pair of length len
 // and scaled score s: 1 the single-wave register kernel, 2 + waves the
 // 16-chunk kernel, 0 the LDS kernel.
-static int bt_family(const tsu_ctx* ctx, int32_t W, int32_t len, int64_t s) {
-    if (tsu_host::reg_chunks(ctx, W) > 0 && len <= 16000 && s <= 16000) return 1;
+int bt_family(const tsu_ctx* ctx, int32_t W, int32_t len, int64_t s) {
+    if (bt_reg_chunks(ctx, W) > 0 && len <= 16000 && s <= 16000) return 1;
     if (int nw = tsu_host::ml_waves(ctx, W, len)) return 2 + nw;
     return 0;
 }
 
 // Highest scaled score a pair of lengths (p, t) can have: min(p, t)
 // mismatches and one gap of |p - t|.
-static int64_t score_bound(const tsu_ctx* ctx, int32_t p, int32_t t) {
+int64_t score_bound(const tsu_ctx* ctx, int32_t p, int32_t t) {
     const int64_t d = std::abs((int64_t)p - t);
     return (int64_t)ctx->xs * std::min(p, t) + (d ? ctx->os + ctx->es * d : 0);
 }
 
 // Highest scaled score whose diagonal range fits a window of W diagonals
 // (-1 if none does).
-static int64_t score_for_capacity(const tsu_ctx* ctx, int32_t W) {
+int64_t score_for_capacity(const tsu_ctx* ctx, int32_t W) {
     int64_t lo = -1, hi = 1ll << 30;
     while (hi - lo > 1) {
         const int64_t mid = lo + (hi - lo) / 2;
@@ -1256,447 +1266,513 @@ static int64_t score_for_capacity(const tsu_ctx* ctx, int32_t W) {
     return lo;
 }
 
-extern "C" int tsu_align_cigar(tsu_ctx* ctx, tsu_batch* b, int32_t* scores, tsu_cigars** out, tsu_timing* t) {
-    if (!ctx || !b || !scores || !out || b->ctx != ctx) return fail(ctx, TSU_EINVAL, "tsu_align_cigar: bad arguments");
-    *out = nullptr;
-    if (ctx->two_piece) return fail(ctx, TSU_EINVAL, "tsu_align_cigar: not available for 2-piece penalties");
-    HIPCHK(hipSetDevice(ctx->device), "hipSetDevice");
-    if (t) std::memset(t, 0, sizeof(*t));
-    const double w0 = now_ms();
-    // freed on every early return; handed to the caller at the end
-    std::unique_ptr<tsu_cigars> cg(new (std::nothrow) tsu_cigars());
-    if (!cg) return fail(ctx, TSU_ENOMEM, "host allocation");
-    cg->off.assign((size_t)b->n, 0);
-    cg->n.assign((size_t)b->n, 0);
-    // a walk reserves 2 * (s / unit) + 2 run words for its scaled score s
-    const int32_t unit = std::min(ctx->xs, ctx->os + ctx->es);
+// One length bucket of the record pass, at its capacity.
+struct RecordPlan {
+    std::vector<int32_t> ids;
+    Shape sh;
+    KArgs a;
+    int32_t W, max_len, step_cap, win;
+    int64_t recs, per_wg;
+    bool at_window;  // the step bound is the window's: a pair past it needs a wider one
+};
+
+// The LDS-kernel launch for pairs wider than every register window.
+struct WidePlan {
+    Shape sh;
+    KArgs a;
+    int32_t W, max_len, step_cap;
+    int64_t recs, per_wg, runs;
+};
+
+// That launch on the second stream, beside the record launches.
+struct WideAsync {
+    DevBuf<int32_t> d_ids;
+    DevBuf<uint8_t> ring;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    bool launched = false;
+    hipStream_t s = nullptr;
+    ~WideAsync() {
+        if (s) (void)hipStreamSynchronize(s);
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+    }
+};
+
+// One tsu_align_cigar call.  Three paths, each over the pairs the ones
+// before it left: record launches without a score pass (register kernels,
+// per length bucket at the batched form's capacity), one LDS-kernel launch
+// over the whole diagonal range for pairs wider than every register window,
+// and scores first (batched form) for the rest, which size the backtrace
+// state exactly.  The walks append their runs to one pool on the device.
+class CigarCall {
+public:
+    CigarCall(tsu_ctx* c, tsu_batch* bt, int32_t* sc, tsu_timing* tm) : ctx(c), b(bt), scores(sc), t(tm) {}
+
+    int run(tsu_cigars** out) {
+        int r = setup();
+        if (r == TSU_OK && ctx->cigar_mode != TSU_CIGAR_CHECKPOINTS && tsu_host::fixed_242(ctx)) r = record_pass();
+        if (r == TSU_OK) r = wide_pass();
+        if (r == TSU_OK) r = score_first_pass();
+        if (r == TSU_OK) r = collect(out);
+        return r;
+    }
+
+private:
+    int setup();
+    int launch_bt(const Shape& sh, KArgs& a, int32_t W, const std::vector<int32_t>& ids, int32_t max_len, int64_t recs,
+                  int32_t step_cap, int64_t per_wg);
+    int grow_runs(int64_t more);
+    WidePlan plan_wide(const std::vector<int32_t>& ids) const;
+    int settle_wide(const std::vector<int32_t>& ids);
+    int launch_wide_async(WidePlan& wp, const std::vector<int32_t>& ids);
+    void plan_records(std::vector<RecordPlan>& plans, int64_t& runs_need);
+    int settle_plan(const RecordPlan& pl);
+    int record_pass();
+    int wide_pass();
+    int score_first_pass();
+    int collect(tsu_cigars** out);
+
+    tsu_ctx* ctx;
+    tsu_batch* b;
+    int32_t* scores;
+    tsu_timing* t;
+    double w0 = 0;
+    std::unique_ptr<tsu_cigars> cg;  // handed to the caller at the end
+    int32_t unit = 1;                // a walk reserves 2 * (s / unit) + 2 run words for its scaled score s
     DevBuf<uint32_t> d_runs;
     DevBuf<unsigned long long> d_used;
     DevBuf<int64_t> d_run_off;
     DevBuf<int32_t> d_run_n;
     DevBuf<int32_t> d_order;
+    int64_t runs_cap = 0;  // words in d_runs
+    double kms_total = 0;
+    int launches = 0;
+    int64_t budget = 0;
+    std::vector<int32_t> sc2;      // scores the backtrace launches returned
+    std::vector<uint8_t> done;
+    std::vector<int32_t> wide;     // pairs that overflowed the widest register window
+    std::vector<int32_t> order;    // pairs of the score-first path
+    WideAsync wide_async;          // declared after the buffers its launch uses: synchronised first
+};
+
+int CigarCall::setup() {
+    HIPCHK(hipSetDevice(ctx->device), "hipSetDevice");
+    if (t) std::memset(t, 0, sizeof(*t));
+    w0 = now_ms();
+    cg.reset(new (std::nothrow) tsu_cigars());
+    if (!cg) return fail(ctx, TSU_ENOMEM, "host allocation");
+    cg->off.assign((size_t)b->n, 0);
+    cg->n.assign((size_t)b->n, 0);
+    unit = std::min(ctx->xs, ctx->os + ctx->es);
     HIPCHK(d_used.alloc(1), "hipMalloc");
     HIPCHK(d_run_off.alloc((size_t)std::max(b->n, 1)), "hipMalloc");
     HIPCHK(d_run_n.alloc((size_t)std::max(b->n, 1)), "hipMalloc");
     HIPCHK(d_order.alloc((size_t)std::max(b->n, 1)), "hipMalloc");
     HIPCHK(hipMemsetAsync(d_used.p, 0, sizeof(unsigned long long), ctx->stream), "hipMemsetAsync");
-    int64_t runs_cap = 0;  // words in d_runs
-    double kms_total = 0;
-    int launches = 0;
     size_t free_b = 0, total_b = 0;
     HIPCHK(hipMemGetInfo(&free_b, &total_b), "hipMemGetInfo");
     // the state this context already holds counts as free
     for (int i = 0; i < 7; ++i) free_b += ctx->bt_bytes[i];
-    const int64_t budget = ctx->cigar_budget > 0 ? ctx->cigar_budget : (int64_t)(0.5 * (double)free_b);
-    // One backtrace launch of `ids` with `recs` record words and `step_cap`
-    // scores of state per workgroup (per_wg bytes in all).
-    auto launch_bt = [&](const Shape& sh, KArgs& a, int32_t W, const std::vector<int32_t>& ids, int32_t max_len,
-                         int64_t recs, int32_t step_cap, int64_t per_wg) -> int {
-        int per_cu = 0;
-        int rr = tsu::occupancy(sh, tsu::lds_bytes(sh, a), &per_cu);
-        if (rr != 0 || per_cu <= 0) return fail(ctx, TSU_EINTERNAL, "backtrace shape has no occupancy");
-        int64_t grid = std::min<int64_t>((int64_t)per_cu * ctx->cu_count, (int64_t)ids.size());
-        grid = std::max<int64_t>(1, std::min<int64_t>(grid, budget / std::max<int64_t>(per_wg, 1)));
-        cg->state_bytes = std::max(cg->state_bytes, per_wg * grid);
-        if (sh.ck) ++cg->checkpoint_launches; else ++cg->record_launches;
-        cg->kernels |= tsu_host::kernel_bit(sh);
-        void *rec = nullptr, *slo = nullptr, *soff = nullptr, *ckb = nullptr;
-        HIPCHK(bt_reserve(ctx, 0, (size_t)(recs * grid) * 4, &rec), "hipMalloc (records)");
-        HIPCHK(bt_reserve(ctx, 1, (size_t)((int64_t)step_cap * grid) * 4, &slo), "hipMalloc (steps)");
-        HIPCHK(bt_reserve(ctx, 2, (size_t)((int64_t)step_cap * grid) * 8, &soff), "hipMalloc (steps)");
-        if (sh.ck) {
-            a.ck_stride = (int64_t)a.ck_cap * a.ck_bytes;
-            HIPCHK(bt_reserve(ctx, 3, (size_t)(a.ck_stride * grid), &ckb), "hipMalloc (checkpoints)");
-            a.ck_g = static_cast<uint8_t*>(ckb);
-        }
-        HIPCHK(hipMemcpyAsync(d_order.p, ids.data(), ids.size() * sizeof(int32_t), hipMemcpyHostToDevice, ctx->stream), "hipMemcpyAsync");
-        a.rec_g = static_cast<uint32_t*>(rec); a.rec_cap = recs;
-        a.step_lo_g = static_cast<int32_t*>(slo); a.step_off_g = static_cast<int64_t*>(soff); a.step_cap = step_cap;
-        a.runs = d_runs.p; a.runs_cap = std::max<int64_t>(runs_cap, 1); a.runs_used = d_used.p;
-        a.run_off = d_run_off.p; a.run_n = d_run_n.p;
-        double kms = 0;
-        int r = run_shape(ctx, io_of(b), sh, W, d_order.p, (int32_t)ids.size(), max_len, grid, &a, &kms);
-        if (r != TSU_OK) return r;
-        kms_total += kms;
-        ++launches;
-        return TSU_OK;
-    };
-    // makes room in the run pool for `more` words after what was reserved so far
-    auto grow_runs = [&](int64_t more) -> int {
-        unsigned long long used0 = 0;
-        HIPCHK(hipMemcpy(&used0, d_used.p, sizeof(used0), hipMemcpyDeviceToHost), "hipMemcpy D2H");
-        const int64_t need = (int64_t)used0 + more;
-        if (need > runs_cap) {
-            DevBuf<uint32_t> grown;
-            HIPCHK(grown.alloc((size_t)need), "hipMalloc (runs)");
-            if (used0) HIPCHK(hipMemcpy(grown.p, d_runs.p, (size_t)used0 * 4, hipMemcpyDeviceToDevice), "hipMemcpy D2D");
-            std::swap(grown.p, d_runs.p);
-            std::swap(grown.n, d_runs.n);
-            runs_cap = need;
-        }
-        return TSU_OK;
-    };
-    std::vector<int32_t> sc2((size_t)b->n);
-    std::vector<uint8_t> done((size_t)b->n, 0);
-    std::vector<int32_t> wide;  // pairs that overflowed the widest register window
+    budget = ctx->cigar_budget > 0 ? ctx->cigar_budget : (int64_t)(0.5 * (double)free_b);
+    sc2.assign((size_t)b->n, 0);
+    done.assign((size_t)b->n, 0);
+    return TSU_OK;
+}
 
-    // Pairs wider than every register window (they overflowed the 16,384
-    // diagonals of the 16-chunk kernel): one LDS-kernel launch over the whole
-    // diagonal range of their lengths, records sized for the highest score
-    // the lengths allow, when that fits the budget (a few long divergent
-    // pairs, each alone on a CU: a score pass first would double their time).
-    struct WidePlan {
-        Shape sh;
-        KArgs a;
-        int32_t W, max_len, step_cap;
-        int64_t recs, per_wg, runs;
-    };
-    auto plan_wide = [&](const std::vector<int32_t>& ids) -> WidePlan {
-        WidePlan wp{};
+// One backtrace launch of `ids` with `recs` record words and `step_cap`
+// scores of state per workgroup (per_wg bytes in all).
+int CigarCall::launch_bt(const Shape& sh, KArgs& a, int32_t W, const std::vector<int32_t>& ids, int32_t max_len,
+                         int64_t recs, int32_t step_cap, int64_t per_wg) {
+    int per_cu = 0;
+    int rr = tsu::occupancy(sh, tsu::lds_bytes(sh, a), &per_cu);
+    if (rr != 0 || per_cu <= 0) return fail(ctx, TSU_EINTERNAL, "backtrace shape has no occupancy");
+    int64_t grid = std::min<int64_t>((int64_t)per_cu * ctx->cu_count, (int64_t)ids.size());
+    grid = std::max<int64_t>(1, std::min<int64_t>(grid, budget / std::max<int64_t>(per_wg, 1)));
+    cg->state_bytes = std::max(cg->state_bytes, per_wg * grid);
+    if (sh.ck) ++cg->checkpoint_launches; else ++cg->record_launches;
+    cg->kernels |= tsu_host::kernel_bit(sh);
+    void *rec = nullptr, *slo = nullptr, *soff = nullptr, *ckb = nullptr;
+    HIPCHK(bt_reserve(ctx, 0, (size_t)(recs * grid) * 4, &rec), "hipMalloc (records)");
+    HIPCHK(bt_reserve(ctx, 1, (size_t)((int64_t)step_cap * grid) * 4, &slo), "hipMalloc (steps)");
+    HIPCHK(bt_reserve(ctx, 2, (size_t)((int64_t)step_cap * grid) * 8, &soff), "hipMalloc (steps)");
+    if (sh.ck) {
+        a.ck_stride = (int64_t)a.ck_cap * a.ck_bytes;
+        HIPCHK(bt_reserve(ctx, 3, (size_t)(a.ck_stride * grid), &ckb), "hipMalloc (checkpoints)");
+        a.ck_g = static_cast<uint8_t*>(ckb);
+    }
+    HIPCHK(hipMemcpyAsync(d_order.p, ids.data(), ids.size() * sizeof(int32_t), hipMemcpyHostToDevice, ctx->stream), "hipMemcpyAsync");
+    a.rec_g = static_cast<uint32_t*>(rec); a.rec_cap = recs;
+    a.step_lo_g = static_cast<int32_t*>(slo); a.step_off_g = static_cast<int64_t*>(soff); a.step_cap = step_cap;
+    a.runs = d_runs.p; a.runs_cap = std::max<int64_t>(runs_cap, 1); a.runs_used = d_used.p;
+    a.run_off = d_run_off.p; a.run_n = d_run_n.p;
+    double kms = 0;
+    int r = run_shape(ctx, io_of(b), sh, W, d_order.p, (int32_t)ids.size(), max_len, grid, &a, &kms);
+    if (r != TSU_OK) return r;
+    kms_total += kms;
+    ++launches;
+    return TSU_OK;
+}
+
+// Makes room in the run pool for `more` words after what was reserved so far.
+int CigarCall::grow_runs(int64_t more) {
+    unsigned long long used0 = 0;
+    HIPCHK(hipMemcpy(&used0, d_used.p, sizeof(used0), hipMemcpyDeviceToHost), "hipMemcpy D2H");
+    const int64_t need = (int64_t)used0 + more;
+    if (need > runs_cap) {
+        DevBuf<uint32_t> grown;
+        HIPCHK(grown.alloc((size_t)need), "hipMalloc (runs)");
+        if (used0) HIPCHK(hipMemcpy(grown.p, d_runs.p, (size_t)used0 * 4, hipMemcpyDeviceToDevice), "hipMemcpy D2D");
+        std::swap(grown.p, d_runs.p);
+        std::swap(grown.n, d_runs.n);
+        runs_cap = need;
+    }
+    return TSU_OK;
+}
+
+// Pairs wider than every register window (they overflowed the 16,384
+// diagonals of the 16-chunk kernel): one LDS-kernel launch over the whole
+// diagonal range of their lengths, records sized for the highest score the
+// lengths allow, when that fits the budget (a few long divergent pairs, each
+// alone on a CU: a score pass first would double their time).
+WidePlan CigarCall::plan_wide(const std::vector<int32_t>& ids) const {
+    WidePlan wp{};
+    int64_t s_len = 0;
+    wp.max_len = 0;
+    wp.runs = 0;
+    for (int32_t i : ids) {
+        const tsu_pair& pr = b->pairs[(size_t)i];
+        wp.max_len = std::max(wp.max_len, std::max(pr.p_len, pr.t_len));
+        const int64_t sb = score_bound(ctx, pr.p_len, pr.t_len);
+        s_len = std::max(s_len, sb);
+        wp.runs += 2 * (sb / unit) + 2;
+    }
+    wp.W = capacity_full(wp.max_len, wp.max_len);
+    wp.step_cap = (int32_t)std::min<int64_t>(s_len + 2, 1 << 30);
+    wp.recs = 64;
+    for (int64_t s = 0; s <= s_len; ++s)
+        wp.recs += std::min<int64_t>(wp.W, 2 * (std::max<int64_t>(0, s - ctx->os) / ctx->es + 2) + 1);
+    wp.a = base_args(ctx);
+    wp.a.W = wp.W;
+    wp.a.seq_words = seq_words_for(wp.max_len);
+    wp.sh = Shape{default_waves(wp.W), wp.max_len <= 32000, true, true, true};
+    wp.sh.fixed_242 = tsu_host::fixed_242(ctx);
+    if (tsu::lds_bytes(wp.sh, wp.a) > ctx->max_lds) { wp.sh.lds_ring = false; wp.sh.nw = 16; }
+    wp.per_wg = wp.recs * 4 + (int64_t)wp.step_cap * 12 +
+                (wp.sh.lds_ring ? 0 : tsu::ring_bytes(wp.sh, wp.W, ctx->rm, ctx->ri));
+    return wp;
+}
+
+int CigarCall::settle_wide(const std::vector<int32_t>& ids) {
+    for (int32_t i : ids) {
+        const int32_t v = sc2[(size_t)i];
+        if (v >= 0) { scores[i] = v; done[(size_t)i] = 1; }
+        else if (v != TSU_EKOVERFLOW && v != TSU_ECIGAR)
+            return fail(ctx, v, "pair " + std::to_string(i) + ": " + tsu_strerror(v));
+    }
+    return TSU_OK;
+}
+
+// The wide launch on the second stream, beside the main stream's launches,
+// with state of its own (context slots 4-6; the run pool was reserved for
+// these pairs up front).  Leaves wide_async.launched false when that state
+// cannot be allocated now: the launch then runs afterwards.
+int CigarCall::launch_wide_async(WidePlan& wp, const std::vector<int32_t>& ids) {
+    int per_cu = 0;
+    if (tsu::occupancy(wp.sh, tsu::lds_bytes(wp.sh, wp.a), &per_cu) != 0 || per_cu <= 0) return TSU_OK;
+    int64_t grid = std::min<int64_t>((int64_t)per_cu * ctx->cu_count, (int64_t)ids.size());
+    grid = std::max<int64_t>(1, std::min<int64_t>(grid, budget / std::max<int64_t>(wp.per_wg, 1)));
+    void *rec = nullptr, *slo = nullptr, *soff = nullptr;
+    if (bt_reserve(ctx, 4, (size_t)(wp.recs * grid) * 4, &rec) != hipSuccess ||
+        bt_reserve(ctx, 5, (size_t)((int64_t)wp.step_cap * grid) * 4, &slo) != hipSuccess ||
+        bt_reserve(ctx, 6, (size_t)((int64_t)wp.step_cap * grid) * 8, &soff) != hipSuccess ||
+        wide_async.d_ids.alloc(ids.size()) != hipSuccess) {
+        (void)hipGetLastError();  // out of memory now: run it afterwards instead
+        return TSU_OK;
+    }
+    HIPCHK(hipEventCreate(&wide_async.e0), "hipEventCreate");
+    HIPCHK(hipEventCreate(&wide_async.e1), "hipEventCreate");
+    HIPCHK(hipMemcpyAsync(wide_async.d_ids.p, ids.data(), ids.size() * sizeof(int32_t), hipMemcpyHostToDevice, ctx->stream2),
+           "hipMemcpyAsync");
+    KArgs a = wp.a;
+    a.rec_g = static_cast<uint32_t*>(rec); a.rec_cap = wp.recs;
+    a.step_lo_g = static_cast<int32_t*>(slo); a.step_off_g = static_cast<int64_t*>(soff); a.step_cap = wp.step_cap;
+    a.runs = d_runs.p; a.runs_cap = std::max<int64_t>(runs_cap, 1); a.runs_used = d_used.p;
+    a.run_off = d_run_off.p; a.run_n = d_run_n.p;
+    wide_async.s = ctx->stream2;
+    int r = launch_shape(ctx, io_of(b), wp.sh, wp.W, wide_async.d_ids.p, (int32_t)ids.size(), wp.max_len, grid, &a, ctx->stream2,
+                         ctx->d_queue + 16, wide_async.e0, wide_async.e1, wide_async.ring, &wide_async.launched);
+    if (r != TSU_OK) return r;
+    cg->state_bytes = std::max(cg->state_bytes, wp.per_wg * grid);
+    ++cg->record_launches;
+    cg->kernels |= tsu_host::kernel_bit(wp.sh);
+    ++launches;
+    return TSU_OK;
+}
+
+// Record launches without a score pass, per length bucket of the batched
+// form at its capacity, where a register kernel records: the state is sized
+// for the highest score the window (or the lengths) allow.  A pair whose
+// window overflows, or whose score passes that step bound, is left to the
+// paths after.
+void CigarCall::plan_records(std::vector<RecordPlan>& plans, int64_t& runs_need) {
+    const int nb = tsu_host::bucket_count();
+    std::vector<std::vector<int32_t>> bk((size_t)nb);
+    for (int32_t i = 0; i < b->n; ++i)
+        bk[(size_t)tsu_host::bucket_of(std::max(b->pairs[(size_t)i].p_len, b->pairs[(size_t)i].t_len))].push_back(i);
+    runs_need = 0;
+    for (int bi = nb - 1; bi >= 0; --bi) {
+        std::vector<int32_t>& ids = bk[(size_t)bi];
+        if (ids.empty()) continue;
+        int32_t max_len = 0;
         int64_t s_len = 0;
-        wp.max_len = 0;
-        wp.runs = 0;
         for (int32_t i : ids) {
             const tsu_pair& pr = b->pairs[(size_t)i];
-            wp.max_len = std::max(wp.max_len, std::max(pr.p_len, pr.t_len));
-            const int64_t sb = score_bound(ctx, pr.p_len, pr.t_len);
-            s_len = std::max(s_len, sb);
-            wp.runs += 2 * (sb / unit) + 2;
+            max_len = std::max(max_len, std::max(pr.p_len, pr.t_len));
+            s_len = std::max(s_len, score_bound(ctx, pr.p_len, pr.t_len));
         }
-        wp.W = capacity_full(wp.max_len, wp.max_len);
-        wp.step_cap = (int32_t)std::min<int64_t>(s_len + 2, 1 << 30);
-        wp.recs = 64;
-        for (int64_t s = 0; s <= s_len; ++s)
-            wp.recs += std::min<int64_t>(wp.W, 2 * (std::max<int64_t>(0, s - ctx->os) / ctx->es + 2) + 1);
-        wp.a = base_args(ctx);
-        wp.a.W = wp.W;
-        wp.a.seq_words = seq_words_for(wp.max_len);
-        wp.sh = Shape{default_waves(wp.W), wp.max_len <= 32000, true, true, true};
-        wp.sh.fixed_242 = tsu_host::fixed_242(ctx);
-        if (tsu::lds_bytes(wp.sh, wp.a) > ctx->max_lds) { wp.sh.lds_ring = false; wp.sh.nw = 16; }
-        wp.per_wg = wp.recs * 4 + (int64_t)wp.step_cap * 12 +
-                    (wp.sh.lds_ring ? 0 : tsu::ring_bytes(wp.sh, wp.W, ctx->rm, ctx->ri));
-        return wp;
-    };
-    auto settle_wide = [&](const std::vector<int32_t>& ids) -> int {
-        for (int32_t i : ids) {
-            const int32_t v = sc2[(size_t)i];
-            if (v >= 0) { scores[i] = v; done[(size_t)i] = 1; }
-            else if (v != TSU_EKOVERFLOW && v != TSU_ECIGAR)
-                return fail(ctx, v, "pair " + std::to_string(i) + ": " + tsu_strerror(v));
+        const int32_t bound = tsu_host::bucket_bound(bi);
+        const tsu_tune_entry* te = tune_lookup(ctx, bound);
+        if (te && te->waves_per_pair > 1) continue;  // tuned for the LDS kernel
+        int32_t W = (te && te->capacity > 0) ? te->capacity : initial_capacity(ctx, std::min(bound, max_len));
+        W = std::min<int32_t>(W, capacity_full(max_len, max_len));
+        Shape sh{default_waves(W), max_len <= 32000, true, true, true};
+        sh.fixed_242 = true;
+        // the kernel's own window: 64 diagonals per chunk
+        int32_t win = 0;
+        const int c1 = bt_reg_chunks(ctx, W);
+        if (c1 > 0 && max_len <= 16000) {
+            sh.nw = 1;
+            sh.reg_c = c1;
+            sh.lring = true;
+            win = 64 * c1;
+        } else if (int nwl = tsu_host::ml_waves(ctx, W, max_len)) {
+            sh.nw = nwl;
+            sh.reg_c = 16;
+            win = 1024 * nwl;
+        } else {
+            continue;  // the LDS kernel records per score's width: sized from scores
         }
-        return TSU_OK;
-    };
-    // The same launch on the second stream, beside the main stream's
-    // launches, with state of its own (context slots 4-6; the run pool was
-    // reserved for these pairs up front).  Leaves `launched` false when that
-    // state cannot be allocated now: the launch then runs afterwards.
-    struct WideAsync {
-        DevBuf<int32_t> d_ids;
-        DevBuf<uint8_t> ring;
-        hipEvent_t e0 = nullptr, e1 = nullptr;
-        bool launched = false;
-        hipStream_t s = nullptr;
-        ~WideAsync() {
-            if (s) (void)hipStreamSynchronize(s);
-            if (e0) (void)hipEventDestroy(e0);
-            if (e1) (void)hipEventDestroy(e1);
-        }
-    } wide_async;
-    auto launch_wide_async = [&](WidePlan& wp, const std::vector<int32_t>& ids) -> int {
+        // a window holding every diagonal of the longest pair never overflows
+        int64_t s_cap = win >= capacity_full(max_len, max_len) ? s_len : std::min(s_len, score_for_capacity(ctx, win));
+        if (s_cap < 0 || (sh.nw == 1 && s_cap > 16000)) continue;
+        KArgs a = base_args(ctx);
+        a.W = W;
+        a.seq_words = seq_words_for(max_len);
         int per_cu = 0;
-        if (tsu::occupancy(wp.sh, tsu::lds_bytes(wp.sh, wp.a), &per_cu) != 0 || per_cu <= 0) return TSU_OK;
-        int64_t grid = std::min<int64_t>((int64_t)per_cu * ctx->cu_count, (int64_t)ids.size());
-        grid = std::max<int64_t>(1, std::min<int64_t>(grid, budget / std::max<int64_t>(wp.per_wg, 1)));
-        void *rec = nullptr, *slo = nullptr, *soff = nullptr;
-        if (bt_reserve(ctx, 4, (size_t)(wp.recs * grid) * 4, &rec) != hipSuccess ||
-            bt_reserve(ctx, 5, (size_t)((int64_t)wp.step_cap * grid) * 4, &slo) != hipSuccess ||
-            bt_reserve(ctx, 6, (size_t)((int64_t)wp.step_cap * grid) * 8, &soff) != hipSuccess ||
-            wide_async.d_ids.alloc(ids.size()) != hipSuccess) {
-            (void)hipGetLastError();  // out of memory now: run it afterwards instead
-            return TSU_OK;
+        if (tsu::occupancy(sh, tsu::lds_bytes(sh, a), &per_cu) != 0 || per_cu <= 0) continue;
+        // the launch's state (records of every resident pair) stays within
+        // the budget: fewer steps if needed, unless that leaves less than a
+        // third of the window's scores
+        const int64_t resident = std::min<int64_t>((int64_t)per_cu * ctx->cu_count, (int64_t)ids.size());
+        const int64_t step_bytes = (int64_t)win * 4 + 12;
+        const int64_t fit = budget / (resident * step_bytes) - 2;
+        const int64_t s_win = s_cap;
+        if (fit < s_cap) {
+            if (fit * 3 < s_cap) continue;
+            s_cap = fit;
         }
-        HIPCHK(hipEventCreate(&wide_async.e0), "hipEventCreate");
-        HIPCHK(hipEventCreate(&wide_async.e1), "hipEventCreate");
-        HIPCHK(hipMemcpyAsync(wide_async.d_ids.p, ids.data(), ids.size() * sizeof(int32_t), hipMemcpyHostToDevice, ctx->stream2),
-               "hipMemcpyAsync");
-        KArgs a = wp.a;
-        a.rec_g = static_cast<uint32_t*>(rec); a.rec_cap = wp.recs;
-        a.step_lo_g = static_cast<int32_t*>(slo); a.step_off_g = static_cast<int64_t*>(soff); a.step_cap = wp.step_cap;
-        a.runs = d_runs.p; a.runs_cap = std::max<int64_t>(runs_cap, 1); a.runs_used = d_used.p;
-        a.run_off = d_run_off.p; a.run_n = d_run_n.p;
-        wide_async.s = ctx->stream2;
-        int r = launch_shape(ctx, io_of(b), wp.sh, wp.W, wide_async.d_ids.p, (int32_t)ids.size(), wp.max_len, grid, &a, ctx->stream2,
-                             ctx->d_queue + 16, wide_async.e0, wide_async.e1, wide_async.ring, &wide_async.launched);
-        if (r != TSU_OK) return r;
-        cg->state_bytes = std::max(cg->state_bytes, wp.per_wg * grid);
-        ++cg->record_launches;
-        cg->kernels |= tsu_host::kernel_bit(wp.sh);
-        ++launches;
-        return TSU_OK;
-    };
+        const int32_t step_cap = (int32_t)(s_cap + 2);
+        const int64_t recs = (int64_t)step_cap * win;
+        if (sh.nw > 1) sh.dbl = max_len <= tsu::kMlDblMaxLen && s_cap <= tsu::kMlDblMaxScore;
+        const int64_t per_wg = recs * 4 + (int64_t)step_cap * 12;
+        // pairs of the widest window may also be walked by the launch for
+        // wider pairs, beside later launches: room for their whole bound
+        for (int32_t i : ids) {
+            const tsu_pair& pr = b->pairs[(size_t)i];
+            const int64_t sb = score_bound(ctx, pr.p_len, pr.t_len);
+            runs_need += 2 * ((win >= 16 * 1024 ? sb : std::min(s_cap + 1, sb)) / unit) + 2;
+        }
+        tsu_host::order_longest_first(ids, b->pairs.data());
+        plans.push_back(RecordPlan{std::move(ids), sh, a, W, max_len, step_cap, win, recs, per_wg,
+                                   s_cap == s_win && s_win < s_len});
+    }
+}
 
-    // 1) Record launches without a score pass, per length bucket of the
-    // batched form at its capacity, where a register kernel records: the
-    // state is sized for the highest score the window (or the lengths)
-    // allow.  A pair whose window overflows, or whose score passes that
-    // step bound, is left to the score-first path below.
-    if (ctx->cigar_mode != TSU_CIGAR_CHECKPOINTS && tsu_host::fixed_242(ctx)) {
-        struct Plan {
-            std::vector<int32_t> ids;
-            Shape sh;
-            KArgs a;
-            int32_t W, max_len, step_cap, win;
-            int64_t recs, per_wg;
-            bool at_window;  // the step bound is the window's: a pair past it needs a wider one
-        };
-        std::vector<Plan> plans;
-        const int nb = tsu_host::bucket_count();
-        std::vector<std::vector<int32_t>> bk((size_t)nb);
-        for (int32_t i = 0; i < b->n; ++i)
-            bk[(size_t)tsu_host::bucket_of(std::max(b->pairs[(size_t)i].p_len, b->pairs[(size_t)i].t_len))].push_back(i);
-        int64_t runs_need = 0;
-        for (int bi = nb - 1; bi >= 0; --bi) {
-            std::vector<int32_t>& ids = bk[(size_t)bi];
-            if (ids.empty()) continue;
-            int32_t max_len = 0;
-            int64_t s_len = 0;
-            for (int32_t i : ids) {
-                const tsu_pair& pr = b->pairs[(size_t)i];
-                max_len = std::max(max_len, std::max(pr.p_len, pr.t_len));
-                s_len = std::max(s_len, score_bound(ctx, pr.p_len, pr.t_len));
-            }
-            const int32_t bound = tsu_host::bucket_bound(bi);
-            const tsu_tune_entry* te = tune_lookup(ctx, bound);
-            if (te && te->waves_per_pair > 1) continue;  // tuned for the LDS kernel
-            int32_t W = (te && te->capacity > 0) ? te->capacity : initial_capacity(ctx, std::min(bound, max_len));
-            W = std::min<int32_t>(W, capacity_full(max_len, max_len));
-            Shape sh{default_waves(W), max_len <= 32000, true, true, true};
-            sh.fixed_242 = true;
-            // the kernel's own window: 64 diagonals per chunk
-            int32_t win = 0;
-            const int c1 = tsu_host::reg_chunks(ctx, W);
-            if (c1 > 0 && max_len <= 16000) {
-                sh.nw = 1;
-                sh.reg_c = c1;
-                sh.lring = true;
-                win = 64 * c1;
-            } else if (int nwl = tsu_host::ml_waves(ctx, W, max_len)) {
-                sh.nw = nwl;
-                sh.reg_c = 16;
-                win = 1024 * nwl;
-            } else {
-                continue;  // the LDS kernel records per score's width: sized from scores
-            }
-            // a window holding every diagonal of the longest pair never overflows
-            int64_t s_cap = win >= capacity_full(max_len, max_len) ? s_len : std::min(s_len, score_for_capacity(ctx, win));
-            if (s_cap < 0 || (sh.nw == 1 && s_cap > 16000)) continue;
-            KArgs a = base_args(ctx);
-            a.W = W;
-            a.seq_words = seq_words_for(max_len);
-            int per_cu = 0;
-            if (tsu::occupancy(sh, tsu::lds_bytes(sh, a), &per_cu) != 0 || per_cu <= 0) continue;
-            // the launch's state (records of every resident pair) stays within
-            // the budget: fewer steps if needed, unless that leaves less than a
-            // third of the window's scores
-            const int64_t resident = std::min<int64_t>((int64_t)per_cu * ctx->cu_count, (int64_t)ids.size());
-            const int64_t step_bytes = (int64_t)win * 4 + 12;
-            const int64_t fit = budget / (resident * step_bytes) - 2;
-            const int64_t s_win = s_cap;
-            if (fit < s_cap) {
-                if (fit * 3 < s_cap) continue;
-                s_cap = fit;
-            }
-            const int32_t step_cap = (int32_t)(s_cap + 2);
-            const int64_t recs = (int64_t)step_cap * win;
-            if (sh.nw > 1) sh.dbl = max_len <= tsu::kMlDblMaxLen && s_cap <= tsu::kMlDblMaxScore;
-            const int64_t per_wg = recs * 4 + (int64_t)step_cap * 12;
-            // pairs of the widest window may also be walked by the launch for
-            // wider pairs, beside later launches: room for their whole bound
-            for (int32_t i : ids) {
-                const tsu_pair& pr = b->pairs[(size_t)i];
-                const int64_t sb = score_bound(ctx, pr.p_len, pr.t_len);
-                runs_need += 2 * ((win >= 16 * 1024 ? sb : std::min(s_cap + 1, sb)) / unit) + 2;
-            }
-            tsu_host::order_longest_first(ids, b->pairs.data());
-            plans.push_back(Plan{std::move(ids), sh, a, W, max_len, step_cap, win, recs, per_wg,
-                                 s_cap == s_win && s_win < s_len});
-        }
-        if (!plans.empty()) {
-            runs_cap = std::max<int64_t>(runs_need, 1);
-            HIPCHK(d_runs.alloc((size_t)runs_cap), "hipMalloc (runs)");
-            auto settle_plan = [&](const Plan& pl) -> int {
-                for (int32_t i : pl.ids) {
-                    const int32_t v = sc2[(size_t)i];
-                    if (v >= 0) { scores[i] = v; done[(size_t)i] = 1; }
-                    else if (pl.win >= 16 * 1024 && (v == TSU_EKOVERFLOW || (v == TSU_ECIGAR && pl.at_window)))
-                        wide.push_back(i);
-                    else if (v != TSU_EKOVERFLOW && v != TSU_ECIGAR)
-                        return fail(ctx, v, "pair " + std::to_string(i) + ": " + tsu_strerror(v));
-                }
-                return TSU_OK;
-            };
-            // the widest windows first: their pairs that need more are then
-            // known, and run on the second stream beside the other launches
-            size_t k = 0;
-            for (; k < plans.size() && plans[k].win >= 16 * 1024; ++k) {
-                int r = launch_bt(plans[k].sh, plans[k].a, plans[k].W, plans[k].ids, plans[k].max_len, plans[k].recs,
-                                  plans[k].step_cap, plans[k].per_wg);
-                if (r != TSU_OK) return r;
-            }
-            if (k > 0) {
-                HIPCHK(hipMemcpy(sc2.data(), b->d_scores, (size_t)b->n * sizeof(int32_t), hipMemcpyDeviceToHost), "hipMemcpy D2H");
-                for (size_t j = 0; j < k; ++j) {
-                    int r = settle_plan(plans[j]);
-                    if (r != TSU_OK) return r;
-                }
-            }
-            if (!wide.empty() && k < plans.size()) {
-                WidePlan wp = plan_wide(wide);
-                if (wp.per_wg * (int64_t)wide.size() <= budget) {
-                    int r = launch_wide_async(wp, wide);
-                    if (r != TSU_OK) return r;
-                }
-            }
-            for (size_t j = k; j < plans.size(); ++j) {
-                int r = launch_bt(plans[j].sh, plans[j].a, plans[j].W, plans[j].ids, plans[j].max_len, plans[j].recs,
-                                  plans[j].step_cap, plans[j].per_wg);
-                if (r != TSU_OK) return r;
-            }
-            if (wide_async.launched) {
-                HIPCHK(hipStreamSynchronize(ctx->stream2), "hipStreamSynchronize");
-                float ms = 0.f;
-                HIPCHK(hipEventElapsedTime(&ms, wide_async.e0, wide_async.e1), "hipEventElapsedTime");
-                kms_total += ms;
-            }
-            HIPCHK(hipMemcpy(sc2.data(), b->d_scores, (size_t)b->n * sizeof(int32_t), hipMemcpyDeviceToHost), "hipMemcpy D2H");
-            for (size_t j = k; j < plans.size(); ++j) {
-                int r = settle_plan(plans[j]);
-                if (r != TSU_OK) return r;
-            }
-            if (wide_async.launched) {
-                int r = settle_wide(wide);
-                if (r != TSU_OK) return r;
-                wide.clear();
-            }
+int CigarCall::settle_plan(const RecordPlan& pl) {
+    for (int32_t i : pl.ids) {
+        const int32_t v = sc2[(size_t)i];
+        if (v >= 0) { scores[i] = v; done[(size_t)i] = 1; }
+        else if (pl.win >= 16 * 1024 && (v == TSU_EKOVERFLOW || (v == TSU_ECIGAR && pl.at_window)))
+            wide.push_back(i);
+        else if (v != TSU_EKOVERFLOW && v != TSU_ECIGAR)
+            return fail(ctx, v, "pair " + std::to_string(i) + ": " + tsu_strerror(v));
+    }
+    return TSU_OK;
+}
+
+// 1) The record pass.  The widest windows go first: their pairs that need
+// more are then known, and run on the second stream beside the other
+// launches.
+int CigarCall::record_pass() {
+    std::vector<RecordPlan> plans;
+    int64_t runs_need = 0;
+    plan_records(plans, runs_need);
+    if (plans.empty()) return TSU_OK;
+    runs_cap = std::max<int64_t>(runs_need, 1);
+    HIPCHK(d_runs.alloc((size_t)runs_cap), "hipMalloc (runs)");
+    size_t k = 0;
+    for (; k < plans.size() && plans[k].win >= 16 * 1024; ++k) {
+        int r = launch_bt(plans[k].sh, plans[k].a, plans[k].W, plans[k].ids, plans[k].max_len, plans[k].recs,
+                          plans[k].step_cap, plans[k].per_wg);
+        if (r != TSU_OK) return r;
+    }
+    if (k > 0) {
+        HIPCHK(hipMemcpy(sc2.data(), b->d_scores, (size_t)b->n * sizeof(int32_t), hipMemcpyDeviceToHost), "hipMemcpy D2H");
+        for (size_t j = 0; j < k; ++j) {
+            int r = settle_plan(plans[j]);
+            if (r != TSU_OK) return r;
         }
     }
-
-    // 2) Pairs wider than every register window, when they did not run
-    // beside the other launches above.
-    if (!wide.empty()) {
+    if (!wide.empty() && k < plans.size()) {
         WidePlan wp = plan_wide(wide);
         if (wp.per_wg * (int64_t)wide.size() <= budget) {
-            int r = grow_runs(wp.runs);
-            if (r != TSU_OK) return r;
-            r = launch_bt(wp.sh, wp.a, wp.W, wide, wp.max_len, wp.recs, wp.step_cap, wp.per_wg);
-            if (r != TSU_OK) return r;
-            HIPCHK(hipMemcpy(sc2.data(), b->d_scores, (size_t)b->n * sizeof(int32_t), hipMemcpyDeviceToHost), "hipMemcpy D2H");
-            r = settle_wide(wide);
+            int r = launch_wide_async(wp, wide);
             if (r != TSU_OK) return r;
         }
     }
+    for (size_t j = k; j < plans.size(); ++j) {
+        int r = launch_bt(plans[j].sh, plans[j].a, plans[j].W, plans[j].ids, plans[j].max_len, plans[j].recs,
+                          plans[j].step_cap, plans[j].per_wg);
+        if (r != TSU_OK) return r;
+    }
+    if (wide_async.launched) {
+        HIPCHK(hipStreamSynchronize(ctx->stream2), "hipStreamSynchronize");
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, wide_async.e0, wide_async.e1), "hipEventElapsedTime");
+        kms_total += ms;
+    }
+    HIPCHK(hipMemcpy(sc2.data(), b->d_scores, (size_t)b->n * sizeof(int32_t), hipMemcpyDeviceToHost), "hipMemcpy D2H");
+    for (size_t j = k; j < plans.size(); ++j) {
+        int r = settle_plan(plans[j]);
+        if (r != TSU_OK) return r;
+    }
+    if (wide_async.launched) {
+        int r = settle_wide(wide);
+        if (r != TSU_OK) return r;
+        wide.clear();
+    }
+    return TSU_OK;
+}
 
-    // 3) The other pairs: scores first (batched form), which size the
-    // backtrace state exactly, then launches grouped by score.
-    std::vector<int32_t> order;
+// 2) Pairs wider than every register window, when they did not run beside
+// the record launches.
+int CigarCall::wide_pass() {
+    if (wide.empty()) return TSU_OK;
+    WidePlan wp = plan_wide(wide);
+    if (wp.per_wg * (int64_t)wide.size() > budget) return TSU_OK;  // scores first instead
+    int r = grow_runs(wp.runs);
+    if (r != TSU_OK) return r;
+    r = launch_bt(wp.sh, wp.a, wp.W, wide, wp.max_len, wp.recs, wp.step_cap, wp.per_wg);
+    if (r != TSU_OK) return r;
+    HIPCHK(hipMemcpy(sc2.data(), b->d_scores, (size_t)b->n * sizeof(int32_t), hipMemcpyDeviceToHost), "hipMemcpy D2H");
+    return settle_wide(wide);
+}
+
+// 3) The other pairs: scores first (batched form), which size the backtrace
+// state exactly, then launches grouped by score.
+int CigarCall::score_first_pass() {
     for (int32_t i = 0; i < b->n; ++i)
         if (!done[(size_t)i]) order.push_back(i);
-    if (!order.empty()) {
-        int r = batched_core(ctx, b, order, scores, t, true);
-        if (r != TSU_OK) return r;
-        for (int32_t i : order)
-            if (scores[i] < 0) return fail(ctx, scores[i], "pair " + std::to_string(i) + ": " + tsu_strerror(scores[i]));
-        tsu_host::order_desc(order, [&](int32_t i) -> int64_t { return scores[i]; });
-        // their runs go after what the record launches above used
-        int64_t more = 0;
-        for (int32_t i : order) more += 2ll * (scores[i] / ctx->g / unit) + 2;
-        r = grow_runs(more);
-        if (r != TSU_OK) return r;
-        // group pairs into launches of similar score so buffers fit the widest
-        size_t pos = 0;
-        while (pos < order.size()) {
-            const int32_t s0 = scores[order[pos]] / ctx->g;  // widest in this group (sorted desc)
-            int32_t max_len = 0;
-            size_t end = pos;
-            // all pairs with score within this group share W and buffers
-            const int32_t W = std::max<int32_t>(capacity_for_score(ctx, s0), 16);
-            const int32_t steps = s0 + 2;
-            KArgs a = base_args(ctx);
-            // the group ends where the kernel family of a pair's own capacity
-            // changes: a few pairs that need the LDS kernel do not take the
-            // register kernels' pairs with them
-            int family = -1;
-            for (end = pos; end < order.size(); ++end) {
-                const tsu_pair& pr = b->pairs[(size_t)order[end]];
-                const int32_t len = std::max(pr.p_len, pr.t_len);
-                const int32_t si = scores[order[end]] / ctx->g;
-                const int f = bt_family(ctx, std::max<int32_t>(capacity_for_score(ctx, si), 16), len, si);
-                if (family < 0) family = f;
-                else if (f != family) break;
-                max_len = std::max(max_len, len);
-                if (end - pos >= 65536) break;
-            }
-            a.W = W;
-            a.seq_words = seq_words_for(max_len);
-            const int64_t group_n = (int64_t)(end - pos);
-            Shape sh{default_waves(W), max_len <= 32000, true, true, true};
-            sh.fixed_242 = tsu_host::fixed_242(ctx);
-            int64_t recs = 0;
-            int32_t step_cap = steps;
-            const int c1 = tsu_host::reg_chunks(ctx, W);
-            if (c1 > 0 && max_len <= 16000 && s0 <= 16000) {
-                // single-wave kernel with the int16 LDS ring: records at a fixed
-                // place per step and chunk, (s * chunks + c) * 64 + lane
-                sh.nw = 1;
-                sh.reg_c = c1;
-                sh.lring = true;
-                recs = (int64_t)steps * c1 * 64;
-            } else if (int nwl = tsu_host::ml_waves(ctx, W, max_len)) {
-                // 16-chunk multi-wave kernel: records at a fixed place per step and
-                // chunk, (s * chunks + j) * 64 + lane
-                sh.nw = nwl;
-                sh.reg_c = 16;
-                sh.dbl = max_len <= tsu::kMlDblMaxLen && s0 <= tsu::kMlDblMaxScore;
-                recs = (int64_t)steps * nwl * 16 * 64;
-            } else {
-                for (int64_t s = 0; s <= s0; ++s)
-                    recs += std::min<int64_t>(W, 2 * (std::max<int64_t>(0, s - ctx->os) / ctx->es + 2) + 1);
-                recs += 64;
-                if (tsu::lds_bytes(sh, a) > ctx->max_lds) { sh.lds_ring = false; sh.nw = 16; }
-            }
-            int64_t per_wg = recs * 4 + (int64_t)steps * 12 + (sh.lds_ring || sh.reg_c ? 0 : tsu::ring_bytes(sh, W, ctx->rm, ctx->ri));
-            // checkpoints instead of records when records do not fit one pair per CU
-            const bool ck = ctx->cigar_mode == TSU_CIGAR_CHECKPOINTS ||
-                            (ctx->cigar_mode == TSU_CIGAR_AUTO && s0 > 0 &&
-                             per_wg * std::min<int64_t>(group_n, ctx->cu_count) > budget);
-            if (ck) {
-                sh = Shape{std::max(4, default_waves(W)), max_len <= 32000, true, true, true};
-                sh.fixed_242 = tsu_host::fixed_242(ctx);
-                sh.ck = true;
-                if (tsu::lds_bytes(sh, a) > ctx->max_lds) { sh.lds_ring = false; sh.nw = 16; }
-                const int32_t nslot = ctx->rm + 2 * ctx->ri;
-                const int32_t osz = sh.off16 ? 2 : 4;
-                // window K balancing snapshots (s0 / K of nslot * W cells) against
-                // records (K * W words)
-                int32_t K = ctx->cigar_ck_every;
-                if (K <= 0) K = (int32_t)std::max(8.0, std::round(std::sqrt((double)s0 * nslot * osz / 4.0)));
-                K = std::max<int32_t>(1, std::min<int32_t>(K, std::max<int32_t>(s0, 1)));
-                a.ck_every = K;
-                a.ck_cap = (std::max<int32_t>(s0, 1) - 1) / K + 1;
-                a.ck_head = (nslot * 8 + 15) / 16 * 16;
-                a.ck_bytes = a.ck_head + ((int64_t)nslot * W * osz + 15) / 16 * 16;
-                recs = (int64_t)K * W + 64;
-                step_cap = K + 1;
-                per_wg = recs * 4 + (int64_t)step_cap * 12 + (int64_t)a.ck_cap * a.ck_bytes +
-                         (sh.lds_ring ? 0 : tsu::ring_bytes(sh, W, ctx->rm, ctx->ri));
-            }
-            std::vector<int32_t> ids(order.begin() + (ptrdiff_t)pos, order.begin() + (ptrdiff_t)end);
-            r = launch_bt(sh, a, W, ids, max_len, recs, step_cap, per_wg);
-            if (r != TSU_OK) return r;
-            pos = end;
+    if (order.empty()) return TSU_OK;
+    int r = batched_core(ctx, b, order, scores, t, true);
+    if (r != TSU_OK) return r;
+    for (int32_t i : order)
+        if (scores[i] < 0) return fail(ctx, scores[i], "pair " + std::to_string(i) + ": " + tsu_strerror(scores[i]));
+    tsu_host::order_desc(order, [&](int32_t i) -> int64_t { return scores[i]; });
+    // their runs go after what the record launches used
+    int64_t more = 0;
+    for (int32_t i : order) more += 2ll * (scores[i] / ctx->g / unit) + 2;
+    r = grow_runs(more);
+    if (r != TSU_OK) return r;
+    // group pairs into launches of similar score so buffers fit the widest
+    size_t pos = 0;
+    while (pos < order.size()) {
+        const int32_t s0 = scores[order[pos]] / ctx->g;  // widest in this group (sorted desc)
+        int32_t max_len = 0;
+        size_t end = pos;
+        // all pairs with score within this group share W and buffers
+        const int32_t W = std::max<int32_t>(capacity_for_score(ctx, s0), 16);
+        const int32_t steps = s0 + 2;
+        KArgs a = base_args(ctx);
+        // the group ends where the kernel family of a pair's own capacity
+        // changes: a few pairs that need the LDS kernel do not take the
+        // register kernels' pairs with them
+        int family = -1;
+        for (end = pos; end < order.size(); ++end) {
+            const tsu_pair& pr = b->pairs[(size_t)order[end]];
+            const int32_t len = std::max(pr.p_len, pr.t_len);
+            const int32_t si = scores[order[end]] / ctx->g;
+            const int f = bt_family(ctx, std::max<int32_t>(capacity_for_score(ctx, si), 16), len, si);
+            if (family < 0) family = f;
+            else if (f != family) break;
+            max_len = std::max(max_len, len);
+            if (end - pos >= 65536) break;
         }
+        a.W = W;
+        a.seq_words = seq_words_for(max_len);
+        const int64_t group_n = (int64_t)(end - pos);
+        Shape sh{default_waves(W), max_len <= 32000, true, true, true};
+        sh.fixed_242 = tsu_host::fixed_242(ctx);
+        int64_t recs = 0;
+        int32_t step_cap = steps;
+        const int c1 = bt_reg_chunks(ctx, W);
+        if (c1 > 0 && max_len <= 16000 && s0 <= 16000) {
+            // single-wave kernel with the int16 LDS ring: records at a fixed
+            // place per step and chunk, (s * chunks + c) * 64 + lane
+            sh.nw = 1;
+            sh.reg_c = c1;
+            sh.lring = true;
+            recs = (int64_t)steps * c1 * 64;
+        } else if (int nwl = tsu_host::ml_waves(ctx, W, max_len)) {
+            // 16-chunk multi-wave kernel: records at a fixed place per step and
+            // chunk, (s * chunks + j) * 64 + lane
+            sh.nw = nwl;
+            sh.reg_c = 16;
+            sh.dbl = max_len <= tsu::kMlDblMaxLen && s0 <= tsu::kMlDblMaxScore;
+            recs = (int64_t)steps * nwl * 16 * 64;
+        } else {
+            for (int64_t s = 0; s <= s0; ++s)
+                recs += std::min<int64_t>(W, 2 * (std::max<int64_t>(0, s - ctx->os) / ctx->es + 2) + 1);
+            recs += 64;
+            if (tsu::lds_bytes(sh, a) > ctx->max_lds) { sh.lds_ring = false; sh.nw = 16; }
+        }
+        int64_t per_wg = recs * 4 + (int64_t)steps * 12 + (sh.lds_ring || sh.reg_c ? 0 : tsu::ring_bytes(sh, W, ctx->rm, ctx->ri));
+        // checkpoints instead of records when records do not fit one pair per CU
+        const bool ck = ctx->cigar_mode == TSU_CIGAR_CHECKPOINTS ||
+                        (ctx->cigar_mode == TSU_CIGAR_AUTO && s0 > 0 &&
+                         per_wg * std::min<int64_t>(group_n, ctx->cu_count) > budget);
+        if (ck) {
+            sh = Shape{std::max(4, default_waves(W)), max_len <= 32000, true, true, true};
+            sh.fixed_242 = tsu_host::fixed_242(ctx);
+            sh.ck = true;
+            if (tsu::lds_bytes(sh, a) > ctx->max_lds) { sh.lds_ring = false; sh.nw = 16; }
+            const int32_t nslot = ctx->rm + 2 * ctx->ri;
+            const int32_t osz = sh.off16 ? 2 : 4;
+            // window K balancing snapshots (s0 / K of nslot * W cells) against
+            // records (K * W words)
+            int32_t K = ctx->cigar_ck_every;
+            if (K <= 0) K = (int32_t)std::max(8.0, std::round(std::sqrt((double)s0 * nslot * osz / 4.0)));
+            K = std::max<int32_t>(1, std::min<int32_t>(K, std::max<int32_t>(s0, 1)));
+            a.ck_every = K;
+            a.ck_cap = (std::max<int32_t>(s0, 1) - 1) / K + 1;
+            a.ck_head = (nslot * 8 + 15) / 16 * 16;
+            a.ck_bytes = a.ck_head + ((int64_t)nslot * W * osz + 15) / 16 * 16;
+            recs = (int64_t)K * W + 64;
+            step_cap = K + 1;
+            per_wg = recs * 4 + (int64_t)step_cap * 12 + (int64_t)a.ck_cap * a.ck_bytes +
+                     (sh.lds_ring ? 0 : tsu::ring_bytes(sh, W, ctx->rm, ctx->ri));
+        }
+        std::vector<int32_t> ids(order.begin() + (ptrdiff_t)pos, order.begin() + (ptrdiff_t)end);
+        r = launch_bt(sh, a, W, ids, max_len, recs, step_cap, per_wg);
+        if (r != TSU_OK) return r;
+        pos = end;
     }
+    return TSU_OK;
+}
+
+// The runs to the host; the score-first path's pairs must score the same in
+// their backtrace launch as in the score pass.
+int CigarCall::collect(tsu_cigars** out) {
     HIPCHK(hipMemcpy(sc2.data(), b->d_scores, (size_t)b->n * sizeof(int32_t), hipMemcpyDeviceToHost), "hipMemcpy D2H");
     unsigned long long used = 0;
     HIPCHK(hipMemcpy(&used, d_used.p, sizeof(used), hipMemcpyDeviceToHost), "hipMemcpy D2H");
@@ -1712,7 +1788,6 @@ extern "C" int tsu_align_cigar(tsu_ctx* ctx, tsu_batch* b, int32_t* scores, tsu_
     }
     const double w1 = now_ms();
     if (t) { t->kernel_ms += kms_total; t->launches += launches; t->wall_ms = w1 - w0; t->pairs = b->n; t->cells = b->cells; }
-    // pairs of the score-first path: the backtrace launch's score equals the score pass's
     for (int32_t i : order) {
         if (sc2[(size_t)i] != scores[i]) {
             const int st = sc2[(size_t)i] < 0 ? sc2[(size_t)i] : TSU_EINTERNAL;
@@ -1722,6 +1797,16 @@ extern "C" int tsu_align_cigar(tsu_ctx* ctx, tsu_batch* b, int32_t* scores, tsu_
     }
     *out = cg.release();
     return TSU_OK;
+}
+
+}  // namespace
+
+extern "C" int tsu_align_cigar(tsu_ctx* ctx, tsu_batch* b, int32_t* scores, tsu_cigars** out, tsu_timing* t) {
+    if (!ctx || !b || !scores || !out || b->ctx != ctx) return fail(ctx, TSU_EINVAL, "tsu_align_cigar: bad arguments");
+    *out = nullptr;
+    if (ctx->two_piece) return fail(ctx, TSU_EINVAL, "tsu_align_cigar: not available for 2-piece penalties");
+    CigarCall call(ctx, b, scores, t);
+    return call.run(out);
 }
 
 extern "C" int tsu_set_cigar_memory(tsu_ctx* ctx, int32_t mode, int32_t checkpoint_every, int64_t budget_bytes) {
@@ -1754,6 +1839,7 @@ extern "C" int tsu_cigars_stats(const tsu_cigars* c, int64_t* state_bytes, int32
 
 uint32_t tsu_host::kernel_bit(const Shape& sh) {
     if (sh.ck) return TSU_KERNEL_LDS_CK;
+    if (sh.reg_c > 0 && sh.nw == 1) return TSU_KERNEL_REG1;
     if (sh.reg_c == 16) return sh.nw <= 4 ? TSU_KERNEL_ML4 : sh.nw == 8 ? TSU_KERNEL_ML8 : TSU_KERNEL_ML16;
     if (sh.reg_c > 0) return TSU_KERNEL_REG1;
     return TSU_KERNEL_LDS;

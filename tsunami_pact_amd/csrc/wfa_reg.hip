@@ -1,6 +1,7 @@
 // Single-wave register kernel instances (scaled penalties 2/3/1, i.e. 4/6/2),
 // windows of 64, 128, 256 and 512 diagonals: the whole M ring in registers,
-// or the older M scores in an int16 LDS ring (sequences up to 16000 bases).
+// or the older M scores in an int16 LDS ring (sequences up to 16000 bases);
+// with records (CIGARs) the int16-ring kernel also takes 1024 diagonals.
 #include "wfa_reg.h"
 #include "wfa_reg1l.h"
 
@@ -15,6 +16,7 @@ KernelFn pick_reg(const Shape& sh) {
         case 2: return wfa_reg1l_kernel<2, true, true>;
         case 4: return wfa_reg1l_kernel<4, true, true>;
         case 8: return wfa_reg1l_kernel<8, true, true>;
+        case 16: return wfa_reg1l_kernel<16, true, true>;  // CIGARs of windows of 513..1024 diagonals
         default: return nullptr;
         }
     }

@@ -39,7 +39,7 @@ constexpr int kMl1MaxScore = 16000;
 
 template <int C, bool QUEUE, bool BT = false>
 __global__ __launch_bounds__(64) void wfa_reg1l_kernel(KArgs a) {
-    static_assert(C >= 1 && C <= 8, "window of 64..512 diagonals");
+    static_assert(C >= 1 && C <= 16, "window of 64..1024 diagonals");
     constexpr int32_t NUL = kRegNull + 0x8000;  // low 16 bits: -32768
     constexpr int W = 64 * C;
     constexpr int MRS = W + 2;  // int16 entries per ring slot, NULL guards at both ends

@@ -69,8 +69,7 @@ struct MlLayout {
     static constexpr int W = 64 * NC;                       // diagonals in the window
     static constexpr int EI = 0;                            // [2][NC+2] I lane 63 per chunk, by parity of s
     static constexpr int ED = EI + 2 * (NC + 2);            // [2][NC+2] D lane 0 per chunk
-    static constexpr int FW = ED + 2 * (NC + 2);            // [5][NW] step masks, by s mod 5
-    static constexpr int MISC = FW + 5 * NW;                // [4] queue index
+    static constexpr int MISC = ED + 2 * (NC + 2);          // [4] queue index
     static constexpr int PG = MISC + 4;                     // [NW] neighbour sync: last score each wave finished
     static constexpr int EV = PG + NW;                      // [4] neighbour sync: first event, score * 4 + kind
     static constexpr int SK = EV + 4;                       // [64+NC] write sink of the lanes that publish nothing
@@ -155,7 +154,7 @@ __global__ __launch_bounds__(NW * 64) void wfa_regml_kernel(KArgs a) {
         }
         for (int i = tid; i < L::WORDS; i += NT) {
             int32_t v = NUL;                                          // I / D edges
-            if (i >= L::FW) v = 0;                                    // masks
+            if (i >= L::MISC) v = 0;
             if (i >= L::MR) v = (int32_t)0x80008000u;                 // two int16 NULLs
             if (i >= L::PG && i < L::EV) v = 0;                       // progress: score 0 done
             if (i == L::EV) v = 0x7fffffff;                           // no event yet
