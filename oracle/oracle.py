@@ -68,6 +68,9 @@ def lib() -> ctypes.CDLL:
         l.orc_wfa2p.restype = ctypes.c_int
         l.orc_wfa2p.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_char_p, ctypes.c_int] + [ctypes.c_int] * 5 + \
             [P(ctypes.c_int64)]
+        l.orc_wfa2p_prio.restype = ctypes.c_int
+        l.orc_wfa2p_prio.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_char_p, ctypes.c_int] + \
+            [ctypes.c_int] * 5 + [P(ctypes.c_int), ctypes.c_char_p, ctypes.c_int, P(ctypes.c_int)]
         l.orc_wfa2p_batch.restype = ctypes.c_int
         l.orc_wfa2p_batch.argtypes = [ctypes.c_char_p, P(ctypes.c_int64), P(ctypes.c_int32), P(ctypes.c_int64),
                                       P(ctypes.c_int32), ctypes.c_int] + [ctypes.c_int] * 6 + \
@@ -138,6 +141,59 @@ def wfa2p(p: str, t: str, x: int, o1: int, e1: int, o2: int, e2: int) -> Tuple[i
     if r < 0:
         raise RuntimeError(f"orc_wfa2p failed: {r}")
     return r, c.value
+
+
+# backtrace type values of the 2-piece walk (common.h:67-77), in the order
+# orc_wfa2p_prio takes them
+PRIORITY_2P = (9, 8, 7, 6, 5, 4, 3, 2, 1)
+PRIORITY_2P_KINDS = ("M", "D2_ext", "D2_open", "D1_ext", "D1_open", "I2_ext", "I2_open", "I1_ext", "I1_open")
+
+
+def wfa2p_cigar(p: str, t: str, x: int, o1: int, e1: int, o2: int, e2: int,
+                prio: Optional[Sequence[int]] = None) -> Tuple[int, str]:
+    """Score and CIGAR ops (forward 'M'/'X'/'I'/'D' string) of the 2-piece WFA
+    backtrace; prio overrides the nine type values (order of PRIORITY_2P_KINDS)."""
+    cap = 2 * (len(p) + len(t)) + 16
+    buf = ctypes.create_string_buffer(cap)
+    n = ctypes.c_int(0)
+    pr = None
+    if prio is not None:
+        pr = (ctypes.c_int * 9)(*prio)
+    r = lib().orc_wfa2p_prio(p.encode(), len(p), t.encode(), len(t), x, o1, e1, o2, e2, pr, buf, cap, ctypes.byref(n))
+    if r < 0:
+        raise RuntimeError(f"orc_wfa2p_prio failed: {r}")
+    return r, buf.raw[:n.value].decode()
+
+
+def rescore2p(ops: str, p: str, t: str, x: int, o1: int, e1: int, o2: int, e2: int) -> int:
+    """2-piece cost of an alignment given as forward ops (a gap run of length L
+    costs min(o1 + L e1, o2 + L e2)); raises if the ops do not align p with t."""
+    i = j = 0
+    cost = 0
+    for c in ops:
+        if c == "M" or c == "X":
+            if i >= len(p) or j >= len(t) or (p[i] == t[j]) != (c == "M"):
+                raise ValueError(f"{c} at ({i}, {j}) does not fit the sequences")
+            cost += x if c == "X" else 0
+            i += 1; j += 1
+        elif c == "I":
+            j += 1
+        elif c == "D":
+            i += 1
+        else:
+            raise ValueError(f"unknown op {c!r}")
+    if i != len(p) or j != len(t):
+        raise ValueError("ops do not cover both sequences")
+    pos = 0
+    while pos < len(ops):
+        end = pos
+        while end < len(ops) and ops[end] == ops[pos]:
+            end += 1
+        if ops[pos] in "ID":
+            n = end - pos
+            cost += min(o1 + e1 * n, o2 + e2 * n)
+        pos = end
+    return cost
 
 
 def wfa2p_batch(pairs: Sequence[Tuple[str, str]], x: int, o1: int, e1: int, o2: int, e2: int,

@@ -26,6 +26,20 @@
  *
  * orc_gotoh2p() is the textbook 2-piece Gotoh DP (five matrices, O(p*t)
  * time, O(t) memory), independent of the WFA code; it pins the scores.
+ *
+ * orc_wfa2p_cigar() keeps every wavefront and walks back from the end cell
+ * as WFA2-lib's affine-2p backtrace does (wavefront_backtrace_affine): at an
+ * M cell the candidates are the mismatch M[s-x][k]+1, the opens
+ * M[s-o-e][k-1]+1 / M[s-o-e][k+1] and the extensions I[s-e][k-1]+1 /
+ * D[s-e][k+1] of both pieces, compared as offset * 16 + type (largest offset
+ * first, ties to the larger type) with the types of common.h:67-77: M 9,
+ * D2 ext 8, D2 open 7, D1 ext 6, D1 open 5, I2 ext 4, I2 open 3, I1 ext 2,
+ * I1 open 1; in an I or D chain only that component's open and extend.
+ * orc_wfa2p_prio() takes the nine type values from the caller (mutation
+ * tests).  Pinned by: the CIGAR's 2-piece cost equals orc_gotoh2p's score;
+ * with the second piece never cheaper, and with two equal pieces, it equals
+ * the pinned 1-piece CIGAR of orc_wfa; hand-derived tie cases in
+ * tests/tie_cases.py, each decided by one comparison of the order above.
  */
 #include <stdint.h>
 #include <stdlib.h>
@@ -125,17 +139,21 @@ enum { C_M = 0, C_I1, C_D1, C_I2, C_D2, C_N };
 
 /*
  * Score of one pair under gap-affine 2-piece penalties (>= 0), or a negative
- * error.  cells (optional) receives the sum of computed range widths.
+ * error.  cells (optional) receives the sum of computed range widths.  With
+ * keep, every score has its own slot and *kept receives the wavefronts
+ * (ring = s_bound + 1 slots of C_N components, slot s), for the caller to
+ * free with p2_release.
  */
-int orc_wfa2p(const char* p, int plen, const char* t, int tlen, int x, int o1, int e1, int o2, int e2,
-              int64_t* cells)
+static int p2_core(const char* p, int plen, const char* t, int tlen, int x, int o1, int e1, int o2, int e2,
+                   int64_t* cells, int keep, p2_wf** kept, int* kept_ring)
 {
     if (plen < 0 || tlen < 0 || x <= 0 || o1 < 0 || e1 <= 0 || o2 < 0 || e2 <= 0) return P2_EINVAL;
-    const int ring = P2_MAX(x, P2_MAX(o1 + e1, o2 + e2)) + 1;
     const int align_k = tlen - plen, align_off = tlen;
     const int64_t gap_tail = plen != tlen ? P2_MIN((int64_t)o1 + (int64_t)e1 * abs(plen - tlen),
                                                    (int64_t)o2 + (int64_t)e2 * abs(plen - tlen)) : 0;
     const int64_t s_bound = (int64_t)x * P2_MIN(plen, tlen) + gap_tail;
+    if (keep && s_bound > (1 << 24)) return P2_EINVAL;
+    const int ring = keep ? (int)s_bound + 1 : P2_MAX(x, P2_MAX(o1 + e1, o2 + e2)) + 1;
     /* wavefronts of the last `ring` scores, slot s % ring */
     p2_wf* W = (p2_wf*)calloc((size_t)ring * C_N, sizeof(p2_wf));
     if (!W) return P2_ENOMEM;
@@ -203,9 +221,148 @@ int orc_wfa2p(const char* p, int plen, const char* t, int tlen, int x, int o1, i
         for (int c = 0; c < C_N; ++c) if (has[c]) p2_trim(out[c], plen, tlen);
     }
 out:
+    if (keep && r >= 0) {
+        *kept = W;
+        *kept_ring = ring;
+    } else {
+        for (int i = 0; i < ring * C_N; ++i) free(W[i].off);
+        free(W);
+    }
+#undef WF
+    return r;
+}
+
+static void p2_release(p2_wf* W, int ring)
+{
     for (int i = 0; i < ring * C_N; ++i) free(W[i].off);
     free(W);
+}
+
+int orc_wfa2p(const char* p, int plen, const char* t, int tlen, int x, int o1, int e1, int o2, int e2,
+              int64_t* cells)
+{
+    return p2_core(p, plen, t, tlen, x, o1, e1, o2, e2, cells, 0, NULL, NULL);
+}
+
+/* backtrace type values of common.h:67-77, in the order of prio[] below */
+static const int P2_PRIO[9] = { 9, 8, 7, 6, 5, 4, 3, 2, 1 };
+enum { B_M = 0, B_D2E, B_D2O, B_D1E, B_D1O, B_I2E, B_I2O, B_I1E, B_I1O };
+
+/*
+ * Walk back from the end cell (WFA2-lib wavefront_backtrace_affine with two
+ * gap pieces, end-to-end).  prio[] gives the type value of M, D2 ext, D2
+ * open, D1 ext, D1 open, I2 ext, I2 open, I1 ext, I1 open.  Returns the
+ * number of ops written (forward order) or a negative error.
+ */
+static int p2_backtrace(const p2_wf* W, int ring, int plen, int tlen, int x, int o1, int e1, int o2, int e2,
+                        int score_end, const int* prio, char* ops, int ops_cap)
+{
+#define WF(sc, c) (&W[((sc) % ring) * C_N + (c)])
+#define PIGGY(off, ty) ((((int64_t)(off)) * 16) | (int64_t)(ty))
+#define EMIT(ch) do { if (n >= ops_cap) return P2_EINVAL; ops[n++] = (ch); } while (0)
+    const int64_t PNULL = ((int64_t)P2_NULL) * 16;
+    int n = 0;
+    int k = tlen - plen;
+    int32_t offset = tlen;
+    int v = plen, h = tlen;
+    int score = score_end;
+    int comp = C_M;
+    while (v > 0 && h > 0 && score > 0) {
+        const int sx = score - x, so1 = score - o1 - e1, se1 = score - e1, so2 = score - o2 - e2, se2 = score - e2;
+        int64_t best = PNULL;
+        int bk = -1;
+#define CAND(cond, sc, c, kk, add, b)                                                  \
+        do {                                                                           \
+            if (cond) {                                                                \
+                const int64_t cv = PIGGY(p2_get(WF(sc, c), kk) + (add), prio[b]);      \
+                if (cv > best) { best = cv; bk = (b); }                                \
+            }                                                                          \
+        } while (0)
+        if (comp == C_M) CAND(sx >= 0, sx, C_M, k, 1, B_M);
+        if (comp == C_M || comp == C_I1) {
+            CAND(so1 >= 0, so1, C_M, k - 1, 1, B_I1O);
+            CAND(se1 >= 0, se1, C_I1, k - 1, 1, B_I1E);
+        }
+        if (comp == C_M || comp == C_D1) {
+            CAND(so1 >= 0, so1, C_M, k + 1, 0, B_D1O);
+            CAND(se1 >= 0, se1, C_D1, k + 1, 0, B_D1E);
+        }
+        if (comp == C_M || comp == C_I2) {
+            CAND(so2 >= 0, so2, C_M, k - 1, 1, B_I2O);
+            CAND(se2 >= 0, se2, C_I2, k - 1, 1, B_I2E);
+        }
+        if (comp == C_M || comp == C_D2) {
+            CAND(so2 >= 0, so2, C_M, k + 1, 0, B_D2O);
+            CAND(se2 >= 0, se2, C_D2, k + 1, 0, B_D2E);
+        }
+#undef CAND
+        if (best < 0 || bk < 0) return P2_EINTERNAL;
+        const int32_t max_off = (int32_t)(best >> 4);
+        if (comp == C_M) {
+            const int nm = offset - max_off;
+            if (nm < 0) return P2_EINTERNAL;
+            for (int i = 0; i < nm; ++i) EMIT('M');
+            offset = max_off;
+            v = offset - k; h = offset;
+            if (v <= 0 || h <= 0) break;
+        }
+        switch (bk) {
+        case B_M:   score = sx;  comp = C_M;  EMIT('X'); --offset; break;
+        case B_I1O: score = so1; comp = C_M;  EMIT('I'); --k; --offset; break;
+        case B_I1E: score = se1; comp = C_I1; EMIT('I'); --k; --offset; break;
+        case B_I2O: score = so2; comp = C_M;  EMIT('I'); --k; --offset; break;
+        case B_I2E: score = se2; comp = C_I2; EMIT('I'); --k; --offset; break;
+        case B_D1O: score = so1; comp = C_M;  EMIT('D'); ++k; break;
+        case B_D1E: score = se1; comp = C_D1; EMIT('D'); ++k; break;
+        case B_D2O: score = so2; comp = C_M;  EMIT('D'); ++k; break;
+        case B_D2E: score = se2; comp = C_D2; EMIT('D'); ++k; break;
+        default: return P2_EINTERNAL;
+        }
+        v = offset - k; h = offset;
+    }
+    if (comp == C_M) {
+        if (v > 0 && h > 0) {
+            const int nm = P2_MIN(v, h);
+            for (int i = 0; i < nm; ++i) EMIT('M');
+            v -= nm; h -= nm;
+        }
+        while (v > 0) { EMIT('D'); --v; }
+        while (h > 0) { EMIT('I'); --h; }
+    } else if (v != 0 || h != 0) {
+        return P2_EINTERNAL;
+    }
+    for (int i = 0, j = n - 1; i < j; ++i, --j) { const char c = ops[i]; ops[i] = ops[j]; ops[j] = c; }
+    return n;
 #undef WF
+#undef PIGGY
+#undef EMIT
+}
+
+/*
+ * Score and CIGAR ops ('M', 'X', 'I', 'D', forward order) of one pair under
+ * 2-piece penalties; prio NULL for common.h's backtrace types, else nine
+ * distinct values in 1..15 (see p2_backtrace).  Returns the score or a
+ * negative error; *ops_len receives the number of ops.
+ */
+int orc_wfa2p_prio(const char* p, int plen, const char* t, int tlen, int x, int o1, int e1, int o2, int e2,
+                   const int* prio, char* ops, int ops_cap, int* ops_len)
+{
+    if (!ops || !ops_len) return P2_EINVAL;
+    if (prio) {
+        for (int i = 0; i < 9; ++i) {
+            if (prio[i] < 1 || prio[i] > 15) return P2_EINVAL;
+            for (int j = 0; j < i; ++j)
+                if (prio[i] == prio[j]) return P2_EINVAL;
+        }
+    }
+    p2_wf* W = NULL;
+    int ring = 0;
+    const int r = p2_core(p, plen, t, tlen, x, o1, e1, o2, e2, NULL, 1, &W, &ring);
+    if (r < 0) return r;
+    const int n = p2_backtrace(W, ring, plen, tlen, x, o1, e1, o2, e2, r, prio ? prio : P2_PRIO, ops, ops_cap);
+    p2_release(W, ring);
+    if (n < 0) return n;
+    *ops_len = n;
     return r;
 }
 

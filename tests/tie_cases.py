@@ -196,3 +196,91 @@ HOMOPOLYMER_462 = [
 ]
 
 ALL_TIES = TIES_462 + TIES_531
+
+
+# Gap-affine 2-piece (a gap of length L costs gap(L) = min(o1 + e1 L,
+# o2 + e2 L)).  The walk is the same with the candidates of both pieces:
+# in M the mismatch, I1/I2 open M[s-o-e][k-1] + 1 and extend I[s-e][k-1] + 1,
+# D1/D2 open M[s-o-e][k+1] and extend D[s-e][k+1] (o, e of the piece), in
+# an I1 / D1 / I2 / D2 chain only that component's open and extend; equal
+# offsets go to the larger type of common.h:67-77: M 9 > D2_ext 8 >
+# D2_open 7 > D1_ext 6 > D1_open 5 > I2_ext 4 > I2_open 3 > I1_ext 2 >
+# I1_open 1.  One case per adjacent pair of that order, so every other
+# order inverts the comparison of at least one case.  Scores are the
+# unscaled penalties of each case.
+#
+# (pattern, text, (x, o1, e1, o2, e2), expected CIGAR, tied candidates, CIGAR of the other choice)
+TIES_2P = [
+    # gap(1..4) = 4, 7, 10, 11 (L=3: piece 1 and 2 both 10).  p=CCGG, t=A,
+    # score 14 = D(3) 10 + X 4.  End cell k=-3, offset 1.
+    #   mismatch M[10][-3]+1 = 0+1 = 1  (delete CCG -> (3,0) at 10; G != A)
+    #   D2_ext   D2[13][-2]  = 1        (mismatch C/A -> (1,1) at 4, D2 open 12 -> (2,1), 13 -> (3,1))
+    #   D1_ext   D1[11][-2]  = 1        (from (1,1) at 4: D1 open 8 -> (2,1), 11 -> (3,1))
+    # Tie at 1; M (9) > D2_ext (8): X, then (3,0) at 10: three leading D.
+    # D2_ext first would follow the D2 chain back to the mismatch at (1,1).
+    ("CCGG", "A", (4, 1, 3, 7, 1), "3D1X", ("M", "D2_ext"), "1X3D"),
+    # gap(1..4) = 5, 8, 11, 12 (L=3 ties).  p=AAAAA, t=CA, score 17 = I 5 +
+    # D(4) 12 = X 6 + D(3) 11.  End cell k=-3, offset 2: D2_ext D2[16][-2] = 2
+    # (the D2 chains after 1I1M, (1,2) at 5, and after 1X1M, (2,2) at 6, both
+    # reach (4,2) at 16) beats D1_ext D1[14][-2] = 2 (8 > 6), no match.  In
+    # D2[16][-2] only D2_ext D2[15][-1] = 2 exists.  In D2[15][-1]:
+    #   D2_ext   D2[14][0] = 2           (1I1M -> (1,2) at 5, D2 open 14 -> (2,2))
+    #   D2_open  M[6][0]   = 2           (1X1M -> (2,2) at 6)
+    # Tie; D2_ext (8) > D2_open (7): D2[14][0], whose only source is D2_open
+    # M[5][1] = 2 ((0,1) at 5 extended A/A); in M[5][1] I1_open M[0][0]+1 = 1:
+    # one M, then v = 0: one leading I.
+    ("AAAAA", "CA", (6, 2, 3, 8, 1), "1I1M4D", ("D2_ext", "D2_open"), "1X1M3D"),
+    # gap(1..7) = 9, 13, 17, 21, 25, 29, 33 (L=1: both pieces 9).  p=CAACAAC,
+    # t=A: t's A against p[1], p[2], p[4] or p[5], each 34 (9+25, 13+21,
+    # 21+13, 25+9).  End cell k=-6, offset 1:
+    #   D2_open  M[25][-5] = 1           (delete CAACA -> (5,0) at 25, extend A/A -> (6,1))
+    #   D1_ext   D1[30][-5] = 1          (the piece-1 chains after p[1], p[2] or p[4] reach (6,1) at 30)
+    #   D1_open  M[25][-5] = 1
+    # Tie; D2_open (7) > D1_ext (6) > D1_open (5): D, then M[25][-5] at
+    # offset 1, pre-extension 0: one M, then (5,0): five leading D.
+    # D1_ext first would stay in the piece-1 chain (D1_ext 6 > D1_open 5 at
+    # each of its cells) back to p[1].
+    ("CAACAAC", "A", (8, 5, 4, 4, 5), "5D1M1D", ("D2_open", "D1_ext"), "1D1M5D"),
+    # gap(1..4) = 4, 6, 8, 10.  p=ACCG, t=C, score 10 = D 4 + D(2) 6 either way.
+    # End cell k=-3, offset 1:
+    #   D1_ext   D1[8][-2] = 1           (delete A -> (1,0) at 4, match C -> (2,1), D1 open 8 -> (3,1))
+    #   D1_open  M[6][-2]  = 1           (delete AC -> (2,0) at 6, match C -> (3,1))
+    # Tie; D1_ext (6) > D1_open (5): D1[8][-2], whose only source is D1_open
+    # M[4][-1] = 1; there one M down to (1,0): one leading D.
+    ("ACCG", "C", (4, 2, 2, 6, 1), "1D1M2D", ("D1_ext", "D1_open"), "2D1M1D"),
+    # gap(1..4) = 4, 7, 10, 11 (L=3 ties).  p=AG, t=GCCA, score 14 = I(3) 10 +
+    # D 4 = D 4 + I(3) 10.  End cell k=2, offset 4:
+    #   D1_open  M[10][3]   = 4          (insert GCC -> (0,3) at 10, match A -> (1,4))
+    #   I2_ext   I2[13][1]+1 = 3+1 = 4   (delete A -> (1,0) at 4, match G -> (2,1),
+    #                                     I2 open 12 -> (2,2), 13 -> (2,3))
+    #   I1_ext   I1[11][1]+1 = 3+1 = 4   (from (2,1) at 4: I1 open 8 -> (2,2), 11 -> (2,3))
+    # Tie; D1_open (5) > I2_ext (4): D, then M[10][3] at offset 4, whose
+    # sources I2_ext I2[9][2]+1 and I1_ext I1[7][2]+1 both give 3 (I2 first):
+    # one M, then the insertion chain back to (0,0).
+    ("AG", "GCCA", (4, 1, 3, 7, 1), "3I1M1D", ("D1_open", "I2_ext"), "1D1M3I"),
+    # The insertion mirror of the D2 case above: gap(1..4) = 5, 8, 11, 12.
+    # p=AG, t=GGCCC, score 17 = D 5 + I(4) 12 = X 6 + I(3) 11.  End cell k=3,
+    # offset 5: I2_ext I2[16][2]+1 = 5 beats I1_ext I1[14][2]+1 = 5 (4 > 2);
+    # in I2[16][2] only I2_ext I2[15][1]+1 = 4.  In I2[15][1]:
+    #   I2_ext   I2[14][0]+1 = 2+1 = 3   (1D1M -> (2,1) at 5, I2 open 14 -> (2,2))
+    #   I2_open  M[6][0]+1   = 2+1 = 3   (1X1M -> (2,2) at 6)
+    # Tie; I2_ext (4) > I2_open (3): I2[14][0], from I2_open M[5][-1]+1 = 2;
+    # in M[5][-1] one M down to (1,0): one leading D.
+    ("AG", "GGCCC", (6, 2, 3, 8, 1), "1D1M4I", ("I2_ext", "I2_open"), "1X1M3I"),
+    # gap(1..5) = 7, 11, 15, 19, 23 (L=1: both pieces 7).  p=C, t=ACACA,
+    # score 22 = I 7 + I(3) 15 either way.  End cell k=4, offset 5:
+    #   I2_open  M[15][3]+1  = 4+1 = 5   (insert ACA -> (0,3) at 15, match C -> (1,4))
+    #   I1_ext   I1[18][3]+1 = 4+1 = 5   (insert A -> (0,1) at 7, match C -> (1,2),
+    #                                     I1 open 14 -> (1,3), 18 -> (1,4))
+    #   I1_open  M[15][3]+1  = 5
+    # Tie; I2_open (3) > I1_ext (2): I, then M[15][3] at offset 4,
+    # pre-extension 3: one M, then (0,3): three leading I.
+    ("C", "ACACA", (4, 3, 4, 2, 5), "3I1M1I", ("I2_open", "I1_ext"), "1I1M3I"),
+    # The insertion mirror of the D1 case: gap(1..4) = 4, 6, 8, 10.  p=A,
+    # t=CAAG, score 10.  End cell k=3, offset 4:
+    #   I1_ext   I1[8][2]+1 = 3+1 = 4    (insert C -> (0,1) at 4, match A -> (1,2), I1 open 8 -> (1,3))
+    #   I1_open  M[6][2]+1  = 3+1 = 4    (insert CA -> (0,2) at 6, match A -> (1,3))
+    # Tie; I1_ext (2) > I1_open (1): I1[8][2], from I1_open M[4][1]+1 = 3;
+    # in M[4][1] one M down to (0,1): one leading I.
+    ("A", "CAAG", (4, 2, 2, 6, 1), "1I1M2I", ("I1_ext", "I1_open"), "2I1M1I"),
+]
