@@ -87,9 +87,10 @@ int tsu_create(tsu_ctx** out, int device, int32_t mismatch, int32_t gap_opening,
  * min(gap_opening1 + L*gap_extension1, gap_opening2 + L*gap_extension2).
  * The reference declares this model (common.h:52-58,80-87: gap_open2,
  * gap_ext2 and the I2/D2 wavefronts) but never computes it, so this entry
- * point has no counterpart there.  Such a context runs the plain and batched
- * forms (one wave per pair, all five wavefronts in LDS); tsu_align_cigar and
- * tsu_serve_open return TSU_EINVAL for it. */
+ * point has no counterpart there.  Such a context runs the plain, batched
+ * and CIGAR forms (one wave per pair, all five wavefronts in LDS; CIGARs
+ * follow the nine backtrace types of common.h:67-77); tsu_serve_open returns
+ * TSU_EINVAL for it. */
 int tsu_create_2p(tsu_ctx** out, int device, int32_t mismatch, int32_t gap_opening1,
                   int32_t gap_extension1, int32_t gap_opening2, int32_t gap_extension2);
 void tsu_destroy(tsu_ctx* ctx);
@@ -247,7 +248,8 @@ int tsu_tune_table(const tsu_ctx* ctx, tsu_tune_entry* entries, int32_t cap, int
  * window holds; pairs wider than the widest register window are recorded in
  * one launch over their whole diagonal range when that fits the budget; the
  * others get their scores first (the batched form), which size the state
- * exactly.  Each CIGAR is a run list, run = (length << 2) | op with
+ * exactly.  With 2-piece penalties every pair gets its score first, then
+ * records on the 2-piece kernel.  Each CIGAR is a run list, run = (length << 2) | op with
  * op 0 'M' match, 1 'X' mismatch, 2 'I' insertion (consumes text), 3 'D'
  * deletion (consumes pattern). */
 int tsu_align_cigar(tsu_ctx* ctx, tsu_batch* b, int32_t* scores, tsu_cigars** out, tsu_timing* t);
@@ -299,6 +301,7 @@ int tsu_cigars_stats(const tsu_cigars* c, int64_t* state_bytes, int32_t* record_
 #define TSU_KERNEL_ML16    0x08u  /* 16 chunks per wave, 16 waves per pair */
 #define TSU_KERNEL_LDS     0x10u  /* rings in LDS (or HBM), records for every score */
 #define TSU_KERNEL_LDS_CK  0x20u  /* rings in LDS (or HBM), ring checkpoints */
+#define TSU_KERNEL_2P      0x40u  /* gap-affine 2-piece kernel, records for every score */
 int tsu_cigars_kernels(const tsu_cigars* c, uint32_t* kernels);
 
 /* ---- serving (no reference counterpart) ---------------------------------- */

@@ -49,15 +49,49 @@ def test_two_piece_plain_and_batched(tsu, oracle, pen):
         a.close()
 
 
-def test_two_piece_forms_without_support_refuse(tsu):
-    """The CIGAR and served forms refuse a 2-piece context with TSU_EINVAL."""
+@pytest.mark.parametrize("pen", PEN_2P)
+def test_two_piece_cigars(tsu, oracle, pen):
+    """2-piece CIGARs (records on the 2-piece kernel, walked with the nine
+    backtrace types) equal the 2-piece oracle's, itself pinned by
+    tie_cases.TIES_2P and Gotoh re-scoring."""
+    a = tsu.Aligner(pen[0], pen[1], pen[2], gap_opening2=pen[3], gap_extension2=pen[4])
+    try:
+        pairs = pairs_for(tsu, SEED + 40 + sum(pen))
+        say(f"2-piece {pen} CIGARs, {len(pairs)} pairs")
+        got, cig, t = a.align_cigar(a.batch(pairs))
+        assert t["kernels"] == {"2p"}, t["kernels"]
+        for i, (p, q) in enumerate(pairs):
+            s, ops = oracle.wfa2p_cigar(p, q, *pen)
+            assert got[i] == s, i
+            assert cig[i] == oracle.rle(ops), (i, p, q)
+    finally:
+        a.close()
+
+
+def test_two_piece_tie_cigars(tsu, oracle):
+    """The hand-derived 2-piece tie cases, alone and behind a common prefix
+    (which only prefixes matches: every cell of the longer pair is the
+    shorter pair's, shifted along the prefix)."""
+    from tie_cases import TIES_2P
+    from test_gpu_cigar_ties import merge_prefix, prefixed
+    for pen in sorted({c[2] for c in TIES_2P}):
+        cases = [c for c in TIES_2P if c[2] == pen]
+        a = tsu.Aligner(pen[0], pen[1], pen[2], gap_opening2=pen[3], gap_extension2=pen[4])
+        try:
+            for n in (0, 300):
+                pairs, want = prefixed(cases, n, 5 + n)
+                say(f"2-piece ties {pen}, prefix {n}: {len(pairs)} pairs")
+                got, cig, t = a.align_cigar(a.batch(pairs))
+                assert cig == want, (pen, n)
+        finally:
+            a.close()
+
+
+def test_two_piece_serve_refuses(tsu):
+    """The served form refuses a 2-piece context with TSU_EINVAL."""
     a = tsu.Aligner(4, 6, 2, gap_opening2=24, gap_extension2=1)
     try:
-        b = a.batch([("ACGT", "AGGT")])
-        say("2-piece cigar / serve refusal")
-        with pytest.raises(tsu.TsuError) as ei:
-            a.align_cigar(b)
-        assert ei.value.status == tsu.TSU_EINVAL
+        say("2-piece serve refusal")
         with pytest.raises(tsu.TsuError) as ei:
             tsu.Server(a)
         assert ei.value.status == tsu.TSU_EINVAL

@@ -1310,9 +1310,13 @@ public:
 
     int run(tsu_cigars** out) {
         int r = setup();
-        if (r == TSU_OK && ctx->cigar_mode != TSU_CIGAR_CHECKPOINTS && tsu_host::fixed_242(ctx)) r = record_pass();
-        if (r == TSU_OK) r = wide_pass();
-        if (r == TSU_OK) r = score_first_pass();
+        if (ctx->two_piece) {
+            if (r == TSU_OK) r = two_piece_pass();
+        } else {
+            if (r == TSU_OK && ctx->cigar_mode != TSU_CIGAR_CHECKPOINTS && tsu_host::fixed_242(ctx)) r = record_pass();
+            if (r == TSU_OK) r = wide_pass();
+            if (r == TSU_OK) r = score_first_pass();
+        }
         if (r == TSU_OK) r = collect(out);
         return r;
     }
@@ -1330,6 +1334,7 @@ private:
     int record_pass();
     int wide_pass();
     int score_first_pass();
+    int two_piece_pass();
     int collect(tsu_cigars** out);
 
     tsu_ctx* ctx;
@@ -1770,6 +1775,54 @@ int CigarCall::score_first_pass() {
     return TSU_OK;
 }
 
+// 2-piece penalties: scores first (batched form), then record launches on
+// the 2-piece kernel in groups of falling score, each at the capacity of its
+// highest score, records for the whole range of every score up to it.  (No
+// checkpointed variant: the records are what CIGARs of 2-piece pairs use.)
+int CigarCall::two_piece_pass() {
+    for (int32_t i = 0; i < b->n; ++i) order.push_back(i);
+    if (order.empty()) return TSU_OK;
+    int r = batched_core(ctx, b, order, scores, t, true);
+    if (r != TSU_OK) return r;
+    for (int32_t i : order)
+        if (scores[i] < 0) return fail(ctx, scores[i], "pair " + std::to_string(i) + ": " + tsu_strerror(scores[i]));
+    tsu_host::order_desc(order, [&](int32_t i) -> int64_t { return scores[i]; });
+    // a walk reserves 2 * (s / unit) + 2 runs, unit the cheapest operation
+    const int32_t unit2 = std::min(ctx->xs, std::min(ctx->os + ctx->es, ctx->o2s + ctx->e2s));
+    int64_t more = 0;
+    for (int32_t i : order) more += 2ll * (scores[i] / ctx->g / unit2) + 2;
+    r = grow_runs(more);
+    if (r != TSU_OK) return r;
+    size_t pos = 0;
+    while (pos < order.size()) {
+        const int32_t s0 = scores[order[pos]] / ctx->g;  // highest score of this group (sorted desc)
+        const int32_t W = std::max<int32_t>(capacity_for_score(ctx, s0), 16);
+        const size_t end = std::min(order.size(), pos + 65536);
+        int32_t max_len = 0;
+        for (size_t j = pos; j < end; ++j) {
+            const tsu_pair& pr = b->pairs[(size_t)order[j]];
+            max_len = std::max(max_len, std::max(pr.p_len, pr.t_len));
+        }
+        KArgs a = base_args(ctx);
+        a.W = W;
+        a.seq_words = seq_words_for(max_len);
+        Shape sh{1, max_len <= 32000, true, true, true};  // records, queue-fed
+        sh.two_piece = true;
+        if (tsu::lds_bytes(sh, a) > ctx->max_lds)
+            return fail(ctx, TSU_EKOVERFLOW, "pair " + std::to_string(order[pos]) +
+                                                 ": its 2-piece wavefront range does not fit the CU's LDS");
+        int64_t recs = 64;
+        for (int64_t sc = 0; sc <= s0; ++sc) recs += std::min<int64_t>(W, capacity_for_score(ctx, sc));
+        const int32_t steps = s0 + 2;
+        const int64_t per_wg = recs * 4 + (int64_t)steps * 12;
+        std::vector<int32_t> ids(order.begin() + (ptrdiff_t)pos, order.begin() + (ptrdiff_t)end);
+        r = launch_bt(sh, a, W, ids, max_len, recs, steps, per_wg);
+        if (r != TSU_OK) return r;
+        pos = end;
+    }
+    return TSU_OK;
+}
+
 // The runs to the host; the score-first path's pairs must score the same in
 // their backtrace launch as in the score pass.
 int CigarCall::collect(tsu_cigars** out) {
@@ -1804,7 +1857,6 @@ int CigarCall::collect(tsu_cigars** out) {
 extern "C" int tsu_align_cigar(tsu_ctx* ctx, tsu_batch* b, int32_t* scores, tsu_cigars** out, tsu_timing* t) {
     if (!ctx || !b || !scores || !out || b->ctx != ctx) return fail(ctx, TSU_EINVAL, "tsu_align_cigar: bad arguments");
     *out = nullptr;
-    if (ctx->two_piece) return fail(ctx, TSU_EINVAL, "tsu_align_cigar: not available for 2-piece penalties");
     CigarCall call(ctx, b, scores, t);
     return call.run(out);
 }
@@ -1839,6 +1891,7 @@ extern "C" int tsu_cigars_stats(const tsu_cigars* c, int64_t* state_bytes, int32
 
 uint32_t tsu_host::kernel_bit(const Shape& sh) {
     if (sh.ck) return TSU_KERNEL_LDS_CK;
+    if (sh.two_piece) return TSU_KERNEL_2P;
     if (sh.reg_c > 0 && sh.nw == 1) return TSU_KERNEL_REG1;
     if (sh.reg_c == 16) return sh.nw <= 4 ? TSU_KERNEL_ML4 : sh.nw == 8 ? TSU_KERNEL_ML8 : TSU_KERNEL_ML16;
     if (sh.reg_c > 0) return TSU_KERNEL_REG1;

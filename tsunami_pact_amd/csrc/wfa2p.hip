@@ -27,9 +27,16 @@
 // NULL over the cells it cuts.  One wave owns everything, and LDS operations
 // of a wave complete in order, so no barrier is needed within a pair.
 // Penalties are runtime values (divided by their gcd on the host).
+//
+// With BT every cell of the range [lo, hi] of a non-null score also writes
+// its record (wfa_bt.h, 2-piece format) to global memory, the scores' cells
+// packed one after another (score s from step_off[s], first diagonal
+// step_lo[s]), and lane 0 walks them into CIGAR runs once the end cell is
+// reached.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "wfa_bt.h"
 #include "wfa_reg.h"
 
 namespace tsu {
@@ -42,7 +49,7 @@ struct P2Src {
     int lo, hi;  // lo > hi: null
 };
 
-template <typename OffT, bool QUEUE>
+template <typename OffT, bool QUEUE, bool BT = false>
 __global__ __launch_bounds__(64) void wfa2p_kernel(KArgs a) {
     constexpr int32_t NUL = P2Null<OffT>::v;
     extern __shared__ __align__(16) uint32_t smem[];
@@ -57,6 +64,14 @@ __global__ __launch_bounds__(64) void wfa2p_kernel(KArgs a) {
     OffT* ring = reinterpret_cast<OffT*>(lim + NS + 1);
     const bool lds_at_zero = (uint32_t)(uintptr_t)sS == 0u;
     const int x = a.x, oe1 = a.o + a.e, oe2 = a.o2 + a.e2, e1 = a.e, e2 = a.e2;
+    uint32_t* rec = nullptr;
+    int32_t* stepLo = nullptr;
+    int64_t* stepOff = nullptr;
+    if constexpr (BT) {
+        rec = a.rec_g + (int64_t)blockIdx.x * a.rec_cap;
+        stepLo = a.step_lo_g + (int64_t)blockIdx.x * a.step_cap;
+        stepOff = a.step_off_g + (int64_t)blockIdx.x * a.step_cap;
+    }
 
     for (int iter = 0;; ++iter) {
         int idx;
@@ -102,6 +117,10 @@ __global__ __launch_bounds__(64) void wfa2p_kernel(KArgs a) {
         __syncthreads();
         int status = (p == t && ext0 == t) ? 1 : 0;  // 1: score 0
         int s = 0;
+        int64_t rec_next = 1;  // record 0: score 0 (never read)
+        if constexpr (BT) {
+            if (lane == 0) { rec[0] = 0u; stepLo[0] = 0; stepOff[0] = 0; }
+        }
         int jm = 0, j1 = 0, j2 = 0;  // slots of score s in the M, I1/D1 and I2/D2 rings
         while (status == 0) {
             ++s;
@@ -139,6 +158,14 @@ __global__ __launch_bounds__(64) void wfa2p_kernel(KArgs a) {
             if (!nd2) { lo = min(lo, Ld2.lo - 1); hi = max(hi, Ld2.hi - 1); }
             const bool null_step = lo > hi;  // every source null (or an empty union)
             if (!null_step && (lo < kmin || hi > kmax)) { status = TSU_EKOVERFLOW; break; }
+            uint32_t* recs = nullptr;
+            if constexpr (BT) {
+                const int64_t width = null_step ? 0 : hi - lo + 1;
+                if (s >= a.step_cap || rec_next + width > a.rec_cap) { status = TSU_ECIGAR; break; }
+                if (lane == 0) { stepLo[s] = null_step ? 0 : lo; stepOff[s] = rec_next; }
+                recs = rec + rec_next - lo;  // indexed by k
+                rec_next += width;
+            }
             const bool h_i1 = !null_step && (!no1 || !ni1), h_d1 = !null_step && (!no1 || !nd1);
             const bool h_i2 = !null_step && (!no2 || !ni2), h_d2 = !null_step && (!no2 || !nd2);
             if (!null_step) { clo = min(clo, lo); chi = max(chi, hi); }
@@ -160,15 +187,33 @@ __global__ __launch_bounds__(64) void wfa2p_kernel(KArgs a) {
                 const bool act = k <= chi;
                 const int kk = act ? k : chi;  // idle lanes read a cell of the region
                 const bool in = k >= lo && k <= hi;
-                const int32_t i1p = max((int32_t)Mo1[kk - 1], (int32_t)I1e[kk - 1]);
-                const int32_t i2p = max((int32_t)Mo2[kk - 1], (int32_t)I2e[kk - 1]);
+                const int32_t mo1l = Mo1[kk - 1], i1el = I1e[kk - 1], mo2l = Mo2[kk - 1], i2el = I2e[kk - 1];
+                const int32_t mo1r = Mo1[kk + 1], d1er = D1e[kk + 1], mo2r = Mo2[kk + 1], d2er = D2e[kk + 1];
+                const int32_t i1p = max(mo1l, i1el);
+                const int32_t i2p = max(mo2l, i2el);
                 int32_t i1 = i1p >= 0 ? min(i1p + 1, t + 1) : NUL;
                 int32_t i2 = i2p >= 0 ? min(i2p + 1, t + 1) : NUL;
-                int32_t d1 = max((int32_t)Mo1[kk + 1], (int32_t)D1e[kk + 1]);
-                int32_t d2 = max((int32_t)Mo2[kk + 1], (int32_t)D2e[kk + 1]);
+                int32_t d1 = max(mo1r, d1er);
+                int32_t d2 = max(mo2r, d2er);
                 const int32_t mxv = Mx[kk];
-                int32_t m = max(max(mxv >= 0 ? mxv + 1 : NUL, max(i1, i2)), max(d1, d2));
+                const int32_t xv = mxv >= 0 ? mxv + 1 : NUL;
+                int32_t m = max(max(xv, max(i1, i2)), max(d1, d2));
                 const bool valid = in && (uint32_t)m <= (uint32_t)t && (uint32_t)(m - k) <= (uint32_t)p;
+                if constexpr (BT) {
+                    if (in) {
+                        // origin of M (ties X > D2 > D1 > I2 > I1), extension bits, M before extension
+                        uint32_t r = (i1el >= mo1l ? 8u : 0u) | (d1er >= mo1r ? 16u : 0u) |
+                                     (i2el >= mo2l ? 32u : 0u) | (d2er >= mo2r ? 64u : 0u);
+                        if (valid) {
+                            const uint32_t org = (xv >= d2 && xv >= d1 && xv >= i2 && xv >= i1) ? 1u
+                                                 : (d2 >= d1 && d2 >= i2 && d2 >= i1) ? 2u
+                                                 : (d1 >= i2 && d1 >= i1) ? 3u
+                                                 : (i2 >= i1) ? 4u : 5u;
+                            r |= ((uint32_t)m << 7) | org;
+                        }
+                        recs[k] = r;
+                    }
+                }
                 if (!valid) m = NUL;
                 if (!in || !h_i1) i1 = NUL;
                 if (!in || !h_d1) d1 = NUL;
@@ -227,12 +272,41 @@ __global__ __launch_bounds__(64) void wfa2p_kernel(KArgs a) {
             __syncthreads();
         }
         const int score = status == 1 ? 0 : status == 2 ? s : status;
-        if (lane == 0) a.scores[pi] = !lds_at_zero ? TSU_EINTERNAL : score < 0 ? score : score * a.g;
+        if constexpr (BT) {
+            __syncthreads();  // every lane's records written
+            if (lane == 0) {
+                int st = !lds_at_zero ? TSU_EINTERNAL : score < 0 ? score : 0;
+                int64_t out_off = 0;
+                int32_t out_n = 0;
+                if (st == 0) {
+                    auto record = [&](int ss, int kk, uint32_t& rr) __attribute__((always_inline)) -> bool {
+                        if (ss <= 0 || ss > s) return false;
+                        const int l0 = stepLo[ss];
+                        const int64_t o0 = stepOff[ss];
+                        const int64_t o1 = ss < s ? stepOff[ss + 1] : rec_next;
+                        const int64_t at = o0 + (kk - l0);
+                        if (kk < l0 || at >= o1) return false;
+                        rr = rec[at];
+                        return true;
+                    };
+                    st = bt2p_walk(record, score, p, t, x, a.o, e1, a.o2, e2, a, &out_off, &out_n);
+                }
+                a.run_off[pi] = out_off;
+                a.run_n[pi] = out_n;
+                a.scores[pi] = st ? st : score * a.g;
+            }
+        } else {
+            if (lane == 0) a.scores[pi] = !lds_at_zero ? TSU_EINTERNAL : score < 0 ? score : score * a.g;
+        }
         __syncthreads();  // LDS reused by the next pair of this workgroup
     }
 }
 
 KernelFn pick_2p(const Shape& sh) {
+    if (sh.bt) {  // CIGAR launches: records, queue-fed
+        if (sh.ck || !sh.queue) return nullptr;
+        return sh.off16 ? wfa2p_kernel<int16_t, true, true> : wfa2p_kernel<int32_t, true, true>;
+    }
     if (sh.off16) return sh.queue ? wfa2p_kernel<int16_t, true> : wfa2p_kernel<int16_t, false>;
     return sh.queue ? wfa2p_kernel<int32_t, true> : wfa2p_kernel<int32_t, false>;
 }

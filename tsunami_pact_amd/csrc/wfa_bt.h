@@ -127,4 +127,55 @@ __device__ __forceinline__ int bt_walk(RecFn record, int score, int p, int t, in
     return bt_end(w, out_off, out_n);
 }
 
+// Gap-affine 2-piece records (wfa2p.hip), one 32-bit word per computed cell:
+//   bits 0-2  origin of M[s][k] before extension: 1 X, 2 D2, 3 D1, 4 I2,
+//             5 I1 (0 = NULL); ties resolved in that order, each gap piece
+//             being max(open, ext): the piggy-backed order of the nine types
+//             of common.h:67-77 (M 9, D2 8/7, D1 6/5, I2 4/3, I1 2/1)
+//   bit  3    I1[s][k] came from I1[s-e1][k-1] (extension) rather than M_o1
+//   bit  4    D1[s][k] came from D1[s-e1][k+1]
+//   bit  5    I2[s][k] came from I2[s-e2][k-1]
+//   bit  6    D2[s][k] came from D2[s-e2][k+1]
+//   bits 7-31 M[s][k] before extension
+// The walk is bt_walk's with five components; runs are reserved for
+// 2 * score / min(x, o1+e1, o2+e2) + 2.
+template <typename RecFn>
+__device__ __forceinline__ int bt2p_walk(RecFn record, int score, int p, int t, int x, int o1, int e1, int o2, int e2,
+                                         const KArgs& a, int64_t* out_off, int32_t* out_n) {
+    BtWalk w;
+    int st = bt_begin(w, score, p, t, x, min(o1 + e1, o2 + e2), 0, a);
+    if (st) return st;
+    int comp = 0;  // 0 M, 1 I1, 2 D1, 3 I2, 4 D2
+    while (w.v > 0 && w.h > 0 && w.ss > 0) {
+        uint32_t r;
+        if (!record(w.ss, w.k, r)) return TSU_EINTERNAL;
+        if (comp == 0) {
+            const int pre = (int)(r >> 7);
+            const uint32_t org = r & 7u;
+            if (org == 0 || org > 5 || pre > w.off) return TSU_EINTERNAL;
+            w.emit(0, w.off - pre);
+            w.off = pre; w.v = w.off - w.k; w.h = w.off;
+            if (w.v <= 0 || w.h <= 0) break;
+            if (org == 1) { w.emit(1, 1); w.ss -= x; --w.off; w.v = w.off - w.k; w.h = w.off; continue; }
+            comp = org == 2 ? 4 : org == 3 ? 2 : org == 4 ? 3 : 1;
+        }
+        const bool piece2 = comp >= 3;
+        const int oo = piece2 ? o2 : o1, ee = piece2 ? e2 : e1;
+        if (comp == 1 || comp == 3) {
+            const bool ext = (r >> (piece2 ? 5 : 3)) & 1u;
+            w.emit(2, 1); --w.k; --w.off;
+            w.ss -= ext ? ee : oo + ee;
+            if (!ext) comp = 0;
+        } else {
+            const bool ext = (r >> (piece2 ? 6 : 4)) & 1u;
+            w.emit(3, 1); ++w.k;
+            w.ss -= ext ? ee : oo + ee;
+            if (!ext) comp = 0;
+        }
+        w.v = w.off - w.k; w.h = w.off;
+    }
+    w.type = comp == 0 ? 0 : 1;
+    return bt_end(w, out_off, out_n);
+}
+
 }  // namespace tsu
