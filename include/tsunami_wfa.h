@@ -87,10 +87,10 @@ int tsu_create(tsu_ctx** out, int device, int32_t mismatch, int32_t gap_opening,
  * min(gap_opening1 + L*gap_extension1, gap_opening2 + L*gap_extension2).
  * The reference declares this model (common.h:52-58,80-87: gap_open2,
  * gap_ext2 and the I2/D2 wavefronts) but never computes it, so this entry
- * point has no counterpart there.  Such a context runs the plain, batched
- * and CIGAR forms (one wave per pair, all five wavefronts in LDS; CIGARs
- * follow the nine backtrace types of common.h:67-77); tsu_serve_open returns
- * TSU_EINVAL for it. */
+ * point has no counterpart there.  Such a context runs every form (one wave
+ * per pair, all five wavefronts in LDS; CIGARs follow the nine backtrace
+ * types of common.h:67-77); a pair whose five rings at its capacity exceed
+ * the CU's LDS fails with TSU_EKOVERFLOW. */
 int tsu_create_2p(tsu_ctx** out, int device, int32_t mismatch, int32_t gap_opening1,
                   int32_t gap_extension1, int32_t gap_opening2, int32_t gap_extension2);
 void tsu_destroy(tsu_ctx* ctx);

@@ -87,13 +87,65 @@ def test_two_piece_tie_cigars(tsu, oracle):
             a.close()
 
 
-def test_two_piece_serve_refuses(tsu):
-    """The served form refuses a 2-piece context with TSU_EINVAL."""
+def test_two_piece_served(tsu, oracle):
+    """The served form with 2-piece penalties: requests submitted from two
+    threads while others are on the device equal the oracle and the same
+    request served alone."""
+    import threading
+    pen = (4, 6, 2, 24, 1)
+    a = tsu.Aligner(pen[0], pen[1], pen[2], gap_opening2=pen[3], gap_extension2=pen[4])
+    try:
+        rng = np.random.default_rng(SEED + 60)
+        reqs = []
+        for r in range(24):
+            k = int(rng.integers(1, 65))
+            ds = tsu.Dataset.generate(k, seed=SEED + 600 + r, len_min=50, len_max=1000, err_min=0.01, err_max=0.15)
+            reqs.append(ds.pairs())
+        srv = tsu.Server(a, hbm_budget_bytes=1 << 30, max_inflight=4, max_batch_pairs=128)
+        results, errors = {}, []
+
+        def client(idx):
+            try:
+                for i in idx:
+                    tk = srv.submit(reqs[i])
+                    results[i] = srv.wait(tk, len(reqs[i]))
+            except Exception as ex:  # surfaced below
+                errors.append(ex)
+
+        say(f"2-piece served: {len(reqs)} requests, {sum(map(len, reqs))} pairs")
+        ths = [threading.Thread(target=client, args=(list(range(j, len(reqs), 2)),)) for j in range(2)]
+        for th in ths:
+            th.start()
+        for th in ths:
+            th.join()
+        assert not errors, errors
+        for i, pairs in enumerate(reqs):
+            st, sc, _ = results[i]
+            assert st == tsu.TSU_OK
+            want, _ = oracle.wfa2p_batch(pairs, *pen)
+            assert (sc == want).all(), i
+        say("2-piece served alone")
+        for i in (0, 5, 17):
+            tk = srv.submit(reqs[i])
+            st, sc, _ = srv.wait(tk, len(reqs[i]))
+            assert st == tsu.TSU_OK and (sc == results[i][1]).all()
+        assert srv.close() == tsu.TSU_OK
+    finally:
+        a.close()
+
+
+def test_two_piece_beyond_lds_reports_overflow(tsu):
+    """A 2-piece pair whose rings at the capacity it needs exceed the LDS (the
+    2-piece kernel has no HBM ring): the batched form widens up to the widest
+    window that fits and then fails with TSU_EKOVERFLOW."""
+    rng = np.random.default_rng(SEED + 70)
+    wide = ("".join(rng.choice(list("ACGT"), 6000)), "".join(rng.choice(list("ACGT"), 6000)))
     a = tsu.Aligner(4, 6, 2, gap_opening2=24, gap_extension2=1)
     try:
-        say("2-piece serve refusal")
+        b = a.batch([("ACGTACGT", "ACGAACGT"), wide])
+        say("2-piece pair beyond the LDS")
         with pytest.raises(tsu.TsuError) as ei:
-            tsu.Server(a)
-        assert ei.value.status == tsu.TSU_EINVAL
+            a.align_batched(b)
+        assert ei.value.status == tsu.TSU_EKOVERFLOW
     finally:
         a.close()

@@ -91,6 +91,14 @@ int32_t capacity_for_score(const tsu_ctx* ctx, int64_t s_scaled) {
 int32_t tsu_host::capacity_full(int32_t p, int32_t t) { return p + t + 3; }
 using tsu_host::capacity_full;
 
+// A pair overflowed the widest window its form may use: the caller's error.
+static int overflow_at(tsu_ctx* ctx, int32_t W, int32_t max_len) {
+    if (W < capacity_full(max_len, max_len))  // 2-piece: the LDS holds no wider rings
+        return fail(ctx, TSU_EKOVERFLOW, "a pair's 2-piece wavefront range does not fit the CU's LDS (" +
+                                             std::to_string(W) + " diagonals)");
+    return fail(ctx, TSU_EINTERNAL, "overflow at full capacity");
+}
+
 // ---------------------------------------------------------------------------
 // context
 // ---------------------------------------------------------------------------
@@ -339,6 +347,22 @@ int default_waves(int32_t W) {
 }
 
 }  // namespace
+
+int32_t tsu_host::capacity_max(const tsu_ctx* ctx, int32_t max_len) {
+    const int32_t full = capacity_full(max_len, max_len);
+    if (!ctx->two_piece) return full;
+    Shape s2{1, max_len <= 32000, true, false, true};
+    s2.two_piece = true;
+    KArgs a = base_args(ctx);
+    a.seq_words = seq_words_for(max_len);
+    int32_t lo = 0, hi = full;  // lo fits (or is 0), hi the highest candidate
+    while (lo < hi) {
+        const int32_t mid = lo + (hi - lo + 1) / 2;
+        a.W = mid;
+        if (tsu::lds_bytes(s2, a) <= ctx->max_lds) lo = mid; else hi = mid - 1;
+    }
+    return std::max<int32_t>(lo, 1);
+}
 
 int64_t tsu_host::resident_grid(const tsu_ctx* ctx, const Shape& sh, const KArgs& a, int* status) {
     int per_cu = 0;
@@ -789,7 +813,7 @@ static int batched_core(tsu_ctx* ctx, tsu_batch* b, const std::vector<int32_t>& 
         const int32_t max_len = max_len_of(bk.idx);
         const tsu_tune_entry* te = use_tune ? tune_lookup(ctx, bk.bound) : nullptr;
         int32_t W = (te && te->capacity > 0) ? te->capacity : initial_capacity(ctx, std::min(bk.bound, max_len));
-        W = std::min<int32_t>(W, capacity_full(max_len, max_len));
+        W = std::min<int32_t>(W, tsu_host::capacity_max(ctx, max_len));
         const int nw_hint = te ? te->waves_per_pair : 0;
         if (tsu_host::probe_bucket(ctx, W, te ? te->capacity : 0, nw_hint)) {
             probed_bucket[(size_t)bi] = 1;
@@ -835,7 +859,7 @@ static int batched_core(tsu_ctx* ctx, tsu_batch* b, const std::vector<int32_t>& 
                     // the probe window overflowed: the bucket's own capacity
                     const int32_t ml = std::max(b->pairs[(size_t)i].p_len, b->pairs[(size_t)i].t_len);
                     const std::pair<int, int32_t> key{bi, std::min<int32_t>(
-                        initial_capacity(ctx, std::min(buckets[(size_t)bi].bound, ml)), capacity_full(ml, ml))};
+                        initial_capacity(ctx, std::min(buckets[(size_t)bi].bound, ml)), tsu_host::capacity_max(ctx, ml))};
                     by[key].push_back(i);
                     by_cost[key] += 0.25 * ml * waves_of(key.second);
                     continue;
@@ -925,7 +949,7 @@ static int batched_core(tsu_ctx* ctx, tsu_batch* b, const std::vector<int32_t>& 
         Group& g = groups[gi];
         auto rn = std::make_unique<Run>();
         rn->max_len = g.max_len;
-        rn->W = std::min<int32_t>(g.W, capacity_full(g.max_len, g.max_len));
+        rn->W = std::min<int32_t>(g.W, tsu_host::capacity_max(ctx, g.max_len));
         rn->nw_hint = g.nw_hint;
         rn->order = d_order.p + off;
         rn->counter = d_counters.p + gi;
@@ -959,10 +983,10 @@ static int batched_core(tsu_ctx* ctx, tsu_batch* b, const std::vector<int32_t>& 
                 if (tmp[(size_t)i] == TSU_EKOVERFLOW) again.push_back(i);
             }
             if (again.empty()) continue;
-            if (rn->W >= capacity_full(rn->max_len, rn->max_len)) return fail(ctx, TSU_EINTERNAL, "overflow at full capacity");
+            if (rn->W >= tsu_host::capacity_max(ctx, rn->max_len)) return overflow_at(ctx, rn->W, rn->max_len);
             retried += (int)again.size();
             auto nx = std::make_unique<Run>();
-            nx->W = (int32_t)std::min<int64_t>(2ll * rn->W, capacity_full(rn->max_len, rn->max_len));
+            nx->W = (int32_t)std::min<int64_t>(2ll * rn->W, tsu_host::capacity_max(ctx, rn->max_len));
             nx->max_len = rn->max_len;
             nx->nw_hint = 0;  // wider re-runs take the default shape of their capacity
             nx->order = rn->order;      // the finished launch's region and counter
@@ -1065,7 +1089,7 @@ extern "C" int tsu_align_batched_noalloc(tsu_ctx* ctx, const uint32_t* d_words, 
         const int nw_hint = te ? te->waves_per_pair : 0;
         int32_t* region = d_order + base[(size_t)bi];
         for (int attempt = 0; !todo.empty(); ++attempt) {
-            W = std::min<int32_t>(W, capacity_full(max_len, max_len));
+            W = std::min<int32_t>(W, tsu_host::capacity_max(ctx, max_len));
             const Shape sh = batched_shape(ctx, W, max_len, attempt == 0 ? nw_hint : 0);
             if (!sh.lds_ring) {
                 for (int32_t i : todo) scores[i] = TSU_EKOVERFLOW;
@@ -1092,9 +1116,9 @@ extern "C" int tsu_align_batched_noalloc(tsu_ctx* ctx, const uint32_t* d_words, 
             for (int32_t i : todo)
                 if (scores[i] == TSU_EKOVERFLOW) again.push_back(i);
             if (!again.empty()) {
-                if (W >= capacity_full(max_len, max_len)) return fail(ctx, TSU_EINTERNAL, "overflow at full capacity");
+                if (W >= tsu_host::capacity_max(ctx, max_len)) return overflow_at(ctx, W, max_len);
                 retried += (int)again.size();
-                W = (int32_t)std::min<int64_t>(2ll * W, capacity_full(max_len, max_len));
+                W = (int32_t)std::min<int64_t>(2ll * W, tsu_host::capacity_max(ctx, max_len));
             }
             todo.swap(again);
         }

@@ -188,7 +188,7 @@ int32_t bucket_capacity(const tsu_ctx* ctx, int bucket, int32_t max_len, int* wa
     const int32_t tcap = tsu_host::tuned_capacity(ctx, bound);
     if (waves) *waves = tsu_host::tuned_waves(ctx, bound);
     return std::min(tcap > 0 ? tcap : tsu_host::initial_capacity(ctx, std::min(bound, max_len)),
-                    tsu_host::capacity_full(max_len, max_len));
+                    tsu_host::capacity_max(ctx, max_len));
 }
 
 // Pairs of this bucket go through the probe stage first.
@@ -216,7 +216,7 @@ int64_t plan_flight(tsu_server* s, Slot& sl, int32_t W_entry, int* status) {
     const int bucket = tsu_host::bucket_of(lp.max_len);
     int waves = 0;
     const int32_t W0 = bucket_capacity(ctx, bucket, lp.max_len, &waves);
-    const int32_t full = tsu_host::capacity_full(lp.max_len, lp.max_len);
+    const int32_t full = tsu_host::capacity_max(ctx, lp.max_len);
     KArgs a = tsu_host::base_args(ctx);
     a.seq_words = tsu_host::seq_words_for(lp.max_len);
     if (sl.probing) {
@@ -449,11 +449,11 @@ void retire(tsu_server* s, Slot& sl) {
         max_len = std::max(max_len, std::max(sl.fpairs[(size_t)i].p_len, sl.fpairs[(size_t)i].t_len));
     }
     const int32_t W = sl.plan.empty() ? 0 : sl.plan[0].W;
-    if (!again.empty() && W < tsu_host::capacity_full(max_len, max_len)) {
+    if (!again.empty() && W < tsu_host::capacity_max(ctx, max_len)) {
         LaunchPlan lp;
         lp.first = 0;
         lp.count = (int32_t)again.size();
-        lp.W = (int32_t)std::min<int64_t>(2ll * W, tsu_host::capacity_full(max_len, max_len));
+        lp.W = (int32_t)std::min<int64_t>(2ll * W, tsu_host::capacity_max(ctx, max_len));
         lp.max_len = max_len;
         lp.sh = tsu_host::batched_shape(ctx, lp.W, max_len, 0);
         KArgs a = tsu_host::base_args(ctx);
@@ -551,7 +551,6 @@ void worker_loop(tsu_server* s) {
 }  // namespace
 
 extern "C" int tsu_serve_open(tsu_ctx* ctx, const tsu_serve_cfg* cfg, tsu_server** out) {
-    if (ctx && ctx->two_piece) return tsu_host::fail(ctx, TSU_EINVAL, "tsu_serve_open: not available for 2-piece penalties");
     if (!ctx || !cfg || !out) return TSU_EINVAL;
     *out = nullptr;
     if (cfg->hbm_budget_bytes < (1 << 20) || cfg->max_inflight < 1 || cfg->max_inflight > 64 || cfg->max_batch_pairs < 1)
