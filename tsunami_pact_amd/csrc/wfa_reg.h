@@ -10,15 +10,14 @@
 // is unrolled over the M ring's depth so every slot index is a constant.
 //
 // All diagonals of the C*64-wide window are computed implicitly: a chunk is
-// computed only when one of its sources can be non-NULL, and a diagonal
-// outside WFA2-lib's range [lo, hi] has only NULL sources, so it comes out
-// NULL exactly as the range-limited computation leaves it.  Activity is
-// tracked per chunk with scalar bit masks: any non-NULL value, and a
-// non-NULL value in lane 0 / lane 63 (only those reach the neighbouring
-// chunk through k-1 / k+1).  Trim-ends (wfa_gpu.cu:633-719) reduce to: NULL
-// every I/D cell below the first and above the last in-bounds cell of that
-// wavefront; M cells outside the bounds are NULL already.  The common case
-// (no out-of-bounds I/D value at all) is detected from one ballot per chunk.
+// computed when it meets the wavefront limits WFA2-lib's range [lo, hi]
+// would have (a scalar recurrence per score), and a diagonal outside that
+// range has only NULL sources, so it comes out NULL exactly as the
+// range-limited computation leaves it.  Trim-ends (wfa_gpu.cu:633-719)
+// reduce to: NULL every I/D cell below the first and above the last
+// in-bounds cell of that wavefront; M cells outside the bounds are NULL
+// already.  The common case (no out-of-bounds I/D value at all) is detected
+// from one ballot per chunk of the maximum before the bounds test.
 //
 // Sequences sit in LDS as interleaved 2-bit packed words (pattern word i at
 // 2i, text word i at 2i+1), so both are addressed from LDS offset 0.
@@ -156,11 +155,10 @@ __device__ __forceinline__ int lcp_interleaved(const uint32_t* S, int v, int h, 
 template <int C, int XS, int OS, int ES, bool QUEUE, bool PROBE = false>
 __global__ __launch_bounds__(64) void wfa_reg_kernel(KArgs a) {
     static_assert(ES == 1, "in-place I/D needs e == 1 after gcd scaling");
-    static_assert(C >= 1 && C <= 8, "window of 64..512 diagonals (8-bit chunk fields)");
+    static_assert(C >= 1 && C <= 8, "window of 64..512 diagonals");
     constexpr int RM = (XS > OS + ES ? XS : OS + ES) + 1;
     constexpr int32_t NUL = kRegNull;
     constexpr int W = 64 * C;
-    constexpr uint32_t ALL = (1u << C) - 1u;
     extern __shared__ __align__(16) uint32_t smem[];
     const int lane = threadIdx.x;
     const int lane2 = 2 * lane;
@@ -211,9 +209,24 @@ __global__ __launch_bounds__(64) void wfa_reg_kernel(KArgs a) {
             Ir[c] = NUL;
             Dr[c] = NUL;
         }
-        uint32_t F[RM];  // step masks of the last RM scores (slot s mod RM)
+        // Window edges that are not bounds of the alignment: a non-NULL cell
+        // in the window's first (last) lane then means the range may leave
+        // the window.  Diagonals -p-1 and t+1 hold no cell in bounds.
+        const uint32_t edges = __builtin_amdgcn_readfirstlane((kmin > -p - 1 ? 1u : 0u) | (kmax < t + 1 ? 2u : 0u));
+        // Ranges of window diagonals (j = k - kmin) that may hold a non-NULL
+        // cell, as WFA2-lib's wavefront limits (wfa_gpu.cu:426-468) with
+        // e = 1: I/D at s within [min(lo M[s-o-e], lo I/D[s-1]) - 1,
+        // max(hi M[s-o-e], hi I/D[s-1]) + 1], M at s the union of that and
+        // M[s-x].  The ranges only grow (trimmed cells are NULLed but the
+        // limits stay), so the range at s also covers every chunk whose ring
+        // slot still holds a non-NULL M of score s-RM.  mtop/mbot hold the M
+        // limits per ring slot, it/ib those of I/D at s-1.
+        constexpr int RNONE = 1 << 20;  // limits of an empty range: top -RNONE, bottom +RNONE
+        int mtop[RM], mbot[RM];
 #pragma unroll
-        for (int r = 0; r < RM; ++r) F[r] = 0;
+        for (int r = 0; r < RM; ++r) { mtop[r] = -RNONE; mbot[r] = RNONE; }
+        mtop[0] = -kmin; mbot[0] = -kmin;  // score 0: M[0][0] only
+        int it = -RNONE, ib = RNONE;
 
         // score 0: M[0][0] = 0, extended (every lane computes it)
         const int ext0 = lcp_interleaved(sS, 0, 0, min(p, t));
@@ -222,7 +235,6 @@ __global__ __launch_bounds__(64) void wfa_reg_kernel(KArgs a) {
 #pragma unroll
             for (int c = 0; c < C; ++c)
                 if (c == c0) Mr[0][c] = lane == l0 ? 2 * ext0 : NUL;
-            F[0] = (1u << c0) | (l0 == 0 ? 1u << (8 + c0) : 0u) | (l0 == 63 ? 1u << (16 + c0) : 0u);
         }
         int s = 0;
 
@@ -242,14 +254,16 @@ __global__ __launch_bounds__(64) void wfa_reg_kernel(KArgs a) {
             // are recomputed, which costs less than the occupancy
             int zero_s;
             asm volatile("s_mov_b32 %0, 0" : "=s"(zero_s));
-            constexpr int P1 = (PH - 1 + RM) % RM;  // score s-1 (its I and D are the I_e, D_e sources)
-            // chunk c computes when it may hold a non-NULL source (scores s-1,
-            // s-x, s-o-e) or an old value to overwrite (scores s-1 and s-RM),
-            // or when lane 63 of chunk c-1 feeds its I (k-1) or lane 0 of
-            // chunk c+1 feeds its D (k+1)
-            const uint32_t src = F[P1] | F[SO];
-            const uint32_t act = ((src | F[SX] | F[PH]) | ((src >> 16) << 1) | (((src >> 8) & 0xffu) >> 1)) & ALL;
-            uint32_t acc = 0;      // step mask of score s, fields shifted up by one bit
+            // limits at s; chunk c computes when it meets the M range
+            it = max(mtop[SO], it) + 1;
+            ib = min(mbot[SO], ib) - 1;
+            mtop[PH] = max(mtop[SX], it);
+            mbot[PH] = min(mbot[SX], ib);
+            const int jlo = max(mbot[PH], 0), jhi = min(mtop[PH], W - 1);
+            const uint32_t act = jlo <= jhi ? (2u << (jhi >> 6)) - (1u << (jlo >> 6)) : 0u;
+            uint32_t eg = edges;  // opaque per step: scalar tests below
+            asm volatile("" : "+s"(eg));
+            uint32_t edge = 0;     // a non-NULL cell in an edge lane of the window
             uint64_t oob = 0;      // some I/D cell out of bounds
             int32_t mend = NUL;    // M[s][t-p]
             int32_t carryI = NUL;  // old I of the previous chunk's lane 63
@@ -274,39 +288,30 @@ __global__ __launch_bounds__(64) void wfa_reg_kernel(KArgs a) {
                     // only produces out-of-bounds values, as in WFA2-lib.
                     const int32_t I = max(mol, iel) + 2;
                     const int32_t D = max(mor, der);
-                    int32_t M = max(max(D, Mr[SX][c] + 2), I);
-                    const bool valid = (uint32_t)M < (uint32_t)mt2;
-                    // lanes where M, I or D may be non-NULL (M before the
-                    // bounds test is their maximum, or a mismatch out of
-                    // bounds); those failing the bounds test include every
-                    // out-of-bounds I or D, so trimming on them is exact (a
-                    // trim that finds nothing out of bounds changes nothing)
-                    const uint64_t b = vote(M >= 0);
-                    oob |= b & ~vote(valid);
-                    M = valid ? M : NUL;
+                    const int32_t Mp = max(max(D, Mr[SX][c] + 2), I);  // M before the bounds test
+                    const bool valid = (uint32_t)Mp < (uint32_t)mt2;
+                    int32_t M = valid ? Mp : NUL;
+                    // Mp is out of bounds where I or D is, or the mismatch
+                    // (then trimming finds nothing and changes nothing)
+                    oob |= vote(Mp >= mt2);
+                    // Mp is non-NULL where M, I or D is, or a mismatch from a
+                    // non-NULL M[s-x] (an edge hit x scores earlier)
+                    if (c == 0) edge |= (uint32_t)(vote(Mp >= 0) & 1ull) & eg;
+                    if (c == C - 1) edge |= (uint32_t)(vote(Mp >= 0) >> 63) & (eg >> 1);
                     const int lim2 = mt2 - 2 - M;  // 2 min(p - v, t - M) for a valid M
                     // extend from the packed words (v >= 0 for every valid M)
                     if (valid && lim2 > 0) M += extend2((uint32_t)(M - k2), (uint32_t)M, lim2);
                     Mr[PH][c] = M;
                     Ir[c] = I;
                     Dr[c] = D;
-                    // Chunk flags as one scalar word pinned to an SGPR, so the
-                    // compiler carries no lane booleans across the chunk
-                    // branches (two vector ops per flag).  The fields sit one
-                    // bit up: a select of 1 : 0 becomes a zero-extension, which
-                    // is also lowered through a VGPR.
-                    uint32_t fl = (b ? 2u : 0u) | ((uint32_t)(b & 1ull) << 9) | ((uint32_t)(b >> 63) << 17);
-                    asm volatile("" : "+s"(fl));
-                    acc |= fl << c;
                     if (c == ctp) mend = __builtin_amdgcn_readlane(M, ltp);
                 } else {
                     carryI = NUL;  // an idle chunk holds only NULL
                 }
             }
-            uint32_t Fs = acc >> 1;
             if (oob) {
                 // exact trim: NULL I/D cells outside [first, last] in-bounds
-                // cell, then rebuild the step mask
+                // cell, then test the edge lanes again
                 auto trim = [&](int32_t (&R)[C]) __attribute__((always_inline)) {
                     int f = 0x3fffffff, l = -1;
 #pragma unroll
@@ -327,20 +332,13 @@ __global__ __launch_bounds__(64) void wfa_reg_kernel(KArgs a) {
                 };
                 trim(Ir);
                 trim(Dr);
-                uint32_t m2 = 0;  // fields shifted up by one bit, as in acc
-#pragma unroll
-                for (int c = 0; c < C; ++c) {
-                    const uint64_t b = vote(max(max(Mr[PH][c], Ir[c]), Dr[c]) >= 0);
-                    m2 |= ((b ? 2u : 0u) | ((uint32_t)(b & 1ull) << 9) | ((uint32_t)(b >> 63) << 17)) << c;
-                }
-                Fs = m2 >> 1;
+                edge = (uint32_t)(vote(max(max(Mr[PH][0], Ir[0]), Dr[0]) >= 0) & 1ull) & eg;
+                edge |= (uint32_t)(vote(max(max(Mr[PH][C - 1], Ir[C - 1]), Dr[C - 1]) >= 0) >> 63) & (eg >> 1);
             }
-            F[PH] = Fs;
             if (mend >= 2 * t) return s;
             // A non-NULL cell at a window edge could extend the range beyond
             // it (lane 0 of chunk 0, lane 63 of chunk C-1)
-            if ((Fs >> 8) & 1u) return TSU_EKOVERFLOW;
-            if ((Fs >> (16 + C - 1)) & 1u) return TSU_EKOVERFLOW;
+            if (edge) return TSU_EKOVERFLOW;
             return 0;
         };
 
