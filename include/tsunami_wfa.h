@@ -241,6 +241,16 @@ int tsu_align_batched_noalloc(tsu_ctx* ctx, const uint32_t* d_words, int64_t n_w
  * context (also used by later tsu_align_batched calls).  Optional. */
 int tsu_autotune(tsu_ctx* ctx, tsu_batch* b, int32_t sample_pairs);
 int tsu_tune_table(const tsu_ctx* ctx, tsu_tune_entry* entries, int32_t cap, int32_t* n);
+/* Replaces the context's per-bucket table: a table saved from
+ * tsu_tune_table, or a first window pinned for a bucket (capacity > 0 runs
+ * the bucket's pairs in that window first, also in the CIGAR form, instead of
+ * placing each pair by the probe).  Entries name buckets by bucket_bound
+ * (one of 128, 256, 512, 1024, 2048, 4096, 8192, 16383, 32000, 67108864),
+ * each at most once; waves_per_pair 0, 1, 2, 4, 8 or 16 (above 1: the LDS
+ * kernel with that many waves per pair, capacity 0).  n = 0 clears the
+ * table.  TSU_EINVAL on any other entry, leaving the table as it was.  No
+ * reference counterpart (the reference has no tuning). */
+int tsu_set_tune_table(tsu_ctx* ctx, const tsu_tune_entry* entries, int32_t n);
 
 /* Scores and CIGARs.  Backtrace follows WFA2-lib's piggy-backed priority
  * (common.h:67-77).  Pairs whose length bucket runs on a register kernel are

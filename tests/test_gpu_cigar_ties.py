@@ -5,7 +5,9 @@ priority each (pinned on the oracle in test_oracle_ties.py).  A common
 prefix of S bases leaves a case's CIGAR as |S|M + its own, so the prefix
 length picks the kernel that records the pair (the CIGAR form chooses by
 length bucket and capacity); the test checks the kernel that actually ran
-through tsu_cigars_kernels.
+through tsu_cigars_kernels.  From 2,000 bases on, the bucket's window is
+pinned through the tune table: left to the probe, these near-identical
+pairs would all be scored by it and recorded in the narrowest window.
 """
 import random
 import sys
@@ -46,25 +48,46 @@ def merge_prefix(n, cigar):
 
 CASES_462 = TIES_462 + HOMOPOLYMER_462
 
-# (name, prefix length, kernel expected to record)
+# (name, prefix length, kernel expected to record, pinned window: (bucket bound, diagonals))
 PATHS_462 = [
-    ("one wave, 1 chunk", 0, "reg1"),
-    ("one wave, 8 chunks", 300, "reg1"),
-    ("one wave, 16 chunks", 700, "reg1"),
-    ("16 chunks x 4 waves", 2000, "ml4"),
-    ("16 chunks x 8 waves", 5000, "ml8"),
-    ("16 chunks x 16 waves, doubled offsets", 10000, "ml16"),
-    ("16 chunks x 16 waves, plain offsets", 20000, "ml16"),
-    ("LDS kernel, int32 offsets", 33000, "lds"),
+    ("one wave, 1 chunk", 0, "reg1", None),
+    ("one wave, 8 chunks", 300, "reg1", None),
+    ("one wave, 16 chunks", 700, "reg1", None),
+    ("16 chunks x 4 waves", 2000, "ml4", (2048, 4096)),
+    ("16 chunks x 8 waves", 5000, "ml8", (8192, 8192)),
+    ("16 chunks x 16 waves, doubled offsets", 10000, "ml16", (16383, 16384)),
+    ("16 chunks x 16 waves, plain offsets", 20000, "ml16", (32000, 16384)),
+    ("LDS kernel, int32 offsets", 33000, "lds", None),
 ]
 
 
-@pytest.mark.parametrize("name,n,kernel", PATHS_462, ids=[p[0] for p in PATHS_462])
-def test_tie_cigars_462(aligner, name, n, kernel):
+@pytest.mark.parametrize("name,n,kernel,pin", PATHS_462, ids=[p[0] for p in PATHS_462])
+def test_tie_cigars_462(aligner, name, n, kernel, pin):
     pairs, want = prefixed(CASES_462, n, 11 + n)
     say(f"tie CIGARs 4/6/2, {name}: {len(pairs)} pairs, prefix {n}")
-    got, cig, t = aligner.align_cigar(aligner.batch(pairs))
+    if pin:
+        aligner.set_tune_table([{"waves_per_pair": 1, "capacity": pin[1], "bucket_bound": pin[0]}])
+    try:
+        got, cig, t = aligner.align_cigar(aligner.batch(pairs))
+    finally:
+        aligner.set_tune_table([])
     assert t["kernels"] == {kernel}, t["kernels"]
+    assert cig == want
+
+
+def test_tie_cigars_462_probe_placed(aligner):
+    """Left to the probe, the prefixed pairs of 2-20 kbp are scored by it
+    (their scores are far below its bound) and recorded at their exact
+    capacity: by the single-wave recorder, or, beyond the 16,000 bases it
+    holds, by the LDS kernel."""
+    pairs, want = [], []
+    for n in (2000, 5000, 10000, 20000):
+        p, w = prefixed(CASES_462, n, 11 + n)
+        pairs += p
+        want += w
+    say(f"tie CIGARs 4/6/2, probe-placed: {len(pairs)} pairs")
+    got, cig, t = aligner.align_cigar(aligner.batch(pairs))
+    assert t["kernels"] == {"reg1", "lds"}, t["kernels"]
     assert cig == want
 
 

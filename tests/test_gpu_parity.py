@@ -395,6 +395,22 @@ def test_autotune_then_batched(tsu, oracle):
         assert table and all(e["waves_per_pair"] in (1, 2, 4, 8, 16) for e in table)
         got, _ = a.align_batched(b)
         assert (got == want).all()
+        # a saved table restores as it was; a bad entry leaves it unchanged
+        a.set_tune_table([])
+        assert a.tune_table() == []
+        a.set_tune_table(table)
+        assert a.tune_table() == table
+        for bad in ({"bucket_bound": 1000}, {"bucket_bound": 2048, "waves_per_pair": 3},
+                    {"bucket_bound": 2048, "capacity": -1}, {"bucket_bound": 2048, "waves_per_pair": 4, "capacity": 512}):
+            with pytest.raises(tsu.TsuError):
+                a.set_tune_table([bad])
+        with pytest.raises(tsu.TsuError):
+            a.set_tune_table([{"bucket_bound": 2048}, {"bucket_bound": 2048}])
+        assert a.tune_table() == table
+        # a pinned first window of 64 diagonals: pairs re-run wider, same scores
+        a.set_tune_table([{"waves_per_pair": 1, "capacity": 64, "bucket_bound": bb} for bb in (512, 1024, 2048, 4096)])
+        got, t = a.align_batched(b)
+        assert (got == want).all() and t["retried"] > 0
     finally:
         a.close()
 

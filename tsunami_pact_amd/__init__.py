@@ -48,7 +48,7 @@ EXPORTS = (
     "tsu_batch_from_dataset", "tsu_batch_create", "tsu_batch_size", "tsu_batch_device", "tsu_batch_bytes",
     "tsu_batch_free",
     "tsu_align_plain", "tsu_align_fixed", "tsu_align_batched", "tsu_align_batched_noalloc", "tsu_autotune",
-    "tsu_tune_table",
+    "tsu_tune_table", "tsu_set_tune_table",
     "tsu_align_cigar", "tsu_cigars_get", "tsu_cigars_string", "tsu_cigars_free",
     "tsu_set_cigar_memory", "tsu_cigars_stats", "tsu_cigars_kernels", "tsu_set_poll_limit",
     "tsu_serve_open", "tsu_serve_submit", "tsu_serve_submit_packed", "tsu_serve_poll", "tsu_serve_wait",
@@ -147,6 +147,7 @@ def library() -> ctypes.CDLL:
         "tsu_align_batched_noalloc": (ctypes.c_int, [vp, vp, i64, vp, P(Pair), i32, vp, P(i32), P(Timing)]),
         "tsu_autotune": (ctypes.c_int, [vp, vp, i32]),
         "tsu_tune_table": (ctypes.c_int, [vp, P(TuneEntry), i32, P(i32)]),
+        "tsu_set_tune_table": (ctypes.c_int, [vp, P(TuneEntry), i32]),
         "tsu_align_cigar": (ctypes.c_int, [vp, vp, P(i32), P(vp), P(Timing)]),
         "tsu_cigars_get": (ctypes.c_int, [vp, i64, P(P(ctypes.c_uint32)), P(i32)]),
         "tsu_cigars_string": (ctypes.c_int, [vp, i64, ctypes.c_char_p, i32]),
@@ -410,6 +411,16 @@ class Aligner:
         self._check(self._lib.tsu_tune_table(self._h, arr, 64, ctypes.byref(n)), "tsu_tune_table")
         return [{"waves_per_pair": arr[i].waves_per_pair, "capacity": arr[i].capacity,
                  "bucket_bound": arr[i].bucket_bound} for i in range(min(n.value, 64))]
+
+    def set_tune_table(self, entries: Sequence[dict]) -> None:
+        """Replaces the per-bucket table (entries as tune_table returns
+        them; an empty list clears it)."""
+        arr = (TuneEntry * max(len(entries), 1))()
+        for i, e in enumerate(entries):
+            arr[i].waves_per_pair = int(e.get("waves_per_pair", 0))
+            arr[i].capacity = int(e.get("capacity", 0))
+            arr[i].bucket_bound = int(e["bucket_bound"])
+        self._check(self._lib.tsu_set_tune_table(self._h, arr, len(entries)), "tsu_set_tune_table")
 
     CIGAR_AUTO, CIGAR_RECORDS, CIGAR_CHECKPOINTS = 0, 1, 2
 

@@ -715,11 +715,13 @@ int tsu_host::tuned_waves(const tsu_ctx* ctx, int32_t bound) {
 using tsu_host::batched_shape;
 using tsu_host::initial_capacity;
 
-static const tsu_tune_entry* tune_lookup(const tsu_ctx* ctx, int32_t bound) {
-    for (const tsu_tune_entry& te : ctx->tune)
+static const tsu_tune_entry* tune_lookup_in(const std::vector<tsu_tune_entry>& table, int32_t bound) {
+    for (const tsu_tune_entry& te : table)
         if (te.bucket_bound == bound) return &te;
     return nullptr;
 }
+
+static const tsu_tune_entry* tune_lookup(const tsu_ctx* ctx, int32_t bound) { return tune_lookup_in(ctx->tune, bound); }
 
 // Probe of the batched form: pairs whose bucket would run on a multi-wave
 // window are first aligned by the single-wave register kernel (512
@@ -759,6 +761,36 @@ int32_t tsu_host::probe_capacity(const tsu_ctx* ctx, const tsu_pair& pr, int32_t
     }
     // (a launch clamps it to what its longest pair can use)
     return (int32_t)std::min<int64_t>(c, 1 << 26);
+}
+
+// The probe launch over `ids` (reordered longest first; d_order holds at
+// least ids.size() entries).  On return sc[i] holds the score of each pair
+// it finished, kProbeStopped for a pair stopped at kProbeCap (prog[i] then
+// holds how far along both sequences it got) or TSU_EKOVERFLOW where the
+// 512-diagonal window overflowed; sc and prog hold b->n entries.
+static int run_probe(tsu_ctx* ctx, tsu_batch* b, std::vector<int32_t>& ids, int32_t* d_order, std::vector<int32_t>& sc,
+                     std::vector<int32_t>& prog, double* kms) {
+    tsu_host::order_longest_first(ids, b->pairs.data());
+    int32_t max_len = 0;
+    for (int32_t i : ids) max_len = std::max(max_len, std::max(b->pairs[(size_t)i].p_len, b->pairs[(size_t)i].t_len));
+    DevBuf<int32_t> d_prog;
+    HIPCHK(d_prog.alloc((size_t)std::max(b->n, 1)), "hipMalloc (probe)");
+    KArgs pa = base_args(ctx);
+    pa.probe_cap = tsu_host::kProbeCap;
+    pa.probe_out = d_prog.p;
+    Shape ps{1, max_len <= 32000, true, false, true};
+    ps.fixed_242 = true;
+    ps.reg_c = 8;
+    ps.probe = true;
+    HIPCHK(hipMemcpyAsync(d_order, ids.data(), ids.size() * sizeof(int32_t), hipMemcpyHostToDevice, ctx->stream),
+           "hipMemcpyAsync (order)");
+    int r = run_shape(ctx, io_of(b), ps, 512, d_order, (int32_t)ids.size(), max_len, 0, &pa, kms);
+    if (r != TSU_OK) return r;
+    sc.assign((size_t)b->n, 0);
+    prog.assign((size_t)b->n, 0);
+    HIPCHK(hipMemcpy(sc.data(), b->d_scores, (size_t)b->n * sizeof(int32_t), hipMemcpyDeviceToHost), "hipMemcpy D2H (scores)");
+    HIPCHK(hipMemcpy(prog.data(), d_prog.p, (size_t)b->n * sizeof(int32_t), hipMemcpyDeviceToHost), "hipMemcpy D2H (probe)");
+    return TSU_OK;
 }
 
 // Aligns the pairs listed in `sel` (pair indices) with persistent launches
@@ -826,27 +858,12 @@ static int batched_core(tsu_ctx* ctx, tsu_batch* b, const std::vector<int32_t>& 
         groups.push_back(std::move(g));
     }
     if (!probe_ids.empty()) {
-        tsu_host::order_longest_first(probe_ids, b->pairs.data());
-        const int32_t max_len = max_len_of(probe_ids);
-        DevBuf<int32_t> d_prog;
-        HIPCHK(d_prog.alloc(probe_ids.size() > 0 ? (size_t)b->n : 1), "hipMalloc (probe)");
-        KArgs pa = base_args(ctx);
-        pa.probe_cap = tsu_host::kProbeCap;
-        pa.probe_out = d_prog.p;
-        Shape ps{1, max_len <= 32000, true, false, true};
-        ps.fixed_242 = true;
-        ps.reg_c = 8;
-        ps.probe = true;
-        HIPCHK(hipMemcpyAsync(d_order.p, probe_ids.data(), probe_ids.size() * sizeof(int32_t), hipMemcpyHostToDevice,
-                              ctx->stream), "hipMemcpyAsync (order)");
         double kms = 0;
-        int r = run_shape(ctx, io_of(b), ps, 512, d_order.p, (int32_t)probe_ids.size(), max_len, 0, &pa, &kms);
+        std::vector<int32_t> prog;
+        int r = run_probe(ctx, b, probe_ids, d_order.p, tmp, prog, &kms);
         if (r != TSU_OK) return r;
         kms_total += kms;
         ++launches;
-        std::vector<int32_t> prog((size_t)b->n, 0);
-        HIPCHK(hipMemcpy(tmp.data(), b->d_scores, (size_t)b->n * sizeof(int32_t), hipMemcpyDeviceToHost), "hipMemcpy D2H (scores)");
-        HIPCHK(hipMemcpy(prog.data(), d_prog.p, (size_t)b->n * sizeof(int32_t), hipMemcpyDeviceToHost), "hipMemcpy D2H (probe)");
         // the unfinished pairs, grouped by bucket and predicted capacity
         std::map<std::pair<int, int32_t>, std::vector<int32_t>> by;
         std::map<std::pair<int, int32_t>, double> by_cost;
@@ -1211,6 +1228,29 @@ extern "C" int tsu_tune_table(const tsu_ctx* ctx, tsu_tune_entry* entries, int32
     return TSU_OK;
 }
 
+// Replaces the context's table (one saved from tsu_tune_table, or a window
+// pinned per bucket).  Each entry names one bucket by its bound; waves_per_pair
+// is 0 or 1 (the automatic shape) or 2-16 (the LDS kernel with that many
+// waves per pair); capacity 0 lets the bucket be placed automatically.
+extern "C" int tsu_set_tune_table(tsu_ctx* ctx, const tsu_tune_entry* entries, int32_t n) {
+    if (!ctx || n < 0 || n > 64 || (n > 0 && !entries)) return fail(ctx, TSU_EINVAL, "tsu_set_tune_table: bad arguments");
+    const int nb = (int)(sizeof(kBucketBounds) / sizeof(kBucketBounds[0]));
+    std::vector<tsu_tune_entry> table;
+    for (int32_t k = 0; k < n; ++k) {
+        const tsu_tune_entry& e = entries[k];
+        const bool bucket = std::find(kBucketBounds, kBucketBounds + nb, e.bucket_bound) != kBucketBounds + nb;
+        const int w = e.waves_per_pair;
+        if (!bucket || (w != 0 && w != 1 && w != 2 && w != 4 && w != 8 && w != 16) || e.capacity < 0 ||
+            e.capacity > (1 << 26) || (w > 1 && e.capacity != 0) || tune_lookup_in(table, e.bucket_bound))
+            return fail(ctx, TSU_EINVAL, "tsu_set_tune_table: entry " + std::to_string(k) +
+                                             " (a bucket bound of the batched form, listed once; waves 0, 1, 2, 4, 8 "
+                                             "or 16; capacity 0..2^26, 0 with waves above 1)");
+        table.push_back(tsu_tune_entry{w, e.capacity, e.bucket_bound, 0});
+    }
+    ctx->tune = table;
+    return TSU_OK;
+}
+
 // ---------------------------------------------------------------------------
 // CIGAR form
 // ---------------------------------------------------------------------------
@@ -1347,13 +1387,15 @@ public:
 
 private:
     int setup();
+    int64_t bt_grid(const Shape& sh, const KArgs& a, int64_t n, int64_t per_wg) const;
     int launch_bt(const Shape& sh, KArgs& a, int32_t W, const std::vector<int32_t>& ids, int32_t max_len, int64_t recs,
                   int32_t step_cap, int64_t per_wg);
     int grow_runs(int64_t more);
     WidePlan plan_wide(const std::vector<int32_t>& ids) const;
     int settle_wide(const std::vector<int32_t>& ids);
     int launch_wide_async(WidePlan& wp, const std::vector<int32_t>& ids);
-    void plan_records(std::vector<RecordPlan>& plans, int64_t& runs_need);
+    int plan_records(std::vector<RecordPlan>& plans, int64_t& runs_need);
+    void plan_group(int32_t W, bool placed, std::vector<int32_t>& ids, std::vector<RecordPlan>& plans, int64_t& runs_need);
     int settle_plan(const RecordPlan& pl);
     int record_pass();
     int wide_pass();
@@ -1379,6 +1421,7 @@ private:
     int64_t budget = 0;
     std::vector<int32_t> sc2;      // scores the backtrace launches returned
     std::vector<uint8_t> done;
+    std::vector<uint8_t> scored;   // scored by the probe: records from the score-first path
     std::vector<int32_t> wide;     // pairs that overflowed the widest register window
     std::vector<int32_t> order;    // pairs of the score-first path
     WideAsync wide_async;          // declared after the buffers its launch uses: synchronised first
@@ -1405,18 +1448,25 @@ int CigarCall::setup() {
     budget = ctx->cigar_budget > 0 ? ctx->cigar_budget : (int64_t)(0.5 * (double)free_b);
     sc2.assign((size_t)b->n, 0);
     done.assign((size_t)b->n, 0);
+    scored.assign((size_t)b->n, 0);
     return TSU_OK;
 }
 
 // One backtrace launch of `ids` with `recs` record words and `step_cap`
 // scores of state per workgroup (per_wg bytes in all).
+// Workgroups of a backtrace launch of n pairs: every pair that fits on the
+// CUs at once, within the budget (0: the shape has no occupancy).
+int64_t CigarCall::bt_grid(const Shape& sh, const KArgs& a, int64_t n, int64_t per_wg) const {
+    int per_cu = 0;
+    if (tsu::occupancy(sh, tsu::lds_bytes(sh, a), &per_cu) != 0 || per_cu <= 0) return 0;
+    const int64_t grid = std::min<int64_t>((int64_t)per_cu * ctx->cu_count, n);
+    return std::max<int64_t>(1, std::min<int64_t>(grid, budget / std::max<int64_t>(per_wg, 1)));
+}
+
 int CigarCall::launch_bt(const Shape& sh, KArgs& a, int32_t W, const std::vector<int32_t>& ids, int32_t max_len,
                          int64_t recs, int32_t step_cap, int64_t per_wg) {
-    int per_cu = 0;
-    int rr = tsu::occupancy(sh, tsu::lds_bytes(sh, a), &per_cu);
-    if (rr != 0 || per_cu <= 0) return fail(ctx, TSU_EINTERNAL, "backtrace shape has no occupancy");
-    int64_t grid = std::min<int64_t>((int64_t)per_cu * ctx->cu_count, (int64_t)ids.size());
-    grid = std::max<int64_t>(1, std::min<int64_t>(grid, budget / std::max<int64_t>(per_wg, 1)));
+    const int64_t grid = bt_grid(sh, a, (int64_t)ids.size(), per_wg);
+    if (grid <= 0) return fail(ctx, TSU_EINTERNAL, "backtrace shape has no occupancy");
     cg->state_bytes = std::max(cg->state_bytes, per_wg * grid);
     if (sh.ck) ++cg->checkpoint_launches; else ++cg->record_launches;
     cg->kernels |= tsu_host::kernel_bit(sh);
@@ -1543,78 +1593,168 @@ int CigarCall::launch_wide_async(WidePlan& wp, const std::vector<int32_t>& ids) 
 // for the highest score the window (or the lengths) allow.  A pair whose
 // window overflows, or whose score passes that step bound, is left to the
 // paths after.
-void CigarCall::plan_records(std::vector<RecordPlan>& plans, int64_t& runs_need) {
+//
+// Buckets whose first window needs several waves are probed first, as in the
+// batched form: a pair the probe finished has its score, and its records come
+// from the score-first path's launches at its exact capacity; the others are
+// grouped by bucket and predicted window, so a low-error pair of 10 kbp
+// records in 4,096 diagonals (four waves, a quarter of the record state)
+// rather than its bucket's 16,384.
+int CigarCall::plan_records(std::vector<RecordPlan>& plans, int64_t& runs_need) {
     const int nb = tsu_host::bucket_count();
     std::vector<std::vector<int32_t>> bk((size_t)nb);
     for (int32_t i = 0; i < b->n; ++i)
         bk[(size_t)tsu_host::bucket_of(std::max(b->pairs[(size_t)i].p_len, b->pairs[(size_t)i].t_len))].push_back(i);
     runs_need = 0;
+    struct Group {
+        int32_t W;    // first capacity
+        bool placed;  // by the probe
+        std::vector<int32_t> ids;
+    };
+    std::vector<Group> groups;
+    std::vector<int32_t> probe_ids;
+    std::vector<int> probed((size_t)nb, 0);
     for (int bi = nb - 1; bi >= 0; --bi) {
         std::vector<int32_t>& ids = bk[(size_t)bi];
         if (ids.empty()) continue;
         int32_t max_len = 0;
-        int64_t s_len = 0;
-        for (int32_t i : ids) {
-            const tsu_pair& pr = b->pairs[(size_t)i];
-            max_len = std::max(max_len, std::max(pr.p_len, pr.t_len));
-            s_len = std::max(s_len, score_bound(ctx, pr.p_len, pr.t_len));
-        }
+        for (int32_t i : ids) max_len = std::max(max_len, std::max(b->pairs[(size_t)i].p_len, b->pairs[(size_t)i].t_len));
         const int32_t bound = tsu_host::bucket_bound(bi);
         const tsu_tune_entry* te = tune_lookup(ctx, bound);
         if (te && te->waves_per_pair > 1) continue;  // tuned for the LDS kernel
         int32_t W = (te && te->capacity > 0) ? te->capacity : initial_capacity(ctx, std::min(bound, max_len));
         W = std::min<int32_t>(W, capacity_full(max_len, max_len));
-        Shape sh{default_waves(W), max_len <= 32000, true, true, true};
-        sh.fixed_242 = true;
-        // the kernel's own window: 64 diagonals per chunk
-        int32_t win = 0;
-        const int c1 = bt_reg_chunks(ctx, W);
-        if (c1 > 0 && max_len <= 16000) {
-            sh.nw = 1;
-            sh.reg_c = c1;
-            sh.lring = true;
-            win = 64 * c1;
-        } else if (int nwl = tsu_host::ml_waves(ctx, W, max_len)) {
-            sh.nw = nwl;
-            sh.reg_c = 16;
-            win = 1024 * nwl;
-        } else {
-            continue;  // the LDS kernel records per score's width: sized from scores
+        if (tsu_host::probe_bucket(ctx, W, te ? te->capacity : 0, te ? te->waves_per_pair : 0)) {
+            probed[(size_t)bi] = 1;
+            probe_ids.insert(probe_ids.end(), ids.begin(), ids.end());
+            continue;
         }
-        // a window holding every diagonal of the longest pair never overflows
-        int64_t s_cap = win >= capacity_full(max_len, max_len) ? s_len : std::min(s_len, score_for_capacity(ctx, win));
-        if (s_cap < 0 || (sh.nw == 1 && s_cap > 16000)) continue;
-        KArgs a = base_args(ctx);
-        a.W = W;
-        a.seq_words = seq_words_for(max_len);
-        int per_cu = 0;
-        if (tsu::occupancy(sh, tsu::lds_bytes(sh, a), &per_cu) != 0 || per_cu <= 0) continue;
-        // the launch's state (records of every resident pair) stays within
-        // the budget: fewer steps if needed, unless that leaves less than a
-        // third of the window's scores
-        const int64_t resident = std::min<int64_t>((int64_t)per_cu * ctx->cu_count, (int64_t)ids.size());
-        const int64_t step_bytes = (int64_t)win * 4 + 12;
-        const int64_t fit = budget / (resident * step_bytes) - 2;
-        const int64_t s_win = s_cap;
-        if (fit < s_cap) {
-            if (fit * 3 < s_cap) continue;
-            s_cap = fit;
+        groups.push_back(Group{W, false, std::move(ids)});
+    }
+    if (!probe_ids.empty()) {
+        std::vector<int32_t> sc, prog;
+        double kms = 0;
+        int r = run_probe(ctx, b, probe_ids, d_order.p, sc, prog, &kms);
+        if (r != TSU_OK) return r;
+        kms_total += kms;
+        ++launches;
+        std::map<std::pair<int, int32_t>, std::vector<int32_t>> by;
+        for (int bi = 0; bi < nb; ++bi) {
+            if (!probed[(size_t)bi]) continue;
+            for (int32_t i : bk[(size_t)bi]) {
+                const tsu_pair& pr = b->pairs[(size_t)i];
+                const int32_t v = sc[(size_t)i];
+                if (v >= 0) { scores[i] = v; scored[(size_t)i] = 1; continue; }
+                const int32_t ml = std::max(pr.p_len, pr.t_len);
+                // a pair that overflowed the probe's window starts at its bucket's capacity
+                const int32_t W = v == tsu::kProbeStopped
+                                      ? tsu_host::probe_capacity(ctx, pr, prog[(size_t)i])
+                                      : initial_capacity(ctx, std::min(tsu_host::bucket_bound(bi), ml));
+                by[{bi, W}].push_back(i);
+            }
         }
-        const int32_t step_cap = (int32_t)(s_cap + 2);
-        const int64_t recs = (int64_t)step_cap * win;
-        if (sh.nw > 1) sh.dbl = max_len <= tsu::kMlDblMaxLen && s_cap <= tsu::kMlDblMaxScore;
-        const int64_t per_wg = recs * 4 + (int64_t)step_cap * 12;
-        // pairs of the widest window may also be walked by the launch for
-        // wider pairs, beside later launches: room for their whole bound
+        for (auto& kv : by) groups.push_back(Group{kv.first.second, true, std::move(kv.second)});
+    }
+    // the groups of the 16-wave window share one launch: a pair fills a CU
+    // in either, and one launch leaves one tail instead of one per bucket
+    {
+        Group merged{0, false, {}};
+        std::vector<Group> rest;
+        for (Group& g : groups) {
+            int32_t ml = 0;
+            for (int32_t i : g.ids) ml = std::max(ml, std::max(b->pairs[(size_t)i].p_len, b->pairs[(size_t)i].t_len));
+            if (tsu_host::ml_waves(ctx, std::min<int32_t>(g.W, capacity_full(ml, ml)), ml) == 16) {
+                merged.W = std::max(merged.W, g.W);
+                merged.ids.insert(merged.ids.end(), g.ids.begin(), g.ids.end());
+            } else {
+                rest.push_back(std::move(g));
+            }
+        }
+        if (!merged.ids.empty()) rest.push_back(std::move(merged));
+        groups.swap(rest);
+    }
+    for (Group& g : groups) plan_group(g.W, g.placed, g.ids, plans, runs_need);
+    // widest windows first, longest pairs first among equals
+    std::stable_sort(plans.begin(), plans.end(), [](const RecordPlan& x, const RecordPlan& y) {
+        return x.win != y.win ? x.win > y.win : x.max_len > y.max_len;
+    });
+    return TSU_OK;
+}
+
+// The record launch of one group at first capacity W, when a register kernel
+// records it (placed: W is the probe's prediction).
+void CigarCall::plan_group(int32_t W, bool placed, std::vector<int32_t>& ids, std::vector<RecordPlan>& plans,
+                           int64_t& runs_need) {
+    int32_t max_len = 0;
+    int64_t s_len = 0;
+    for (int32_t i : ids) {
+        const tsu_pair& pr = b->pairs[(size_t)i];
+        max_len = std::max(max_len, std::max(pr.p_len, pr.t_len));
+        s_len = std::max(s_len, score_bound(ctx, pr.p_len, pr.t_len));
+    }
+    W = std::min<int32_t>(W, capacity_full(max_len, max_len));
+    // the single-wave recorder holds sequences of up to 16,000 bases:
+    // longer pairs of a narrow window take the 16-chunk kernel's narrowest
+    if (max_len > 16000 && W <= kMlMinCapacity) W = std::min<int32_t>(4 * 1024, capacity_full(max_len, max_len));
+    Shape sh{default_waves(W), max_len <= 32000, true, true, true};
+    sh.fixed_242 = true;
+    // the kernel's own window: 64 diagonals per chunk
+    int32_t win = 0;
+    const int c1 = bt_reg_chunks(ctx, W);
+    if (c1 > 0 && max_len <= 16000) {
+        sh.nw = 1;
+        sh.reg_c = c1;
+        sh.lring = true;
+        win = 64 * c1;
+    } else if (int nwl = tsu_host::ml_waves(ctx, W, max_len)) {
+        sh.nw = nwl;
+        sh.reg_c = 16;
+        win = 1024 * nwl;
+    } else if (placed && W > 16 * 1024) {
+        // predicted beyond every register window: straight to the launch
+        // over the whole diagonal range (room in the run pool for their bound)
         for (int32_t i : ids) {
             const tsu_pair& pr = b->pairs[(size_t)i];
-            const int64_t sb = score_bound(ctx, pr.p_len, pr.t_len);
-            runs_need += 2 * ((win >= 16 * 1024 ? sb : std::min(s_cap + 1, sb)) / unit) + 2;
+            runs_need += 2 * (score_bound(ctx, pr.p_len, pr.t_len) / unit) + 2;
+            wide.push_back(i);
         }
-        tsu_host::order_longest_first(ids, b->pairs.data());
-        plans.push_back(RecordPlan{std::move(ids), sh, a, W, max_len, step_cap, win, recs, per_wg,
-                                   s_cap == s_win && s_win < s_len});
+        return;
+    } else {
+        return;  // the LDS kernel records per score's width: sized from scores
     }
+    // a window holding every diagonal of the longest pair never overflows
+    int64_t s_cap = win >= capacity_full(max_len, max_len) ? s_len : std::min(s_len, score_for_capacity(ctx, win));
+    if (s_cap < 0 || (sh.nw == 1 && s_cap > 16000)) return;
+    KArgs a = base_args(ctx);
+    a.W = W;
+    a.seq_words = seq_words_for(max_len);
+    int per_cu = 0;
+    if (tsu::occupancy(sh, tsu::lds_bytes(sh, a), &per_cu) != 0 || per_cu <= 0) return;
+    // the launch's state (records of every resident pair) stays within
+    // the budget: fewer steps if needed, unless that leaves less than a
+    // third of the window's scores
+    const int64_t resident = std::min<int64_t>((int64_t)per_cu * ctx->cu_count, (int64_t)ids.size());
+    const int64_t step_bytes = (int64_t)win * 4 + 12;
+    const int64_t fit = budget / (resident * step_bytes) - 2;
+    const int64_t s_win = s_cap;
+    if (fit < s_cap) {
+        if (fit * 3 < s_cap) return;
+        s_cap = fit;
+    }
+    const int32_t step_cap = (int32_t)(s_cap + 2);
+    const int64_t recs = (int64_t)step_cap * win;
+    if (sh.nw > 1) sh.dbl = max_len <= tsu::kMlDblMaxLen && s_cap <= tsu::kMlDblMaxScore;
+    const int64_t per_wg = recs * 4 + (int64_t)step_cap * 12;
+    // pairs of the widest window may also be walked by the launch for
+    // wider pairs, beside later launches: room for their whole bound
+    for (int32_t i : ids) {
+        const tsu_pair& pr = b->pairs[(size_t)i];
+        const int64_t sb = score_bound(ctx, pr.p_len, pr.t_len);
+        runs_need += 2 * ((win >= 16 * 1024 ? sb : std::min(s_cap + 1, sb)) / unit) + 2;
+    }
+    tsu_host::order_longest_first(ids, b->pairs.data());
+    plans.push_back(RecordPlan{std::move(ids), sh, a, W, max_len, step_cap, win, recs, per_wg,
+                               s_cap == s_win && s_win < s_len});
 }
 
 int CigarCall::settle_plan(const RecordPlan& pl) {
@@ -1635,8 +1775,9 @@ int CigarCall::settle_plan(const RecordPlan& pl) {
 int CigarCall::record_pass() {
     std::vector<RecordPlan> plans;
     int64_t runs_need = 0;
-    plan_records(plans, runs_need);
-    if (plans.empty()) return TSU_OK;
+    int r0 = plan_records(plans, runs_need);
+    if (r0 != TSU_OK) return r0;
+    if (plans.empty() && wide.empty()) return TSU_OK;
     runs_cap = std::max<int64_t>(runs_need, 1);
     HIPCHK(d_runs.alloc((size_t)runs_cap), "hipMalloc (runs)");
     size_t k = 0;
@@ -1653,6 +1794,20 @@ int CigarCall::record_pass() {
         }
     }
     if (!wide.empty() && k < plans.size()) {
+        // the state of the launches after it is reserved before the wide
+        // launch starts: freeing a buffer to grow it waits for the whole
+        // device, so growing one later would stall them until it ends
+        size_t need[3] = {0, 0, 0};
+        for (size_t j = k; j < plans.size(); ++j) {
+            const int64_t grid = bt_grid(plans[j].sh, plans[j].a, (int64_t)plans[j].ids.size(), plans[j].per_wg);
+            need[0] = std::max(need[0], (size_t)(plans[j].recs * grid) * 4);
+            need[1] = std::max(need[1], (size_t)((int64_t)plans[j].step_cap * grid) * 4);
+            need[2] = std::max(need[2], (size_t)((int64_t)plans[j].step_cap * grid) * 8);
+        }
+        for (int i = 0; i < 3; ++i) {
+            void* p = nullptr;
+            HIPCHK(bt_reserve(ctx, i, need[i], &p), "hipMalloc (records)");
+        }
         WidePlan wp = plan_wide(wide);
         if (wp.per_wg * (int64_t)wide.size() <= budget) {
             int r = launch_wide_async(wp, wide);
@@ -1700,10 +1855,14 @@ int CigarCall::wide_pass() {
 // 3) The other pairs: scores first (batched form), which size the backtrace
 // state exactly, then launches grouped by score.
 int CigarCall::score_first_pass() {
+    std::vector<int32_t> need;  // pairs without a score yet
     for (int32_t i = 0; i < b->n; ++i)
-        if (!done[(size_t)i]) order.push_back(i);
+        if (!done[(size_t)i]) {
+            order.push_back(i);
+            if (!scored[(size_t)i]) need.push_back(i);
+        }
     if (order.empty()) return TSU_OK;
-    int r = batched_core(ctx, b, order, scores, t, true);
+    int r = need.empty() ? TSU_OK : batched_core(ctx, b, need, scores, t, true);
     if (r != TSU_OK) return r;
     for (int32_t i : order)
         if (scores[i] < 0) return fail(ctx, scores[i], "pair " + std::to_string(i) + ": " + tsu_strerror(scores[i]));
