@@ -157,10 +157,14 @@ void tsu_dataset_free(tsu_dataset* ds);
 
 /* ---- device-resident batches (wfa_gpu.cu:96-111: cudaMalloc + H2D) ------- */
 
-/* Both sequences of a pair are staged in LDS while it is aligned, so a
- * sequence may hold at most about 320,000 bases (the exact bound follows from
- * the device's LDS size); batches (and server requests) with a longer one
- * are refused with TSU_EINVAL. */
+/* Sequences of up to about 327,000 bases (the exact bound follows from the
+ * device's LDS size) are staged in LDS while their pair is aligned; a pair
+ * with a longer one runs on the LDS kernel reading its packed words in place
+ * from HBM, as the reference reads its sequences from global memory
+ * (wfa_gpu.cu:196-197,266-276), in every form.  A sequence may hold at most
+ * 2^28 bases; 2-piece contexts stage both sequences in LDS and refuse
+ * longer ones.  Batches (and server requests) beyond these limits are refused
+ * with TSU_EINVAL. */
 
 /* Packs pairs [first, first+count) of ds and copies them to HBM. */
 int tsu_batch_from_dataset(tsu_ctx* ctx, const tsu_dataset* ds, int64_t first, int64_t count,

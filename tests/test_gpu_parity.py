@@ -157,14 +157,19 @@ def test_single_wave_kernels_short_and_long_sequences(aligner, tsu, oracle):
         assert (got == want).all(), name
 
 
-def test_sequence_longer_than_lds_is_refused(aligner, tsu):
-    """Both sequences of a pair are staged in LDS; longer ones are refused
-    when the batch is made, with TSU_EINVAL and a message."""
+def test_sequence_limit_2p_is_lds(tsu):
+    """The 2-piece kernel stages both sequences in LDS: a 2-piece context
+    refuses longer ones when the batch is made, with TSU_EINVAL and a message
+    (1-piece contexts read them from HBM: test_gpu_long_sequences.py)."""
     huge = tsu.Dataset.generate(1, seed=5, len_min=400_000, err_min=0.0)
-    say("refuse 400 kbp")
-    with pytest.raises(tsu.TsuError) as ei:
-        aligner.batch(huge)
-    assert ei.value.status == tsu.TSU_EINVAL and "LDS" in str(ei.value)
+    a = tsu.Aligner(4, 6, 2, gap_opening2=24, gap_extension2=1)
+    try:
+        say("2-piece: refuse 400 kbp")
+        with pytest.raises(tsu.TsuError) as ei:
+            a.batch(huge)
+        assert ei.value.status == tsu.TSU_EINVAL and "LDS" in str(ei.value)
+    finally:
+        a.close()
 
 
 def test_fixed_layout_entry_point(aligner, tsu, oracle):

@@ -81,11 +81,12 @@ def test_serve_poll_budget_and_empty(aligner, tsu):
     with pytest.raises(tsu.TsuError) as ei:
         srv.submit(big)
     assert ei.value.status == tsu.TSU_EBUDGET
-    # a sequence longer than the kernels stage in LDS is refused as invalid
+    # a pair of 400 kbp: its flight reserves the ring of every widening up to
+    # the full 800,003 diagonals (32 MB), more than the whole budget
     huge = tsu.Dataset.generate(1, seed=4, len_min=400_000, err_min=0.0).pairs()
     with pytest.raises(tsu.TsuError) as ei:
         srv.submit(huge)
-    assert ei.value.status == tsu.TSU_EINVAL
+    assert ei.value.status == tsu.TSU_EBUDGET
     assert srv.close() == tsu.TSU_OK
 
 

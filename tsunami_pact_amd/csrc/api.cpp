@@ -233,10 +233,10 @@ static int batch_upload(tsu_ctx* ctx, const uint32_t* words, int64_t n_words, co
         if (p.p_len < 0 || p.t_len < 0 || p.p_off < 0 || p.t_off < 0 ||
             p.p_off + tsu_packed_words(p.p_len) > n_words || p.t_off + tsu_packed_words(p.t_len) > n_words)
             return fail(ctx, TSU_EINVAL, "pair " + std::to_string(i) + " lies outside the packed arena");
-        if (p.p_len > tsu_host::max_seq_len(ctx) || p.t_len > tsu_host::max_seq_len(ctx))
+        if (p.p_len > tsu_host::seq_len_limit(ctx) || p.t_len > tsu_host::seq_len_limit(ctx))
             return fail(ctx, TSU_EINVAL, "pair " + std::to_string(i) + ": a sequence is longer than " +
-                                             std::to_string(tsu_host::max_seq_len(ctx)) +
-                                             " bases (both sequences of a pair are staged in LDS)");
+                                             std::to_string(tsu_host::seq_len_limit(ctx)) + " bases" +
+                                             (ctx->two_piece ? " (the 2-piece kernel stages both sequences in LDS)" : ""));
     }
     tsu_batch* b = new (std::nothrow) tsu_batch();
     if (!b) return fail(ctx, TSU_ENOMEM, "host allocation");
@@ -323,14 +323,22 @@ KArgs tsu_host::base_args(const tsu_ctx* ctx) {
     return a;
 }
 
-int32_t tsu_host::seq_words_for(int32_t max_len) { return (int32_t)((max_len + 15) / 16 + 2); }
+int32_t tsu_host::seq_words_for(const tsu_ctx* ctx, int32_t max_len) {
+    return max_len > max_seq_len(ctx) ? 0 : (int32_t)((max_len + 15) / 16 + 2);
+}
 
-// Longest sequence the kernels take: both sequences of a pair are staged in
-// LDS, next to the LDS kernel's slot limits and misc words.
+void tsu_host::place_sequences(const tsu_ctx* ctx, Shape& sh, int32_t max_len) {
+    sh.gseq = sh.reg_c == 0 && !sh.two_piece && max_len > max_seq_len(ctx);
+}
+
+// Longest sequence staged in LDS: both sequences of a pair, next to the LDS
+// kernel's slot limits and misc words.
 int32_t tsu_host::max_seq_len(const tsu_ctx* ctx) {
     const int64_t words = ctx->max_lds / 4 - 2 * (ctx->rm + 2 * ctx->ri + 1) - 16 - 4;
     return (int32_t)std::max<int64_t>(0, (words / 2 - 2) * 16);
 }
+
+int32_t tsu_host::seq_len_limit(const tsu_ctx* ctx) { return ctx->two_piece ? max_seq_len(ctx) : kMaxSeqLen; }
 
 using tsu_host::base_args;
 using tsu_host::seq_words_for;
@@ -354,7 +362,7 @@ int32_t tsu_host::capacity_max(const tsu_ctx* ctx, int32_t max_len) {
     Shape s2{1, max_len <= 32000, true, false, true};
     s2.two_piece = true;
     KArgs a = base_args(ctx);
-    a.seq_words = seq_words_for(max_len);
+    a.seq_words = seq_words_for(ctx, max_len);
     int32_t lo = 0, hi = full;  // lo fits (or is 0), hi the highest candidate
     while (lo < hi) {
         const int32_t mid = lo + (hi - lo + 1) / 2;
@@ -390,6 +398,13 @@ static int launch_shape(tsu_ctx* ctx, const DevIO& io, Shape sh, int32_t W, cons
                         int32_t max_len, int64_t grid_cap, KArgs* extra, hipStream_t stream, int32_t* queue,
                         hipEvent_t e0, hipEvent_t e1, DevBuf<uint8_t>& ring, bool* launched) {
     *launched = false;
+    if (max_len > tsu_host::max_seq_len(ctx)) {
+        // only the LDS kernel reads sequences in place
+        if (sh.reg_c > 0 || sh.two_piece)
+            return fail(ctx, TSU_EINTERNAL, "a register or 2-piece launch was given sequences longer than LDS holds");
+        sh.gseq = true;
+        sh.off16 = false;
+    }
     KArgs a = extra ? *extra : base_args(ctx);
     a.x = ctx->xs; a.o = ctx->os; a.e = ctx->es; a.g = ctx->g; a.rm = ctx->rm; a.ri = ctx->ri;
     a.words = io.words;
@@ -398,7 +413,7 @@ static int launch_shape(tsu_ctx* ctx, const DevIO& io, Shape sh, int32_t W, cons
     a.n = n_entries;
     a.scores = io.scores;
     a.W = W;
-    a.seq_words = seq_words_for(max_len);
+    a.seq_words = seq_words_for(ctx, max_len);
     a.queue = queue;
     const int64_t lds = tsu::lds_bytes(sh, a);
     if (lds > ctx->max_lds) return fail(ctx, TSU_EINVAL, "shape does not fit in LDS");
@@ -482,6 +497,7 @@ int tsu_host::ml_waves(const tsu_ctx* ctx, int32_t W, int32_t max_len) {
 // 16 chunks (older M scores in LDS) for up to 16384 (and the two sequences
 // fit in LDS).
 static void use_register_kernel(const tsu_ctx* ctx, int32_t W, int32_t max_len, Shape& sh) {
+    if (max_len > tsu_host::max_seq_len(ctx)) return;  // the register kernels stage both sequences in LDS
     Shape r = sh;
     if (int c = tsu_host::reg_chunks(ctx, W)) {
         r.reg_c = c;
@@ -503,7 +519,7 @@ static void use_register_kernel(const tsu_ctx* ctx, int32_t W, int32_t max_len, 
     }
     KArgs a = base_args(ctx);
     a.W = W;
-    a.seq_words = seq_words_for(max_len);
+    a.seq_words = seq_words_for(ctx, max_len);
     if (tsu::lds_bytes(r, a) <= ctx->max_lds) sh = r;
 }
 
@@ -513,16 +529,17 @@ static Shape plain_shape(const tsu_ctx* ctx, int32_t W, int32_t max_len, bool* f
         s2.two_piece = true;
         KArgs a = base_args(ctx);
         a.W = W;
-        a.seq_words = seq_words_for(max_len);
+        a.seq_words = seq_words_for(ctx, max_len);
         *fits = tsu::lds_bytes(s2, a) <= ctx->max_lds;
         return s2;
     }
     Shape sh{default_waves(W), max_len <= 32000, true, false, false};
     sh.fixed_242 = tsu_host::fixed_242(ctx);
     use_register_kernel(ctx, W, max_len, sh);
+    tsu_host::place_sequences(ctx, sh, max_len);
     KArgs a = base_args(ctx);
     a.W = W;
-    a.seq_words = seq_words_for(max_len);
+    a.seq_words = seq_words_for(ctx, max_len);
     *fits = tsu::lds_bytes(sh, a) <= ctx->max_lds;
     return sh;
 }
@@ -569,9 +586,10 @@ extern "C" int tsu_align_plain(tsu_ctx* ctx, tsu_batch* b, int32_t wf_length, in
         if (!again.empty()) {
             Shape s2{default_waves(W), b->max_len <= 32000, true, false, true};
             s2.fixed_242 = sh.fixed_242;
+            tsu_host::place_sequences(ctx, s2, b->max_len);
             KArgs a2 = base_args(ctx);
             a2.W = W;
-            a2.seq_words = seq_words_for(b->max_len);
+            a2.seq_words = seq_words_for(ctx, b->max_len);
             if (tsu::lds_bytes(s2, a2) > ctx->max_lds) { s2.lds_ring = false; s2.nw = 16; }
             DevBuf<int32_t> d_order;
             HIPCHK(d_order.alloc(again.size()), "hipMalloc (order)");
@@ -690,9 +708,10 @@ Shape tsu_host::batched_shape(const tsu_ctx* ctx, int32_t W, int32_t max_len, in
         use_register_kernel(ctx, W, max_len, sh);
         if (sh.reg_c) return sh;
     }
+    tsu_host::place_sequences(ctx, sh, max_len);
     KArgs a = base_args(ctx);
     a.W = W;
-    a.seq_words = seq_words_for(max_len);
+    a.seq_words = seq_words_for(ctx, max_len);
     if (tsu::lds_bytes(sh, a) > ctx->max_lds) {
         sh.lds_ring = false;
         if (nw_hint <= 0) sh.nw = 16;
@@ -847,7 +866,7 @@ static int batched_core(tsu_ctx* ctx, tsu_batch* b, const std::vector<int32_t>& 
         int32_t W = (te && te->capacity > 0) ? te->capacity : initial_capacity(ctx, std::min(bk.bound, max_len));
         W = std::min<int32_t>(W, tsu_host::capacity_max(ctx, max_len));
         const int nw_hint = te ? te->waves_per_pair : 0;
-        if (tsu_host::probe_bucket(ctx, W, te ? te->capacity : 0, nw_hint)) {
+        if (tsu_host::probe_bucket(ctx, W, te ? te->capacity : 0, nw_hint, max_len)) {
             probed_bucket[(size_t)bi] = 1;
             probe_ids.insert(probe_ids.end(), bk.idx.begin(), bk.idx.end());
             continue;
@@ -1066,9 +1085,8 @@ extern "C" int tsu_align_batched_noalloc(tsu_ctx* ctx, const uint32_t* d_words, 
         max_all = std::max(max_all, std::max(p.p_len, p.t_len));
         cells += (double)p.p_len * (double)p.t_len;
     }
-    if (max_all > tsu_host::max_seq_len(ctx))
-        return fail(ctx, TSU_EINVAL, "a sequence is longer than " + std::to_string(tsu_host::max_seq_len(ctx)) +
-                                         " bases (both sequences of a pair are staged in LDS)");
+    if (max_all > tsu_host::seq_len_limit(ctx))
+        return fail(ctx, TSU_EINVAL, "a sequence is longer than " + std::to_string(tsu_host::seq_len_limit(ctx)) + " bases");
     HIPCHK(hipSetDevice(ctx->device), "hipSetDevice");
     if (t) std::memset(t, 0, sizeof(*t));
     const double w0 = now_ms();
@@ -1532,9 +1550,10 @@ WidePlan CigarCall::plan_wide(const std::vector<int32_t>& ids) const {
         wp.recs += std::min<int64_t>(wp.W, 2 * (std::max<int64_t>(0, s - ctx->os) / ctx->es + 2) + 1);
     wp.a = base_args(ctx);
     wp.a.W = wp.W;
-    wp.a.seq_words = seq_words_for(wp.max_len);
+    wp.a.seq_words = seq_words_for(ctx, wp.max_len);
     wp.sh = Shape{default_waves(wp.W), wp.max_len <= 32000, true, true, true};
     wp.sh.fixed_242 = tsu_host::fixed_242(ctx);
+    tsu_host::place_sequences(ctx, wp.sh, wp.max_len);
     if (tsu::lds_bytes(wp.sh, wp.a) > ctx->max_lds) { wp.sh.lds_ring = false; wp.sh.nw = 16; }
     wp.per_wg = wp.recs * 4 + (int64_t)wp.step_cap * 12 +
                 (wp.sh.lds_ring ? 0 : tsu::ring_bytes(wp.sh, wp.W, ctx->rm, ctx->ri));
@@ -1624,7 +1643,7 @@ int CigarCall::plan_records(std::vector<RecordPlan>& plans, int64_t& runs_need) 
         if (te && te->waves_per_pair > 1) continue;  // tuned for the LDS kernel
         int32_t W = (te && te->capacity > 0) ? te->capacity : initial_capacity(ctx, std::min(bound, max_len));
         W = std::min<int32_t>(W, capacity_full(max_len, max_len));
-        if (tsu_host::probe_bucket(ctx, W, te ? te->capacity : 0, te ? te->waves_per_pair : 0)) {
+        if (tsu_host::probe_bucket(ctx, W, te ? te->capacity : 0, te ? te->waves_per_pair : 0, max_len)) {
             probed[(size_t)bi] = 1;
             probe_ids.insert(probe_ids.end(), ids.begin(), ids.end());
             continue;
@@ -1727,7 +1746,7 @@ void CigarCall::plan_group(int32_t W, bool placed, std::vector<int32_t>& ids, st
     if (s_cap < 0 || (sh.nw == 1 && s_cap > 16000)) return;
     KArgs a = base_args(ctx);
     a.W = W;
-    a.seq_words = seq_words_for(max_len);
+    a.seq_words = seq_words_for(ctx, max_len);
     int per_cu = 0;
     if (tsu::occupancy(sh, tsu::lds_bytes(sh, a), &per_cu) != 0 || per_cu <= 0) return;
     // the launch's state (records of every resident pair) stays within
@@ -1897,7 +1916,7 @@ int CigarCall::score_first_pass() {
             if (end - pos >= 65536) break;
         }
         a.W = W;
-        a.seq_words = seq_words_for(max_len);
+        a.seq_words = seq_words_for(ctx, max_len);
         const int64_t group_n = (int64_t)(end - pos);
         Shape sh{default_waves(W), max_len <= 32000, true, true, true};
         sh.fixed_242 = tsu_host::fixed_242(ctx);
@@ -1922,6 +1941,7 @@ int CigarCall::score_first_pass() {
             for (int64_t s = 0; s <= s0; ++s)
                 recs += std::min<int64_t>(W, 2 * (std::max<int64_t>(0, s - ctx->os) / ctx->es + 2) + 1);
             recs += 64;
+            tsu_host::place_sequences(ctx, sh, max_len);
             if (tsu::lds_bytes(sh, a) > ctx->max_lds) { sh.lds_ring = false; sh.nw = 16; }
         }
         int64_t per_wg = recs * 4 + (int64_t)steps * 12 + (sh.lds_ring || sh.reg_c ? 0 : tsu::ring_bytes(sh, W, ctx->rm, ctx->ri));
@@ -1933,6 +1953,7 @@ int CigarCall::score_first_pass() {
             sh = Shape{std::max(4, default_waves(W)), max_len <= 32000, true, true, true};
             sh.fixed_242 = tsu_host::fixed_242(ctx);
             sh.ck = true;
+            tsu_host::place_sequences(ctx, sh, max_len);
             if (tsu::lds_bytes(sh, a) > ctx->max_lds) { sh.lds_ring = false; sh.nw = 16; }
             const int32_t nslot = ctx->rm + 2 * ctx->ri;
             const int32_t osz = sh.off16 ? 2 : 4;
@@ -1988,7 +2009,7 @@ int CigarCall::two_piece_pass() {
         }
         KArgs a = base_args(ctx);
         a.W = W;
-        a.seq_words = seq_words_for(max_len);
+        a.seq_words = seq_words_for(ctx, max_len);
         Shape sh{1, max_len <= 32000, true, true, true};  // records, queue-fed
         sh.two_piece = true;
         if (tsu::lds_bytes(sh, a) > ctx->max_lds)

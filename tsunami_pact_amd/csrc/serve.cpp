@@ -196,7 +196,7 @@ bool probed_bucket(const tsu_ctx* ctx, int bucket, int32_t max_len) {
     const int32_t bound = tsu_host::bucket_bound(bucket);
     int waves = 0;
     const int32_t W = bucket_capacity(ctx, bucket, max_len, &waves);
-    return tsu_host::probe_bucket(ctx, W, tsu_host::tuned_capacity(ctx, bound), waves);
+    return tsu_host::probe_bucket(ctx, W, tsu_host::tuned_capacity(ctx, bound), waves, max_len);
 }
 
 // Plans the flight's one launch (its pairs share one queue: one length
@@ -218,7 +218,7 @@ int64_t plan_flight(tsu_server* s, Slot& sl, int32_t W_entry, int* status) {
     const int32_t W0 = bucket_capacity(ctx, bucket, lp.max_len, &waves);
     const int32_t full = tsu_host::capacity_max(ctx, lp.max_len);
     KArgs a = tsu_host::base_args(ctx);
-    a.seq_words = tsu_host::seq_words_for(lp.max_len);
+    a.seq_words = tsu_host::seq_words_for(ctx, lp.max_len);
     if (sl.probing) {
         lp.probe = true;
         lp.W = 512;
@@ -267,7 +267,7 @@ int launch_plan(tsu_server* s, Slot& sl) {
         a.n = lp.count;
         a.scores = sl.d_scores;
         a.W = lp.W;
-        a.seq_words = tsu_host::seq_words_for(lp.max_len);
+        a.seq_words = tsu_host::seq_words_for(ctx, lp.max_len);
         a.queue = sl.d_queue + li;
         if (lp.probe) {
             a.probe_cap = tsu_host::kProbeCap;
@@ -458,7 +458,7 @@ void retire(tsu_server* s, Slot& sl) {
         lp.sh = tsu_host::batched_shape(ctx, lp.W, max_len, 0);
         KArgs a = tsu_host::base_args(ctx);
         a.W = lp.W;
-        a.seq_words = tsu_host::seq_words_for(max_len);
+        a.seq_words = tsu_host::seq_words_for(ctx, max_len);
         int st = 0;
         lp.grid = std::min<int64_t>(tsu_host::resident_grid(ctx, lp.sh, a, &st), lp.count);
         // the flight reserved the ring of every widening up to full capacity
@@ -591,7 +591,7 @@ extern "C" int tsu_serve_submit(tsu_server* s, const char* const* patterns, cons
         r->pairs.resize((size_t)n);
         int64_t w = 0;
         for (int32_t i = 0; i < n; ++i) {
-            const int32_t lmax = tsu_host::max_seq_len(s->ctx);  // both sequences are staged in LDS
+            const int32_t lmax = tsu_host::seq_len_limit(s->ctx);
             if (p_len[i] < 0 || t_len[i] < 0 || p_len[i] > lmax || t_len[i] > lmax) { delete r; return TSU_EINVAL; }
             r->pairs[(size_t)i] = tsu_pair{w, w + tsu_packed_words(p_len[i]), p_len[i], t_len[i]};
             w += tsu_packed_words(p_len[i]) + tsu_packed_words(t_len[i]);
@@ -615,7 +615,7 @@ extern "C" int tsu_serve_submit(tsu_server* s, const char* const* patterns, cons
 extern "C" int tsu_serve_submit_packed(tsu_server* s, const uint32_t* words, int64_t n_words, const tsu_pair* pairs,
                                        int32_t n, uint64_t* ticket) {
     if (!s || !ticket || n < 0 || n_words < 0 || (n > 0 && (!words || !pairs))) return TSU_EINVAL;
-    const int32_t lmax = tsu_host::max_seq_len(s->ctx);  // both sequences are staged in LDS
+    const int32_t lmax = tsu_host::seq_len_limit(s->ctx);
     for (int32_t i = 0; i < n; ++i) {
         const tsu_pair& pr = pairs[i];
         if (pr.p_len < 0 || pr.t_len < 0 || pr.p_len > lmax || pr.t_len > lmax || pr.p_off < 0 || pr.t_off < 0 ||

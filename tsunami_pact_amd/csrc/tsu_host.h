@@ -85,7 +85,10 @@ namespace tsu_host {
 // Kernel arguments with the context's scaled penalties and ring depths.
 KArgs base_args(const tsu_ctx* ctx);
 // LDS words per staged sequence for sequences up to max_len bases.
-int32_t seq_words_for(int32_t max_len);
+int32_t seq_words_for(const tsu_ctx* ctx, int32_t max_len);
+// Marks an LDS-kernel shape whose sequences are longer than LDS holds: that
+// kernel then reads them from HBM (max_seq_len; seq_words_for is then 0).
+void place_sequences(const tsu_ctx* ctx, Shape& sh, int32_t max_len);
 // Length bucket bounds on max(p, t) and the bucket of a length.
 int bucket_count();
 int32_t bucket_bound(int i);
@@ -115,8 +118,14 @@ int fail(tsu_ctx* ctx, int status, const std::string& msg);
 bool fixed_242(const tsu_ctx* ctx);
 // Chunks of 64 diagonals of the register-resident kernel for capacity W (0: LDS kernel).
 int reg_chunks(const tsu_ctx* ctx, int32_t W);
-// Longest sequence a pair may hold (both sequences are staged in LDS).
+// Longest sequence staged in LDS (both sequences of a pair are, next to the
+// kernel's own state); longer ones are read from HBM by the LDS kernel.
 int32_t max_seq_len(const tsu_ctx* ctx);
+// Longest sequence a pair may hold: kMaxSeqLen (int32 offsets and capacities
+// of up to 2^29 diagonals), or max_seq_len with 2-piece penalties, whose
+// kernel stages both sequences in LDS.
+constexpr int32_t kMaxSeqLen = 1 << 28;
+int32_t seq_len_limit(const tsu_ctx* ctx);
 // Waves of the multi-wave register kernel for capacity W (0: not eligible).
 int reg_waves(const tsu_ctx* ctx, int32_t W);
 // Waves of the 16-chunk multi-wave kernel (older M scores in an int16 LDS
@@ -130,9 +139,10 @@ int ml_waves(const tsu_ctx* ctx, int32_t W, int32_t max_len);
 constexpr int32_t kProbeCap = 240;
 int32_t probe_capacity(const tsu_ctx* ctx, const tsu_pair& pr, int32_t progress);
 // Pairs of a length bucket are probed when the register kernels take them
-// and the bucket's first capacity (W) needs several waves.
-inline bool probe_bucket(const tsu_ctx* ctx, int32_t W, int32_t tuned_cap, int tuned_waves) {
-    return fixed_242(ctx) && tuned_cap <= 0 && tuned_waves <= 1 && W > 1024;
+// (their sequences, up to max_len bases, fit in LDS) and the bucket's first
+// capacity (W) needs several waves.
+inline bool probe_bucket(const tsu_ctx* ctx, int32_t W, int32_t tuned_cap, int tuned_waves, int32_t max_len) {
+    return fixed_242(ctx) && tuned_cap <= 0 && tuned_waves <= 1 && W > 1024 && max_len <= max_seq_len(ctx);
 }
 
 // Reorders idx by key(i) descending, keeping the order of equal keys (a

@@ -93,6 +93,7 @@ struct Shape {
     bool ck = false;         // backtrace from ring checkpoints (bt and queue set, LDS kernel)
     bool dbl = false;        // 16-chunk kernel with doubled offsets (sequences up to kMlDblMaxLen bases)
     bool probe = false;      // single-wave register kernel stopping at KArgs::probe_cap
+    bool gseq = false;       // LDS kernel reading the sequences from HBM (too long for LDS; int32 offsets)
 };
 
 // Doubled-offset 16-chunk kernel: longest sequence (2t fits int16) and the

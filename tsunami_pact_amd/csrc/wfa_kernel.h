@@ -63,11 +63,25 @@ template <> struct Null<int32_t> { static constexpr int32_t v = -1073741824; };
 
 constexpr int kSmemInts = 16;  // misc uniform values in LDS
 
+// Word reader of one 2-bit packed sequence (16 bases per word, LSB first).
+// G false: the copy staged in LDS, with two zero words of padding after the
+// last base, so index j+2 is always in bounds.  G true: the packed words in
+// HBM (sequences too long for LDS), read through the caches; words past the
+// sequence's own read as 0, so no read leaves it.  Bases past the end never
+// decide a result: every comparison is cut at the shorter remaining length.
+template <bool G>
+struct SeqWords {
+    const uint32_t* w;
+    int n;  // words of the sequence (G only)
+    __device__ __forceinline__ uint32_t operator[](int j) const {
+        if constexpr (G) return j < n ? w[j] : 0u;
+        else return w[j];
+    }
+};
+
 // Longest common prefix of pattern[v..] and text[h..], at most lim bases.
-// P and T are 2-bit packed (16 bases per word, LSB first) with two zero
-// words of padding after the last base, so index j+2 is always in bounds.
-__device__ __forceinline__ int lcp_packed(const uint32_t* __restrict__ P, const uint32_t* __restrict__ T,
-                                          int v, int h, int lim) {
+template <bool G>
+__device__ __forceinline__ int lcp_packed(SeqWords<G> P, SeqWords<G> T, int v, int h, int lim) {
     int n = 0;
     while (n < lim) {
         const int a = v + n, b = h + n;
@@ -113,9 +127,11 @@ struct Pen {
     static constexpr int ri = fixed ? ES + 1 : 0;                  // I/D ring depth
 };
 
-template <int NW, typename OffT, bool LDS_RING, bool BT, bool QUEUE, int XS, int OS, int ES, bool CK = false>
+template <int NW, typename OffT, bool LDS_RING, bool BT, bool QUEUE, int XS, int OS, int ES, bool CK = false,
+          bool GSEQ = false>
 __global__ __launch_bounds__(NW * 64) void wfa_kernel(KArgs a) {
     static_assert(!CK || (BT && QUEUE), "checkpointed backtrace runs in persistent backtrace launches");
+    static_assert(!GSEQ || sizeof(OffT) == 4, "sequences read from HBM are longer than int16 offsets hold");
     using PP = Pen<XS, OS, ES>;
     extern __shared__ __align__(16) uint32_t smem[];
     constexpr int NT = NW * 64;
@@ -171,10 +187,16 @@ __global__ __launch_bounds__(NW * 64) void wfa_kernel(KArgs a) {
         const int p = uni(pr.p_len), t = uni(pr.t_len);
         const int pw = (p + 15) >> 4, tw = (t + 15) >> 4;
 
-        // ---- stage sequences, clear the ring --------------------------------
-        for (int i = tid; i < a.seq_words; i += NT) {
-            sP[i] = i < pw ? a.words[pr.p_off + i] : 0u;
-            sT[i] = i < tw ? a.words[pr.t_off + i] : 0u;
+        // ---- stage sequences (or read them in place), clear the ring --------
+        SeqWords<GSEQ> P{sP, pw}, T{sT, tw};
+        if constexpr (GSEQ) {
+            P.w = a.words + pr.p_off;
+            T.w = a.words + pr.t_off;
+        } else {
+            for (int i = tid; i < a.seq_words; i += NT) {
+                sP[i] = i < pw ? a.words[pr.p_off + i] : 0u;
+                sT[i] = i < tw ? a.words[pr.t_off + i] : 0u;
+            }
         }
         auto fill_ring = [&]() __attribute__((always_inline)) {
             // slots are padded to a multiple of 16 bytes: fill with uint4
@@ -218,7 +240,7 @@ __global__ __launch_bounds__(NW * 64) void wfa_kernel(KArgs a) {
 
         // ---- score 0: M[0][0] = 0, extended --------------------------------
         if (tid == 0) {
-            const int ext = lcp_packed(sP, sT, 0, 0, min(p, t));
+            const int ext = lcp_packed(P, T, 0, 0, min(p, t));
             ring[0 * stride + (0 - kmin + 1)] = (OffT)ext;
             sMisc[1] = (p == t && ext == t) ? 1 : 0;
             if constexpr (BT && !CK) { rec[0] = 0u; stepLo[0] = 0; stepOff[0] = 0; }
@@ -349,8 +371,8 @@ __global__ __launch_bounds__(NW * 64) void wfa_kernel(KArgs a) {
                         const int pb = go ? M + n : 0;
                         const int ja = pa >> 4, sa = (pa & 15) << 1;
                         const int jb = pb >> 4, sb = (pb & 15) << 1;
-                        const uint32_t p0 = sP[ja], p1 = sP[ja + 1], p2 = sP[ja + 2];
-                        const uint32_t t0 = sT[jb], t1 = sT[jb + 1], t2 = sT[jb + 2];
+                        const uint32_t p0 = P[ja], p1 = P[ja + 1], p2 = P[ja + 2];
+                        const uint32_t t0 = T[jb], t1 = T[jb + 1], t2 = T[jb + 2];
                         const uint32_t xl = __builtin_amdgcn_alignbit(p1, p0, sa) ^ __builtin_amdgcn_alignbit(t1, t0, sb);
                         const uint32_t xh = __builtin_amdgcn_alignbit(p2, p1, sa) ^ __builtin_amdgcn_alignbit(t2, t1, sb);
                         // first differing bit of the 64-bit window (64 when equal)
@@ -561,20 +583,39 @@ __global__ __launch_bounds__(NW * 64) void wfa_kernel(KArgs a) {
 
 typedef void (*KernelFn)(KArgs);
 
-template <int NW, typename OffT, bool LDS, bool BT, bool Q>
+template <int NW, typename OffT, bool LDS, bool BT, bool Q, bool G = false>
 inline KernelFn fn(bool fixed_242) {
     // scaled penalties (2, 3, 1) are the reference's 4/6/2 divided by 2
-    return fixed_242 ? wfa_kernel<NW, OffT, LDS, BT, Q, 2, 3, 1> : wfa_kernel<NW, OffT, LDS, BT, Q, 0, 0, 0>;
+    return fixed_242 ? wfa_kernel<NW, OffT, LDS, BT, Q, 2, 3, 1, false, G> : wfa_kernel<NW, OffT, LDS, BT, Q, 0, 0, 0, false, G>;
 }
 
-template <int NW, typename OffT, bool LDS>
+template <int NW, typename OffT, bool LDS, bool G = false>
 inline KernelFn fn_ck(bool fixed_242) {
-    return fixed_242 ? wfa_kernel<NW, OffT, LDS, true, true, 2, 3, 1, true> : wfa_kernel<NW, OffT, LDS, true, true, 0, 0, 0, true>;
+    return fixed_242 ? wfa_kernel<NW, OffT, LDS, true, true, 2, 3, 1, true, G>
+                     : wfa_kernel<NW, OffT, LDS, true, true, 0, 0, 0, true, G>;
+}
+
+// Instances reading the sequences from HBM (int32 offsets only).
+template <int NW>
+KernelFn pick_gseq(const Shape& sh) {
+    const bool f = sh.fixed_242;
+    if (sh.off16) return nullptr;
+    if (sh.ck) {
+        if (!sh.queue || !sh.bt || NW < 4) return nullptr;
+        return sh.lds_ring ? fn_ck<NW, int32_t, true, true>(f) : fn_ck<NW, int32_t, false, true>(f);
+    }
+    if (!sh.queue) {
+        if (!sh.lds_ring || sh.bt) return nullptr;
+        return fn<NW, int32_t, true, false, false, true>(f);
+    }
+    if (sh.lds_ring) return sh.bt ? fn<NW, int32_t, true, true, true, true>(f) : fn<NW, int32_t, true, false, true, true>(f);
+    return sh.bt ? fn<NW, int32_t, false, true, true, true>(f) : fn<NW, int32_t, false, false, true, true>(f);
 }
 
 template <int NW>
 KernelFn pick_nw_impl(const Shape& sh) {
     const bool f = sh.fixed_242;
+    if (sh.gseq) return pick_gseq<NW>(sh);
     if (sh.ck) {
         // checkpointed backtrace: persistent launches of 4 or more waves
         if (!sh.queue || !sh.bt || NW < 4) return nullptr;
