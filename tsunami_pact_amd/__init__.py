@@ -293,6 +293,14 @@ class Dataset:
             pass
 
 
+def _out_scores(out: Optional[np.ndarray], n: int) -> np.ndarray:
+    if out is None:
+        return np.zeros(max(n, 1), np.int32)
+    if out.dtype != np.int32 or out.ndim != 1 or not out.flags.c_contiguous or len(out) < max(n, 1):
+        raise ValueError(f"out must be a contiguous int32 array of at least {max(n, 1)} entries")
+    return out
+
+
 def pack(seq: str) -> np.ndarray:
     lib = library()
     n = int(lib.tsu_packed_words(len(seq)))
@@ -375,9 +383,11 @@ class Aligner:
         self._batches.add(b)
         return b
 
-    def align_plain(self, batch: Batch, wf_length: int, raise_on_error: bool = True):
+    def align_plain(self, batch: Batch, wf_length: int, raise_on_error: bool = True, out: Optional[np.ndarray] = None):
+        """Scores of every pair (plain form).  out: an int32 array of at least
+        len(batch) entries to write them to (returned as its first len(batch))."""
         n = len(batch)
-        scores = np.zeros(max(n, 1), np.int32)
+        scores = _out_scores(out, n)
         t = Timing()
         r = self._lib.tsu_align_plain(self._h, batch._h, wf_length, _i32p(scores), ctypes.byref(t))
         if raise_on_error:
@@ -394,9 +404,10 @@ class Aligner:
                     "tsu_align_fixed")
         return scores[:n], t.as_dict()
 
-    def align_batched(self, batch: Batch):
+    def align_batched(self, batch: Batch, out: Optional[np.ndarray] = None):
+        """Scores of every pair (batched form); out as for align_plain."""
         n = len(batch)
-        scores = np.zeros(max(n, 1), np.int32)
+        scores = _out_scores(out, n)
         t = Timing()
         self._check(self._lib.tsu_align_batched(self._h, batch._h, _i32p(scores), ctypes.byref(t)), "tsu_align_batched")
         return scores[:n], t.as_dict()

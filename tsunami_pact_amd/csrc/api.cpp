@@ -544,6 +544,19 @@ static Shape plain_shape(const tsu_ctx* ctx, int32_t W, int32_t max_len, bool* f
     return sh;
 }
 
+// Lowest of n scores: only a negative one (a status) calls for the per-pair
+// checks after a call.
+static int32_t min_score(const int32_t* s, int32_t n) {
+    int32_t m = INT32_MAX;
+#pragma omp simd reduction(min : m)
+    for (int32_t i = 0; i < n; ++i) m = s[i] < m ? s[i] : m;
+    return m;
+}
+
+// (below) with min32 non-null the copied bytes are int32 scores and *min32
+// receives their lowest
+static int copy_to_host(tsu_ctx* ctx, void* dst, const void* src, size_t bytes, int32_t* min32 = nullptr);
+
 extern "C" int tsu_align_plain(tsu_ctx* ctx, tsu_batch* b, int32_t wf_length, int32_t* scores, tsu_timing* t) {
     if (!ctx || !b || !scores || b->ctx != ctx) return fail(ctx, TSU_EINVAL, "tsu_align_plain: bad arguments");
     if (wf_length < 8) wf_length = 8;
@@ -571,10 +584,13 @@ extern "C" int tsu_align_plain(tsu_ctx* ctx, tsu_batch* b, int32_t wf_length, in
     if (r != TSU_OK) return r;
     double wall = now_ms() - w0;
     double d0 = now_ms();
-    HIPCHK(hipMemcpy(scores, b->d_scores, (size_t)b->n * sizeof(int32_t), hipMemcpyDeviceToHost), "hipMemcpy D2H (scores)");
+    int32_t lowest = 0;
+    r = copy_to_host(ctx, scores, b->d_scores, (size_t)b->n * sizeof(int32_t), &lowest);
+    if (r != TSU_OK) return r;
     double d2h = now_ms() - d0;
     int launches = 1;
-    if (sh.reg_c == 16 || sh.lring) {
+    const bool clean = lowest >= 0;
+    if (!clean && (sh.reg_c == 16 || sh.lring)) {
         // the kernels with an int16 LDS ring stop past their score limit with
         // an overflow status: such pairs get the LDS kernel at the same
         // capacity, so whether a pair fits depends on the capacity and not on
@@ -601,7 +617,8 @@ extern "C" int tsu_align_plain(tsu_ctx* ctx, tsu_batch* b, int32_t wf_length, in
             ++launches;
             wall += now_ms() - r0;
             d0 = now_ms();
-            HIPCHK(hipMemcpy(scores, b->d_scores, (size_t)b->n * sizeof(int32_t), hipMemcpyDeviceToHost), "hipMemcpy D2H (scores)");
+            r = copy_to_host(ctx, scores, b->d_scores, (size_t)b->n * sizeof(int32_t));
+            if (r != TSU_OK) return r;
             d2h += now_ms() - d0;
         }
     }
@@ -610,6 +627,7 @@ extern "C" int tsu_align_plain(tsu_ctx* ctx, tsu_batch* b, int32_t wf_length, in
         t->kernel_ms = kms; t->wall_ms = wall; t->d2h_ms = d2h;
         t->pairs = b->n; t->cells = b->cells; t->launches = launches;
     }
+    if (clean) return TSU_OK;
     for (int32_t i = 0; i < b->n; ++i) {
         if (scores[i] == TSU_EKOVERFLOW) return fail(ctx, TSU_EKOVERFLOW, "pair " + std::to_string(i) + ": diagonal range exceeded wf_length " + std::to_string(wf_length));
     }
@@ -1052,6 +1070,7 @@ extern "C" int tsu_align_batched(tsu_ctx* ctx, tsu_batch* b, int32_t* scores, ts
     const double w1 = now_ms();
     if (t) { t->wall_ms = w1 - w0; t->pairs = b->n; t->cells = b->cells; }
     if (r != TSU_OK) return r;
+    if (min_score(scores, b->n) >= 0) return TSU_OK;
     for (int32_t i = 0; i < b->n; ++i)
         if (scores[i] < 0) return fail(ctx, scores[i], "pair " + std::to_string(i) + ": " + tsu_strerror(scores[i]));
     return TSU_OK;
@@ -1272,12 +1291,15 @@ extern "C" int tsu_set_tune_table(tsu_ctx* ctx, const tsu_tune_entry* entries, i
 // ---------------------------------------------------------------------------
 // CIGAR form
 // ---------------------------------------------------------------------------
-// Device-to-host copy of a large buffer into pageable memory: slices go
-// through a pinned staging buffer of two halves kept by the context; while
-// host threads move one slice on (touching the destination's fresh pages in
-// parallel), the next slice's DMA fills the other half.
-static int copy_to_host(tsu_ctx* ctx, void* dst, const void* src, size_t bytes) {
+// Device-to-host copy into pageable memory, queued behind the work on the
+// context's stream: slices go through a pinned staging buffer of two halves
+// kept by the context; while host threads move one slice on (touching the
+// destination's fresh pages in parallel), the next slice's DMA fills the
+// other half.  (A pageable hipMemcpy goes through the runtime's own staging
+// in small pieces: the scores of a million pairs took about 1 ms that way.)
+static int copy_to_host(tsu_ctx* ctx, void* dst, const void* src, size_t bytes, int32_t* min32) {
     constexpr size_t kSlice = 32u << 20;
+    if (min32) *min32 = INT32_MAX;
     if (bytes == 0) return TSU_OK;
     const size_t want = 2 * kSlice;
     if (ctx->h_stage_bytes < want) {
@@ -1301,12 +1323,28 @@ static int copy_to_host(tsu_ctx* ctx, void* dst, const void* src, size_t bytes) 
         const size_t off = k * kSlice;
         const size_t n = std::min(kSlice, bytes - off);
         const uint8_t* st = half[k & 1];
-        const size_t part = (n + 15) / 16;
-#pragma omp parallel for schedule(static)
+        // each thread also takes the lowest int32 of its part while that is
+        // in its own cache (slices and parts are multiples of 64 bytes)
+        auto part_min = [&](size_t b0, size_t b1) -> int32_t {
+            return min_score(reinterpret_cast<const int32_t*>(static_cast<uint8_t*>(dst) + off + b0),
+                             (int32_t)((b1 - b0) / 4));
+        };
+        if (n <= (1u << 20)) {  // small: one thread
+            std::memcpy(static_cast<uint8_t*>(dst) + off, st, n);
+            if (min32) *min32 = std::min(*min32, part_min(0, n));
+            continue;
+        }
+        const size_t part = (n + 16 * 64 - 1) / (16 * 64) * 64;
+        int32_t m = INT32_MAX;
+#pragma omp parallel for schedule(static) reduction(min : m)
         for (int64_t i = 0; i < 16; ++i) {
             const size_t b0 = std::min(n, (size_t)i * part), b1 = std::min(n, b0 + part);
-            if (b1 > b0) std::memcpy(static_cast<uint8_t*>(dst) + off + b0, st + b0, b1 - b0);
+            if (b1 > b0) {
+                std::memcpy(static_cast<uint8_t*>(dst) + off + b0, st + b0, b1 - b0);
+                if (min32) m = std::min(m, part_min(b0, b1));
+            }
         }
+        if (min32) *min32 = std::min(*min32, m);
     }
     return TSU_OK;
 }
