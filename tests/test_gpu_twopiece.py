@@ -1,12 +1,21 @@
 """GPU parity for gap-affine 2-piece penalties (a gap of length L costs
 min(o1 + L e1, o2 + L e2)): libtsunami_hip.so against the 2-piece oracle
 (oracle/wfa2p_oracle.c, itself pinned by a 2-piece Gotoh DP), bit-exact."""
+import math
+
 import numpy as np
 import pytest
 
 pytestmark = pytest.mark.gpu
 SEED = 0x5453554E
-PEN_2P = [(4, 6, 2, 24, 1), (4, 6, 2, 12, 2), (5, 1, 3, 10, 1)]
+# the first and the last two have e1 = 2, e2 = 1 after the gcd: the
+# single-wave register kernel takes their windows of up to 1,024 diagonals
+PEN_2P = [(4, 6, 2, 24, 1), (4, 6, 2, 12, 2), (5, 1, 3, 10, 1), (3, 4, 2, 20, 1), (6, 5, 2, 30, 1)]
+
+
+def register_kernel(pen):
+    g = math.gcd(*pen)
+    return pen[2] // g == 2 and pen[4] // g == 1
 
 
 def say(msg):
@@ -59,11 +68,44 @@ def test_two_piece_cigars(tsu, oracle, pen):
         pairs = pairs_for(tsu, SEED + 40 + sum(pen))
         say(f"2-piece {pen} CIGARs, {len(pairs)} pairs")
         got, cig, t = a.align_cigar(a.batch(pairs))
-        assert t["kernels"] == {"2p"}, t["kernels"]
+        if register_kernel(pen):  # the widest pairs need the LDS-ring kernel
+            assert "2p_reg" in t["kernels"] and t["kernels"] <= {"2p", "2p_reg"}, t["kernels"]
+        else:
+            assert t["kernels"] == {"2p"}, t["kernels"]
         for i, (p, q) in enumerate(pairs):
             s, ops = oracle.wfa2p_cigar(p, q, *pen)
             assert got[i] == s, i
             assert cig[i] == oracle.rle(ops), (i, p, q)
+    finally:
+        a.close()
+
+
+@pytest.mark.parametrize("pen", [p for p in PEN_2P if register_kernel(p)])
+def test_two_piece_register_kernel_1000bp(tsu, oracle, pen):
+    """Pairs of the benchmark's shape (1000 bp, 1-8% mixed edits) in windows
+    of 256, 512, 768 and 1,024 diagonals of the register kernel: plain scores
+    and CIGARs equal the oracle's (and a window too narrow for a pair
+    reports TSU_EKOVERFLOW for it, as the LDS-ring kernel does)."""
+    a = tsu.Aligner(pen[0], pen[1], pen[2], gap_opening2=pen[3], gap_extension2=pen[4])
+    try:
+        pairs = tsu.Dataset.generate(300, seed=SEED + 80 + sum(pen), len_min=1000, err_min=0.01,
+                                     err_max=0.08).pairs()
+        want, _ = oracle.wfa2p_batch(pairs, *pen)
+        b = a.batch(pairs)
+        for W in (256, 512, 768, 1024):
+            say(f"2-piece {pen} register kernel, plain W={W}")
+            got, _ = a.align_plain(b, W, raise_on_error=False)
+            ok = got != tsu.TSU_EKOVERFLOW
+            assert (got[ok] == want[ok]).all(), (pen, W)
+            if W == 1024:
+                assert ok.sum() > len(pairs) // 2
+        say(f"2-piece {pen} register kernel, CIGARs")
+        got, cig, t = a.align_cigar(b)
+        assert "2p_reg" in t["kernels"], t["kernels"]
+        assert (got == want).all()
+        for i in range(0, len(pairs), 10):
+            s, ops = oracle.wfa2p_cigar(pairs[i][0], pairs[i][1], *pen)
+            assert cig[i] == oracle.rle(ops), i
     finally:
         a.close()
 

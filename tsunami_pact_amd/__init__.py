@@ -37,7 +37,7 @@ TSU_ECIGAR = -10
 OPS = "MXID"
 
 # TSU_KERNEL_* bits (tsu_cigars_kernels): which kernels recorded backtraces
-KERNEL_BITS = {0x01: "reg1", 0x02: "ml4", 0x04: "ml8", 0x08: "ml16", 0x10: "lds", 0x20: "lds_ck", 0x40: "2p"}
+KERNEL_BITS = {0x01: "reg1", 0x02: "ml4", 0x04: "ml8", 0x08: "ml16", 0x10: "lds", 0x20: "lds_ck", 0x40: "2p", 0x80: "2p_reg"}
 
 # Every symbol include/tsunami_wfa.h declares.
 EXPORTS = (

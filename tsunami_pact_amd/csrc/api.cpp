@@ -472,6 +472,19 @@ int tsu_host::reg_chunks(const tsu_ctx* ctx, int32_t W) {
     return c;
 }
 
+int tsu_host::reg2p_chunks(const tsu_ctx* ctx, int32_t W, int32_t max_len) {
+    if (!ctx->two_piece || ctx->es != 2 || ctx->e2s != 1 || W > 1024 || max_len > 16000) return 0;
+    int c = 1;
+    while (64 * c < W) c = c < 8 ? 2 * c : c + 4;  // 1, 2, 4, 8, 12, 16
+    Shape sh{1, true, true, false, true};
+    sh.two_piece = true;
+    sh.reg_c = c;
+    KArgs a = base_args(ctx);
+    a.W = 64 * c;
+    a.seq_words = seq_words_for(ctx, max_len);
+    return tsu::lds_bytes(sh, a) <= ctx->max_lds ? c : 0;
+}
+
 int tsu_host::reg_waves(const tsu_ctx* ctx, int32_t W) {
     if (!tsu_host::fixed_242(ctx) || W <= 512 || W > 16 * 512) return 0;
     int nw = 2;
@@ -527,6 +540,7 @@ static Shape plain_shape(const tsu_ctx* ctx, int32_t W, int32_t max_len, bool* f
     if (ctx->two_piece) {
         Shape s2{1, max_len <= 32000, true, false, false};
         s2.two_piece = true;
+        s2.reg_c = tsu_host::reg2p_chunks(ctx, W, max_len);
         KArgs a = base_args(ctx);
         a.W = W;
         a.seq_words = seq_words_for(ctx, max_len);
@@ -590,7 +604,7 @@ extern "C" int tsu_align_plain(tsu_ctx* ctx, tsu_batch* b, int32_t wf_length, in
     double d2h = now_ms() - d0;
     int launches = 1;
     const bool clean = lowest >= 0;
-    if (!clean && (sh.reg_c == 16 || sh.lring)) {
+    if (!clean && (sh.reg_c == 16 || sh.lring || (sh.two_piece && sh.reg_c))) {
         // the kernels with an int16 LDS ring stop past their score limit with
         // an overflow status: such pairs get the LDS kernel at the same
         // capacity, so whether a pair fits depends on the capacity and not on
@@ -602,6 +616,10 @@ extern "C" int tsu_align_plain(tsu_ctx* ctx, tsu_batch* b, int32_t wf_length, in
         if (!again.empty()) {
             Shape s2{default_waves(W), b->max_len <= 32000, true, false, true};
             s2.fixed_242 = sh.fixed_242;
+            if (sh.two_piece) {  // the 2-piece LDS-ring kernel
+                s2.nw = 1;
+                s2.two_piece = true;
+            }
             tsu_host::place_sequences(ctx, s2, b->max_len);
             KArgs a2 = base_args(ctx);
             a2.W = W;
@@ -714,9 +732,10 @@ int32_t tsu_host::initial_capacity(const tsu_ctx* ctx, int32_t bound) {
 }
 
 Shape tsu_host::batched_shape(const tsu_ctx* ctx, int32_t W, int32_t max_len, int nw_hint) {
-    if (ctx->two_piece) {  // one shape: the 2-piece kernel, fed from the queue
+    if (ctx->two_piece) {  // the 2-piece kernels, fed from the queue
         Shape s2{1, max_len <= 32000, true, false, true};
         s2.two_piece = true;
+        s2.reg_c = tsu_host::reg2p_chunks(ctx, W, max_len);
         return s2;
     }
     Shape sh{nw_hint > 0 ? nw_hint : default_waves(W), max_len <= 32000, true, false, true};
@@ -2035,11 +2054,21 @@ int CigarCall::two_piece_pass() {
     for (int32_t i : order) more += 2ll * (scores[i] / ctx->g / unit2) + 2;
     r = grow_runs(more);
     if (r != TSU_OK) return r;
+    // the kernel (and register window) a pair's own capacity takes: groups
+    // end where it changes, so the widest pairs do not take the others onto
+    // the LDS-ring kernel
+    auto family = [&](int32_t i) {
+        const tsu_pair& pr = b->pairs[(size_t)i];
+        const int32_t Wi = std::max<int32_t>(capacity_for_score(ctx, scores[i] / ctx->g), 16);
+        return tsu_host::reg2p_chunks(ctx, Wi, std::max(pr.p_len, pr.t_len));
+    };
     size_t pos = 0;
     while (pos < order.size()) {
         const int32_t s0 = scores[order[pos]] / ctx->g;  // highest score of this group (sorted desc)
         const int32_t W = std::max<int32_t>(capacity_for_score(ctx, s0), 16);
-        const size_t end = std::min(order.size(), pos + 65536);
+        const int f0 = family(order[pos]);
+        size_t end = pos + 1;
+        while (end < order.size() && end - pos < 65536 && family(order[end]) == f0) ++end;
         int32_t max_len = 0;
         for (size_t j = pos; j < end; ++j) {
             const tsu_pair& pr = b->pairs[(size_t)order[j]];
@@ -2050,12 +2079,18 @@ int CigarCall::two_piece_pass() {
         a.seq_words = seq_words_for(ctx, max_len);
         Shape sh{1, max_len <= 32000, true, true, true};  // records, queue-fed
         sh.two_piece = true;
-        if (tsu::lds_bytes(sh, a) > ctx->max_lds)
-            return fail(ctx, TSU_EKOVERFLOW, "pair " + std::to_string(order[pos]) +
-                                                 ": its 2-piece wavefront range does not fit the CU's LDS");
-        int64_t recs = 64;
-        for (int64_t sc = 0; sc <= s0; ++sc) recs += std::min<int64_t>(W, capacity_for_score(ctx, sc));
         const int32_t steps = s0 + 2;
+        int64_t recs = 64;
+        if (int c = tsu_host::reg2p_chunks(ctx, W, max_len)) {
+            // the register kernel: records at a fixed place per score and chunk
+            sh.reg_c = c;
+            recs = (int64_t)steps * c * 64;
+        } else {
+            if (tsu::lds_bytes(sh, a) > ctx->max_lds)
+                return fail(ctx, TSU_EKOVERFLOW, "pair " + std::to_string(order[pos]) +
+                                                     ": its 2-piece wavefront range does not fit the CU's LDS");
+            for (int64_t sc = 0; sc <= s0; ++sc) recs += std::min<int64_t>(W, capacity_for_score(ctx, sc));
+        }
         const int64_t per_wg = recs * 4 + (int64_t)steps * 12;
         std::vector<int32_t> ids(order.begin() + (ptrdiff_t)pos, order.begin() + (ptrdiff_t)end);
         r = launch_bt(sh, a, W, ids, max_len, recs, steps, per_wg);
@@ -2133,7 +2168,7 @@ extern "C" int tsu_cigars_stats(const tsu_cigars* c, int64_t* state_bytes, int32
 
 uint32_t tsu_host::kernel_bit(const Shape& sh) {
     if (sh.ck) return TSU_KERNEL_LDS_CK;
-    if (sh.two_piece) return TSU_KERNEL_2P;
+    if (sh.two_piece) return sh.reg_c ? TSU_KERNEL_2P_REG : TSU_KERNEL_2P;
     if (sh.reg_c > 0 && sh.nw == 1) return TSU_KERNEL_REG1;
     if (sh.reg_c == 16) return sh.nw <= 4 ? TSU_KERNEL_ML4 : sh.nw == 8 ? TSU_KERNEL_ML8 : TSU_KERNEL_ML16;
     if (sh.reg_c > 0) return TSU_KERNEL_REG1;

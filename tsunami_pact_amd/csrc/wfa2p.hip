@@ -36,6 +36,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "wfa2p_reg.h"
 #include "wfa_bt.h"
 #include "wfa_reg.h"
 
@@ -302,7 +303,24 @@ __global__ __launch_bounds__(64) void wfa2p_kernel(KArgs a) {
     }
 }
 
+template <int C>
+KernelFn pick_2p_reg(const Shape& sh) {
+    if (sh.bt) return sh.queue && !sh.ck ? wfa2p_reg_kernel<C, true, true> : nullptr;
+    return sh.queue ? wfa2p_reg_kernel<C, true> : wfa2p_reg_kernel<C, false>;
+}
+
 KernelFn pick_2p(const Shape& sh) {
+    if (sh.reg_c > 0) {  // the single-wave register kernel (e1 = 2, e2 = 1)
+        switch (sh.reg_c) {
+        case 1: return pick_2p_reg<1>(sh);
+        case 2: return pick_2p_reg<2>(sh);
+        case 4: return pick_2p_reg<4>(sh);
+        case 8: return pick_2p_reg<8>(sh);
+        case 12: return pick_2p_reg<12>(sh);
+        case 16: return pick_2p_reg<16>(sh);
+        default: return nullptr;
+        }
+    }
     if (sh.bt) {  // CIGAR launches: records, queue-fed
         if (sh.ck || !sh.queue) return nullptr;
         return sh.off16 ? wfa2p_kernel<int16_t, true, true> : wfa2p_kernel<int32_t, true, true>;
@@ -311,8 +329,13 @@ KernelFn pick_2p(const Shape& sh) {
     return sh.queue ? wfa2p_kernel<int32_t, true> : wfa2p_kernel<int32_t, false>;
 }
 
-// LDS bytes of the 2-piece kernel: sequences, misc, slot limits, rings.
+// LDS bytes of the 2-piece kernels: sequences, misc, slot limits, rings
+// (the register kernel: sequences, misc and the int16 M ring).
 int64_t lds_bytes_2p(const Shape& sh, const KArgs& a) {
+    if (sh.reg_c > 0) {
+        const int64_t ring = (int64_t)a.rm * (64ll * sh.reg_c + 2) * 2;
+        return 4 * (2 * (int64_t)a.seq_words + 4) + ((ring + 3) & ~3ll);
+    }
     const int64_t ns = (int64_t)a.rm + 2 * a.ri + 2 * a.r2 + 1;
     const int64_t ring = ns * ((int64_t)a.W + 2) * (sh.off16 ? 2 : 4);
     return 4 * (2 * (int64_t)a.seq_words + 8) + 8 * ns + ((ring + 3) & ~3ll);
