@@ -316,7 +316,7 @@ int tsu_cigars_stats(const tsu_cigars* c, int64_t* state_bytes, int32_t* record_
 #define TSU_KERNEL_LDS     0x10u  /* rings in LDS (or HBM), records for every score */
 #define TSU_KERNEL_LDS_CK  0x20u  /* rings in LDS (or HBM), ring checkpoints */
 #define TSU_KERNEL_2P      0x40u  /* gap-affine 2-piece kernel, records for every score */
-#define TSU_KERNEL_2P_REG  0x80u  /* gap-affine 2-piece single-wave register kernel (e1 = 2, e2 = 1) */
+#define TSU_KERNEL_2P_REG  0x80u  /* gap-affine 2-piece single-wave register kernel ((e1, e2) small) */
 int tsu_cigars_kernels(const tsu_cigars* c, uint32_t* kernels);
 
 /* ---- serving (no reference counterpart) ---------------------------------- */

@@ -8,14 +8,16 @@ import pytest
 
 pytestmark = pytest.mark.gpu
 SEED = 0x5453554E
-# the first and the last two have e1 = 2, e2 = 1 after the gcd: the
-# single-wave register kernel takes their windows of up to 1,024 diagonals
-PEN_2P = [(4, 6, 2, 24, 1), (4, 6, 2, 12, 2), (5, 1, 3, 10, 1), (3, 4, 2, 20, 1), (6, 5, 2, 30, 1)]
+# extensions (e1, e2) after the gcd: (2, 1), (1, 1), (3, 1), (2, 1), (2, 1)
+# take the single-wave register kernel in windows of up to 1,024 diagonals;
+# (3, 2) the LDS-ring kernel
+PEN_2P = [(4, 6, 2, 24, 1), (4, 6, 2, 12, 2), (5, 1, 3, 10, 1), (3, 4, 2, 20, 1), (6, 5, 2, 30, 1),
+          (4, 5, 3, 20, 2)]
 
 
 def register_kernel(pen):
     g = math.gcd(*pen)
-    return pen[2] // g == 2 and pen[4] // g == 1
+    return (pen[2] // g, pen[4] // g) in ((1, 1), (2, 1), (3, 1))
 
 
 def say(msg):

@@ -473,12 +473,14 @@ int tsu_host::reg_chunks(const tsu_ctx* ctx, int32_t W) {
 }
 
 int tsu_host::reg2p_chunks(const tsu_ctx* ctx, int32_t W, int32_t max_len) {
-    if (!ctx->two_piece || ctx->es != 2 || ctx->e2s != 1 || W > 1024 || max_len > 16000) return 0;
+    const int e = 10 * ctx->es + ctx->e2s;
+    if (!ctx->two_piece || (e != 11 && e != 21 && e != 31) || W > 1024 || max_len > 16000) return 0;
     int c = 1;
     while (64 * c < W) c = c < 8 ? 2 * c : c + 4;  // 1, 2, 4, 8, 12, 16
     Shape sh{1, true, true, false, true};
     sh.two_piece = true;
     sh.reg_c = c;
+    sh.reg_e = e;
     KArgs a = base_args(ctx);
     a.W = 64 * c;
     a.seq_words = seq_words_for(ctx, max_len);
@@ -541,6 +543,7 @@ static Shape plain_shape(const tsu_ctx* ctx, int32_t W, int32_t max_len, bool* f
         Shape s2{1, max_len <= 32000, true, false, false};
         s2.two_piece = true;
         s2.reg_c = tsu_host::reg2p_chunks(ctx, W, max_len);
+        s2.reg_e = 10 * ctx->es + ctx->e2s;
         KArgs a = base_args(ctx);
         a.W = W;
         a.seq_words = seq_words_for(ctx, max_len);
@@ -736,6 +739,7 @@ Shape tsu_host::batched_shape(const tsu_ctx* ctx, int32_t W, int32_t max_len, in
         Shape s2{1, max_len <= 32000, true, false, true};
         s2.two_piece = true;
         s2.reg_c = tsu_host::reg2p_chunks(ctx, W, max_len);
+        s2.reg_e = 10 * ctx->es + ctx->e2s;
         return s2;
     }
     Shape sh{nw_hint > 0 ? nw_hint : default_waves(W), max_len <= 32000, true, false, true};
@@ -2084,6 +2088,7 @@ int CigarCall::two_piece_pass() {
         if (int c = tsu_host::reg2p_chunks(ctx, W, max_len)) {
             // the register kernel: records at a fixed place per score and chunk
             sh.reg_c = c;
+            sh.reg_e = 10 * ctx->es + ctx->e2s;
             recs = (int64_t)steps * c * 64;
         } else {
             if (tsu::lds_bytes(sh, a) > ctx->max_lds)

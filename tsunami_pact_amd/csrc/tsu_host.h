@@ -129,8 +129,9 @@ int32_t seq_len_limit(const tsu_ctx* ctx);
 // Waves of the multi-wave register kernel for capacity W (0: not eligible).
 int reg_waves(const tsu_ctx* ctx, int32_t W);
 // Chunks (1, 2, 4, 8, 12 or 16) of the 2-piece single-wave register kernel
-// for capacity W: scaled extensions e1 = 2 and e2 = 1, sequences up to 16000
-// bases, its M ring in LDS (0: the LDS-ring 2-piece kernel).
+// for capacity W: extensions (e1, e2) of (1, 1), (2, 1) or (3, 1) after the
+// gcd, sequences up to 16000 bases, its M ring in LDS (0: the LDS-ring
+// 2-piece kernel).
 int reg2p_chunks(const tsu_ctx* ctx, int32_t W, int32_t max_len);
 // Waves of the 16-chunk multi-wave kernel (older M scores in an int16 LDS
 // ring) for capacity W and sequences up to max_len (0: not eligible).

@@ -94,6 +94,7 @@ struct Shape {
     bool dbl = false;        // 16-chunk kernel with doubled offsets (sequences up to kMlDblMaxLen bases)
     bool probe = false;      // single-wave register kernel stopping at KArgs::probe_cap
     bool gseq = false;       // LDS kernel reading the sequences from HBM (too long for LDS; int32 offsets)
+    int reg_e = 0;           // 2-piece register kernel: 10 e1 + e2 after the gcd (11, 21 or 31)
 };
 
 // Doubled-offset 16-chunk kernel: longest sequence (2t fits int16) and the

@@ -36,7 +36,6 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
-#include "wfa2p_reg.h"
 #include "wfa_bt.h"
 #include "wfa_reg.h"
 
@@ -303,21 +302,17 @@ __global__ __launch_bounds__(64) void wfa2p_kernel(KArgs a) {
     }
 }
 
-template <int C>
-KernelFn pick_2p_reg(const Shape& sh) {
-    if (sh.bt) return sh.queue && !sh.ck ? wfa2p_reg_kernel<C, true, true> : nullptr;
-    return sh.queue ? wfa2p_reg_kernel<C, true> : wfa2p_reg_kernel<C, false>;
-}
+// the register kernel's instances, per (e1, e2) after the gcd (wfa2p_reg_eXY.hip)
+KernelFn pick_2p_reg_11(const Shape& sh);
+KernelFn pick_2p_reg_21(const Shape& sh);
+KernelFn pick_2p_reg_31(const Shape& sh);
 
 KernelFn pick_2p(const Shape& sh) {
-    if (sh.reg_c > 0) {  // the single-wave register kernel (e1 = 2, e2 = 1)
-        switch (sh.reg_c) {
-        case 1: return pick_2p_reg<1>(sh);
-        case 2: return pick_2p_reg<2>(sh);
-        case 4: return pick_2p_reg<4>(sh);
-        case 8: return pick_2p_reg<8>(sh);
-        case 12: return pick_2p_reg<12>(sh);
-        case 16: return pick_2p_reg<16>(sh);
+    if (sh.reg_c > 0) {  // the single-wave register kernel
+        switch (sh.reg_e) {
+        case 11: return pick_2p_reg_11(sh);
+        case 21: return pick_2p_reg_21(sh);
+        case 31: return pick_2p_reg_31(sh);
         default: return nullptr;
         }
     }

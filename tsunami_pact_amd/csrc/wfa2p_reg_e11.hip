@@ -1,0 +1,7 @@
+// Instances of the 2-piece register kernel for gap extensions e1 = 1,
+// e2 = 1 after the gcd (one translation unit per pair of extensions).
+#include "wfa2p_reg.h"
+
+namespace tsu {
+KernelFn pick_2p_reg_11(const Shape& sh) { return pick_2p_reg_impl<1, 1>(sh); }
+}  // namespace tsu
