@@ -66,7 +66,11 @@ typedef struct {
     double kernel_ms;  /* HIP-event time of the alignment kernels, recorded on
                           the stream they run on */
     double wall_ms;    /* host clock around launch + synchronise: the window
-                          the reference times (wfa_gpu.cu:116-125) */
+                          the reference times (wfa_gpu.cu:116-125); the plain
+                          form waits once for the kernel and the scores' copy
+                          to pinned host memory queued behind it, so its
+                          window also holds that copy (4 MB per million
+                          pairs) */
     double h2d_ms;     /* host->device copies made by this call (0 when the
                           batch is already resident) */
     double d2h_ms;     /* device->host copy of scores (wfa_gpu.cu:128) */

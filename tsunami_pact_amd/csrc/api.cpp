@@ -571,8 +571,10 @@ static int32_t min_score(const int32_t* s, int32_t n) {
 }
 
 // (below) with min32 non-null the copied bytes are int32 scores and *min32
-// receives their lowest
-static int copy_to_host(tsu_ctx* ctx, void* dst, const void* src, size_t bytes, int32_t* min32 = nullptr);
+// receives their lowest; *synced_ms (if given) the host clock once the first
+// slice (all of it up to 32 MiB) is in pinned memory
+static int copy_to_host(tsu_ctx* ctx, void* dst, const void* src, size_t bytes, int32_t* min32 = nullptr,
+                        double* synced_ms = nullptr);
 
 extern "C" int tsu_align_plain(tsu_ctx* ctx, tsu_batch* b, int32_t wf_length, int32_t* scores, tsu_timing* t) {
     if (!ctx || !b || !scores || b->ctx != ctx) return fail(ctx, TSU_EINVAL, "tsu_align_plain: bad arguments");
@@ -586,25 +588,34 @@ extern "C" int tsu_align_plain(tsu_ctx* ctx, tsu_batch* b, int32_t wf_length, in
     Shape sh = plain_shape(ctx, W, b->max_len, &fits);
     if (sh.two_piece && !fits)
         return fail(ctx, TSU_EINVAL, "tsu_align_plain: 2-piece rings for wf_length " + std::to_string(W) + " do not fit in LDS");
-    double kms = 0;
-    int r;
-    if (fits) {
-        r = run_shape(ctx, io_of(b), sh, W, nullptr, b->n, b->max_len, 0, nullptr, &kms);
-    } else {
+    if (!fits) {
         // ring too wide for LDS: same one-workgroup-per-pair work, ring in HBM,
         // resident workgroups fed from the queue
         sh.queue = true;
         sh.lds_ring = false;
         sh.nw = 16;
-        r = run_shape(ctx, io_of(b), sh, W, nullptr, b->n, b->max_len, 0, nullptr, &kms);
     }
+    // the launch, and the scores' copy queued behind it on the same stream:
+    // one wait for both (the HBM ring, if any, outlives the wait)
+    DevBuf<uint8_t> ring;
+    bool launched = false;
+    int r = launch_shape(ctx, io_of(b), sh, W, nullptr, b->n, b->max_len, 0, nullptr, ctx->stream, ctx->d_queue,
+                         ctx->ev0, ctx->ev1, ring, &launched);
     if (r != TSU_OK) return r;
-    double wall = now_ms() - w0;
-    double d0 = now_ms();
     int32_t lowest = 0;
-    r = copy_to_host(ctx, scores, b->d_scores, (size_t)b->n * sizeof(int32_t), &lowest);
+    double synced = w0;
+    r = copy_to_host(ctx, scores, b->d_scores, (size_t)b->n * sizeof(int32_t), &lowest, &synced);
     if (r != TSU_OK) return r;
-    double d2h = now_ms() - d0;
+    // wall: launch until the scores are in pinned host memory; d2h: from
+    // there into the caller's array
+    double wall = synced - w0;
+    double d2h = now_ms() - synced;
+    double kms = 0;
+    if (launched) {
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1), "hipEventElapsedTime");
+        kms = ms;
+    }
     int launches = 1;
     const bool clean = lowest >= 0;
     if (!clean && (sh.reg_c == 16 || sh.lring || (sh.two_piece && sh.reg_c))) {
@@ -637,7 +648,7 @@ extern "C" int tsu_align_plain(tsu_ctx* ctx, tsu_batch* b, int32_t wf_length, in
             kms += kms2;
             ++launches;
             wall += now_ms() - r0;
-            d0 = now_ms();
+            const double d0 = now_ms();
             r = copy_to_host(ctx, scores, b->d_scores, (size_t)b->n * sizeof(int32_t));
             if (r != TSU_OK) return r;
             d2h += now_ms() - d0;
@@ -1320,7 +1331,7 @@ extern "C" int tsu_set_tune_table(tsu_ctx* ctx, const tsu_tune_entry* entries, i
 // destination's fresh pages in parallel), the next slice's DMA fills the
 // other half.  (A pageable hipMemcpy goes through the runtime's own staging
 // in small pieces: the scores of a million pairs took about 1 ms that way.)
-static int copy_to_host(tsu_ctx* ctx, void* dst, const void* src, size_t bytes, int32_t* min32) {
+static int copy_to_host(tsu_ctx* ctx, void* dst, const void* src, size_t bytes, int32_t* min32, double* synced_ms) {
     constexpr size_t kSlice = 32u << 20;
     if (min32) *min32 = INT32_MAX;
     if (bytes == 0) return TSU_OK;
@@ -1338,6 +1349,7 @@ static int copy_to_host(tsu_ctx* ctx, void* dst, const void* src, size_t bytes, 
     HIPCHK(hipMemcpyAsync(half[0], from, std::min(kSlice, bytes), hipMemcpyDeviceToHost, ctx->stream), "hipMemcpyAsync D2H");
     for (size_t k = 0; k < slices; ++k) {
         HIPCHK(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");  // slice k is in half[k & 1]
+        if (k == 0 && synced_ms) *synced_ms = now_ms();
         if (k + 1 < slices) {
             const size_t off1 = (k + 1) * kSlice;
             HIPCHK(hipMemcpyAsync(half[(k + 1) & 1], from + off1, std::min(kSlice, bytes - off1), hipMemcpyDeviceToHost,
