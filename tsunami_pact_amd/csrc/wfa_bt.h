@@ -29,12 +29,13 @@ struct BtWalk {
     int64_t base, need, nr;
     uint32_t cur_op, cur_len;
     bool full;
+    bool writer = true;  // this lane stores the runs (one lane of a wave-wide walk)
     int k, off, ss, type, v, h;  // type 0 M, 1 I, 2 D
     int x, o, e;
 
     __device__ __forceinline__ void put(uint32_t run) {
         if (nr >= need) { full = true; return; }
-        out[need - 1 - nr] = run;
+        if (writer) out[need - 1 - nr] = run;
         ++nr;
     }
     __device__ __forceinline__ void emit(uint32_t op, int n) {
@@ -123,6 +124,63 @@ __device__ __forceinline__ int bt_walk(RecFn record, int score, int p, int t, in
     int st = bt_begin(w, score, p, t, x, o, e, a);
     if (st) return st;
     st = bt_run(w, record, 0);
+    if (st) return st;
+    return bt_end(w, out_off, out_n);
+}
+
+// Wave-wide form of bt_walk, for records that sit at a fixed place per score
+// and diagonal.  The walk is a chain of dependent reads, one per step, each
+// a round trip to device memory; here the 64 lanes of one wave load the
+// 8-score x 8-diagonal block of records around the walk's current cell in
+// one round trip (scores ss-7..ss, diagonals k-3..k+4), and the walk reads
+// them from there until a step leaves the block: a mismatch moves 2 scores
+// down the same diagonal, a gap 1 to 4 scores and one diagonal, so a block
+// serves several steps.  The walk's state is the same on every lane (each
+// value read is broadcast from its lane); lane 0 reserves and writes the
+// runs.  load(ss, kk, r) is record's per-lane form: it reads nothing and
+// returns false where (ss, kk) has no record.  Call it from all 64 lanes of
+// the wave.
+template <typename LoadFn>
+__device__ __forceinline__ int bt_walk_wave(LoadFn load, int score, int p, int t, int x, int o, int e, const KArgs& a,
+                                            int64_t* out_off, int32_t* out_n) {
+    const int lane = (int)(threadIdx.x & 63);
+    BtWalk w;
+    const int unit = min(x, o + e);
+    w.need = 2 * (int64_t)(score / unit) + 2;
+    unsigned long long b0 = 0;
+    if (lane == 0) b0 = atomicAdd(a.runs_used, (unsigned long long)w.need);
+    const uint32_t blo = __builtin_amdgcn_readfirstlane((uint32_t)b0);
+    const uint32_t bhi = __builtin_amdgcn_readfirstlane((uint32_t)(b0 >> 32));
+    w.base = (int64_t)(((uint64_t)bhi << 32) | blo);
+    if (w.base + w.need > a.runs_cap) return TSU_ECIGAR;
+    w.out = a.runs + w.base;
+    w.nr = 0;
+    w.cur_op = 4; w.cur_len = 0;
+    w.full = false;
+    w.writer = lane == 0;
+    w.k = t - p; w.off = t; w.ss = score; w.type = 0;
+    w.v = p; w.h = t;
+    w.x = x; w.o = o; w.e = e;
+    bool have = false;    // a block is loaded
+    int bs = 0, bk = 0;   // its top score and its diagonal k (column 3)
+    uint32_t blk = 0;     // this lane's record: score bs - lane/8, diagonal bk - 3 + lane%8
+    uint64_t bvalid = 0;  // lanes whose record exists
+    auto record = [&](int ss, int kk, uint32_t& r) __attribute__((always_inline)) -> bool {
+        int ds = bs - ss, dk = kk - bk + 3;
+        if (!have || ds < 0 || ds > 7 || dk < 0 || dk > 7) {
+            bs = ss; bk = kk; have = true;
+            uint32_t v = 0;
+            const bool ok = load(ss - (lane >> 3), kk - 3 + (lane & 7), v);
+            blk = ok ? v : 0u;
+            bvalid = __ballot(ok);
+            ds = 0; dk = 3;
+        }
+        const int li = __builtin_amdgcn_readfirstlane(8 * ds + dk);
+        if (!((bvalid >> li) & 1ull)) return false;
+        r = (uint32_t)__builtin_amdgcn_readlane((int)blk, li);
+        return true;
+    };
+    int st = bt_run(w, record, 0);
     if (st) return st;
     return bt_end(w, out_off, out_n);
 }

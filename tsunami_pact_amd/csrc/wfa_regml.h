@@ -48,7 +48,7 @@
 // With BT every computed chunk also writes its 64 cell records (wfa_bt.h)
 // to global memory at ((s * NC + j) * 64 + lane) of the workgroup's region
 // (j the global chunk index), one coalesced 256-byte store per chunk and
-// step; after the end cell is reached, thread 0 walks them into CIGAR runs.
+// step; after the end cell is reached, wave 0 walks them into CIGAR runs.
 #pragma once
 
 #include "wfa_bt.h"
@@ -361,7 +361,7 @@ __global__ __launch_bounds__(NW * 64) void wfa_regml_kernel(KArgs a) {
         }
         if constexpr (BT) {
             __syncthreads();  // every wave's records written
-            if (tid == 0) {
+            if (w == 0) {     // wave 0 walks (bt_walk_wave: a block of records per round trip)
                 int st = !lds_at_zero ? TSU_EINTERNAL : score < 0 ? score : 0;
                 int64_t out_off = 0;
                 int32_t out_n = 0;
@@ -373,11 +373,13 @@ __global__ __launch_bounds__(NW * 64) void wfa_regml_kernel(KArgs a) {
                         rr = rec[((int64_t)ss * NC + (j >> 6)) * 64 + (j & 63)];
                         return true;
                     };
-                    st = bt_walk(record, score, p, t, 2, 3, 1, a, &out_off, &out_n);
+                    st = bt_walk_wave(record, score, p, t, 2, 3, 1, a, &out_off, &out_n);
                 }
-                a.run_off[pi] = out_off;
-                a.run_n[pi] = out_n;
-                a.scores[pi] = st ? st : score * a.g;
+                if (lane == 0) {
+                    a.run_off[pi] = out_off;
+                    a.run_n[pi] = out_n;
+                    a.scores[pi] = st ? st : score * a.g;
+                }
             }
         } else {
             if (tid == 0) a.scores[pi] = !lds_at_zero ? TSU_EINTERNAL : score < 0 ? score : score * a.g;
