@@ -458,7 +458,7 @@ def main():
     ap.add_argument("--requests", type=int, default=3000, help="served: requests per rate")
     ap.add_argument("--serve-budget", type=int, default=16384, help="served: HBM budget in MiB")
     ap.add_argument("--inflight", type=int, default=4, help="served: launches in flight")
-    ap.add_argument("--batch-pairs", type=int, default=2048, help="served: pairs merged per launch")
+    ap.add_argument("--batch-pairs", type=int, default=1024, help="served: pairs merged per launch")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
                     help="only run the streaming copy kernel over BYTES (HBM counter calibration)")
     ap.add_argument("--headline-only", action="store_true",

@@ -513,7 +513,7 @@ class Server:
     """Continuous-batching server on one Aligner's device (tsu_serve_*)."""
 
     def __init__(self, aligner: "Aligner", hbm_budget_bytes: int = 8 << 30, max_inflight: int = 4,
-                 max_batch_pairs: int = 4096):
+                 max_batch_pairs: int = 1024):
         self._lib = library()
         self._a = aligner
         cfg = ServeCfg(hbm_budget_bytes, max_inflight, max_batch_pairs)
