@@ -356,7 +356,7 @@ def section_cigar_cfg3(tsu, args):
     a = tsu.Aligner(*PEN)
     b = a.batch(ds)
     tune = a.autotune(b, args.tune_sample)
-    a.align_cigar(b, strings=False)                 # warmup
+    _, _, tw = a.align_cigar(b, strings=False, count_runs=True)   # warmup; the same CIGARs as the timed call
     t0 = time.perf_counter()
     m = 0 if args.no_cpu else 200
     scores, cigars, t = a.align_cigar(b, strings=m)  # CIGAR strings of the first m pairs only
@@ -368,6 +368,15 @@ def section_cigar_cfg3(tsu, args):
            "backtrace": {"state_bytes": t["state_bytes"], "record_launches": t["record_launches"],
                          "checkpoint_launches": t["checkpoint_launches"], "kernels": sorted(t["kernels"])},
            "tune": tune}
+    # HBM: packed sequences and scores, plus the CIGAR runs (4 bytes each) and
+    # a run count per pair; traffic from the committed counter runs of this call
+    algo = algorithmic_bytes(pl, tl) + 4 * tw["cigar_runs"] + 4 * n
+    ach = algo / (t["kernel_ms"] / 1e3) / 1e9
+    traffic = read_traffic("wfa_cigar_cfg3") if n == WORKLOADS["cfg3"][0] else None
+    res["roofline_hbm"] = {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                           "frac": ach / HBM_PEAK_GBS, "algorithmic_bytes": algo, "cigar_runs": tw["cigar_runs"],
+                           "traffic": traffic,
+                           "traffic_frac": None if traffic is None else traffic / (t["kernel_ms"] / 1e3) / 1e9 / HBM_PEAK_GBS}
     if m:
         from oracle import oracle
         want, ops = oracle_sample(ds, m, cigar=True)

@@ -450,14 +450,16 @@ class Aligner:
         the default."""
         self._check(self._lib.tsu_set_poll_limit(self._h, polls), "tsu_set_poll_limit")
 
-    def align_cigar(self, batch: Batch, strings=True):
+    def align_cigar(self, batch: Batch, strings=True, count_runs: bool = False):
         """(scores, CIGAR strings, timing).  With strings=False the CIGARs are
         computed and copied to the host by the library as usual but not
         turned into Python strings (None is returned in their place); an
         integer m returns the strings of the first m pairs only.  The
         timing dict also holds the backtrace state: state_bytes (largest of
         one launch), record_launches, checkpoint_launches and kernels (the
-        set of KERNEL_BITS names of the kernels that recorded)."""
+        set of KERNEL_BITS names of the kernels that recorded); with
+        count_runs, cigar_runs is the number of runs over all pairs (one
+        library call per pair, so not for timed calls)."""
         n = len(batch)
         scores = np.zeros(max(n, 1), np.int32)
         t = Timing()
@@ -472,6 +474,12 @@ class Aligner:
             for i in range(count):
                 self._check(self._lib.tsu_cigars_get(h, i, ctypes.byref(rp), ctypes.byref(nr)), "tsu_cigars_get")
                 cigars.append(cigar_string(rp[j] for j in range(nr.value)))
+            runs = None
+            if count_runs:
+                runs = 0
+                for i in range(n):
+                    self._check(self._lib.tsu_cigars_get(h, i, ctypes.byref(rp), ctypes.byref(nr)), "tsu_cigars_get")
+                    runs += nr.value
             sb, rl, cl = ctypes.c_int64(), ctypes.c_int32(), ctypes.c_int32()
             self._check(self._lib.tsu_cigars_stats(h, ctypes.byref(sb), ctypes.byref(rl), ctypes.byref(cl)),
                         "tsu_cigars_stats")
@@ -482,6 +490,8 @@ class Aligner:
         td = t.as_dict()
         td.update(state_bytes=sb.value, record_launches=rl.value, checkpoint_launches=cl.value,
                   kernels={name for bit, name in KERNEL_BITS.items() if km.value & bit})
+        if count_runs:
+            td["cigar_runs"] = runs
         return scores[:n], cigars, td
 
     def calibrate_copy(self, nbytes: int) -> float:
