@@ -101,10 +101,6 @@ int32_t capacity_full(int32_t p, int32_t t);
 // most max_len bases: capacity_full, or with 2-piece penalties (whose kernel
 // keeps its five rings in LDS) the widest window whose rings fit the LDS.
 int32_t capacity_max(const tsu_ctx* ctx, int32_t max_len);
-// Highest capacity the batched and served forms widen to for pairs of at
-// most max_len bases: capacity_full, or with 2-piece penalties (whose kernel
-// keeps its five rings in LDS) the widest window whose rings fit the LDS.
-int32_t capacity_max(const tsu_ctx* ctx, int32_t max_len);
 // Persistent-launch shape for capacity W (ring in HBM when LDS is too small).
 Shape batched_shape(const tsu_ctx* ctx, int32_t W, int32_t max_len, int nw_hint);
 // Tuned waves per pair for a bucket (0 when the autotuner has not run).
