@@ -184,7 +184,10 @@ __global__ __launch_bounds__(64) void wfa2p_reg_kernel(KArgs a) {
             uint32_t edge = 0;
             int32_t mend = NUL;
             int32_t m0 = NUL, mlast = NUL;  // M of the edge chunks (for the edge test after a trim)
-            int32_t cI1 = NUL, cI2 = NUL;   // old I of the previous chunk's lane 63
+            // the previous chunk's old I1 / I2 rotated by one lane (its lane
+            // 63 in lane 0); the computed chunks are contiguous, so they are
+            // NULL before the first of them and written by each
+            int32_t rI1 = NUL, rI2 = NUL;
 #pragma unroll
             for (int c = 0; c < C; ++c) {
                 if (act & (1u << c)) {
@@ -192,10 +195,12 @@ __global__ __launch_bounds__(64) void wfa2p_reg_kernel(KArgs a) {
                     const int32_t mx = rx[64 * c];
                     const int32_t m1l = r1[64 * c], m1r = r1[64 * c + 2];
                     const int32_t m2l = r2[64 * c], m2r = r2[64 * c + 2];
-                    const int32_t i1n = dpp_shr1(cI1, I1r[P1][c]);
-                    cI1 = __builtin_amdgcn_readlane(I1r[P1][c], 63);
-                    const int32_t i2n = dpp_shr1(cI2, I2r[P2][c]);
-                    cI2 = __builtin_amdgcn_readlane(I2r[P2][c], 63);
+                    const int32_t i1n = dpp_shr1(rI1, I1r[P1][c]);
+                    const int32_t i2n = dpp_shr1(rI2, I2r[P2][c]);
+                    if (c < C - 1) {
+                        rI1 = dpp_ror1(I1r[P1][c]);
+                        rI2 = dpp_ror1(I2r[P2][c]);
+                    }
                     const int32_t d1n =
                         dpp_shl1(c < C - 1 ? dpp_rol1(D1r[P1][c < C - 1 ? c + 1 : c]) : NUL, D1r[P1][c]);
                     const int32_t d2n =
@@ -236,9 +241,6 @@ __global__ __launch_bounds__(64) void wfa2p_reg_kernel(KArgs a) {
                     if (c == 0) m0 = M;
                     if (c == C - 1) mlast = M;
                     if (em & (1u << c)) mend = __builtin_amdgcn_readlane(M, ltp);
-                } else {
-                    cI1 = NUL;  // an idle chunk holds only NULL
-                    cI2 = NUL;
                 }
             }
             if (oob) {

@@ -38,7 +38,7 @@ namespace tsu {
 constexpr int kMl1MaxScore = 16000;
 
 template <int C, bool QUEUE, bool BT = false>
-__global__ __launch_bounds__(64) void wfa_reg1l_kernel(KArgs a) {
+__global__ __launch_bounds__(64, (C == 8 && !QUEUE && !BT) ? 7 : 1) void wfa_reg1l_kernel(KArgs a) {
     static_assert(C >= 1 && C <= 16, "window of 64..1024 diagonals");
     constexpr int32_t NUL = kRegNull + 0x8000;  // low 16 bits: -32768
     constexpr int W = 64 * C;
@@ -184,7 +184,10 @@ __global__ __launch_bounds__(64) void wfa_reg1l_kernel(KArgs a) {
             uint64_t oob = 0;
             uint32_t edge = 0;  // a non-NULL cell in an edge lane of the window
             int32_t mend = NUL;
-            int32_t carryI = NUL;  // old I of the previous chunk's lane 63
+            // the previous chunk's old I rotated by one lane (its lane 63 in
+            // lane 0); the computed chunks are contiguous, so it is NULL
+            // before the first of them and written by each
+            int32_t rotI = NUL;
 #pragma unroll
             for (int c = 0; c < C; ++c) {
                 if (act & (1u << c)) {
@@ -193,9 +196,9 @@ __global__ __launch_bounds__(64) void wfa_reg1l_kernel(KArgs a) {
                     mwr[64 * c] = (int16_t)mx;  // exactly NULL or an offset below 32768
                     const int32_t mol = mrd[64 * c];
                     const int32_t mor = mrd[64 * c + 2];
-                    const int32_t iel = dpp_shr1(carryI, Ir[c]);
+                    const int32_t iel = dpp_shr1(rotI, Ir[c]);
                     const int32_t der = dpp_shl1(c < C - 1 ? dpp_rol1(Dr[c < C - 1 ? c + 1 : c]) : NUL, Dr[c]);
-                    carryI = __builtin_amdgcn_readlane(Ir[c], 63);
+                    if (c < C - 1) rotI = dpp_ror1(Ir[c]);
                     const int32_t I = max(mol, iel) + 2;
                     const int32_t D = max(mor, der);
                     const int32_t X = mx + 2;
@@ -225,8 +228,6 @@ __global__ __launch_bounds__(64) void wfa_reg1l_kernel(KArgs a) {
                     Ir[c] = I;
                     Dr[c] = D;
                     if (em & (1u << c)) mend = __builtin_amdgcn_readlane(M, ltp);
-                } else {
-                    carryI = NUL;  // an idle chunk holds only NULL
                 }
             }
             if (oob) {
