@@ -37,6 +37,10 @@ namespace tsu {
 // -32768 + 2 * kMl1MaxScore stays negative
 constexpr int kMl1MaxScore = 16000;
 
+// The plain 512-diagonal instance is held to 7 waves per SIMD (72 VGPRs, no
+// spills): left alone the compiler takes 73, which the 8-register
+// allocation granule turns into 80 and 6 waves.  The queue-fed instance
+// spills under the same bound and keeps 6.
 template <int C, bool QUEUE, bool BT = false>
 __global__ __launch_bounds__(64, (C == 8 && !QUEUE && !BT) ? 7 : 1) void wfa_reg1l_kernel(KArgs a) {
     static_assert(C >= 1 && C <= 16, "window of 64..1024 diagonals");
