@@ -7,6 +7,7 @@
 // the workgroups that are resident at once (the rings live in LDS), not for
 // all N pairs (wfa_gpu.cu:100-108 allocate num_wf*wf_length*N shorts x3).
 #include <hip/hip_runtime_api.h>
+#include <sys/mman.h>
 
 #include <algorithm>
 #include <chrono>
@@ -90,6 +91,14 @@ int32_t capacity_for_score(const tsu_ctx* ctx, int64_t s_scaled) {
 // [-p-1, t+1].
 int32_t tsu_host::capacity_full(int32_t p, int32_t t) { return p + t + 3; }
 using tsu_host::capacity_full;
+
+void* alloc_host_huge(size_t bytes) {
+    constexpr size_t kHuge = 2u << 20;
+    const size_t n = (std::max<size_t>(bytes, 1) + kHuge - 1) / kHuge * kHuge;
+    void* p = std::aligned_alloc(kHuge, n);
+    if (p) (void)madvise(p, n, MADV_HUGEPAGE);  // advisory: plain pages where not granted
+    return p;
+}
 
 // A pair overflowed the widest window its form may use: the caller's error.
 static int overflow_at(tsu_ctx* ctx, int32_t W, int32_t max_len) {
@@ -2123,7 +2132,7 @@ int CigarCall::collect(tsu_cigars** out) {
     HIPCHK(hipMemcpy(sc2.data(), b->d_scores, (size_t)b->n * sizeof(int32_t), hipMemcpyDeviceToHost), "hipMemcpy D2H");
     unsigned long long used = 0;
     HIPCHK(hipMemcpy(&used, d_used.p, sizeof(used), hipMemcpyDeviceToHost), "hipMemcpy D2H");
-    cg->runs.reset(new (std::nothrow) uint32_t[(size_t)std::max<unsigned long long>(used, 1)]);
+    cg->runs.reset(static_cast<uint32_t*>(alloc_host_huge((size_t)std::max<unsigned long long>(used, 1) * 4)));
     if (!cg->runs) return fail(ctx, TSU_ENOMEM, "host allocation");
     if (used) {
         int r2 = copy_to_host(ctx, cg->runs.get(), d_runs.p, (size_t)used * 4);

@@ -5,6 +5,7 @@
 
 #include <algorithm>
 #include <climits>
+#include <cstdlib>
 #include <memory>
 #include <string>
 #include <vector>
@@ -65,8 +66,16 @@ struct tsu_batch {
     double cells = 0.0;
 };
 
+// Host buffers of many MB that are written once right after allocation (the
+// CIGAR run pool): 2 MiB aligned and advised as transparent huge pages, so
+// the first writes take a fault per 2 MiB instead of per 4 KiB page.
+struct HugeHostFree {
+    void operator()(void* p) const { std::free(p); }
+};
+void* alloc_host_huge(size_t bytes);
+
 struct tsu_cigars {
-    std::unique_ptr<uint32_t[]> runs;  // run pool as copied from the device (not zero-filled)
+    std::unique_ptr<uint32_t, HugeHostFree> runs;  // run pool as copied from the device (not zero-filled)
     std::vector<int64_t> off;
     std::vector<int32_t> n;
     int64_t state_bytes = 0;       // largest backtrace state of one launch (device)
