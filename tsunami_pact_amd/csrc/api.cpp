@@ -94,7 +94,8 @@ using tsu_host::capacity_full;
 
 void* alloc_host_huge(size_t bytes) {
     constexpr size_t kHuge = 2u << 20;
-    const size_t n = (std::max<size_t>(bytes, 1) + kHuge - 1) / kHuge * kHuge;
+    if (bytes < 4 * kHuge) return std::malloc(std::max<size_t>(bytes, 1));  // small: no 2 MiB rounding
+    const size_t n = (bytes + kHuge - 1) / kHuge * kHuge;
     void* p = std::aligned_alloc(kHuge, n);
     if (p) (void)madvise(p, n, MADV_HUGEPAGE);  // advisory: plain pages where not granted
     return p;

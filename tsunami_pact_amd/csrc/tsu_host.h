@@ -66,9 +66,10 @@ struct tsu_batch {
     double cells = 0.0;
 };
 
-// Host buffers of many MB that are written once right after allocation (the
-// CIGAR run pool): 2 MiB aligned and advised as transparent huge pages, so
-// the first writes take a fault per 2 MiB instead of per 4 KiB page.
+// Host buffers written once right after allocation (the CIGAR run pool):
+// from 8 MiB on, 2 MiB aligned and advised as transparent huge pages, so the
+// first writes take a fault per 2 MiB instead of per 4 KiB page; smaller
+// ones plain malloc.  Either is released with free.
 struct HugeHostFree {
     void operator()(void* p) const { std::free(p); }
 };
