@@ -202,6 +202,9 @@ def test_cigars_match_oracle(aligner, tsu, oracle, fixtures):
     for i, (p, t) in enumerate(pairs):
         assert cig[i] == oracle.rle(ops[i]), i
         assert oracle.rescore(oracle.expand(cig[i]), p, t, 4, 6, 2) == got[i]
+    # the run count over all pairs: one run per letter group of each CIGAR
+    _, _, tc = aligner.align_cigar(aligner.batch(ds), strings=False, count_runs=True)
+    assert tc["cigar_runs"] == sum(sum(1 for ch in c if ch.isalpha()) for c in cig)
     cases = [c for c in fixtures if tuple(c["penalties"]) == (4, 6, 2)]
     say(f"cigar fixtures {len(cases)}")
     got, cig, _ = aligner.align_cigar(aligner.batch([(c["pattern"], c["text"]) for c in cases]))
