@@ -272,7 +272,7 @@ def run_refgrid(a, tsu, args):
 
 
 def oracle_sample(ds, m, cigar=False):
-    """Oracle scores (and op strings) of the first m pairs of ds: the checker."""
+    """Oracle scores (and op strings) of the first m pairs of ds, to check the GPU's."""
     from oracle import oracle
     sample = [ds.pair(i) for i in range(min(m, len(ds)))]
     scores, ops, _ = oracle.wfa_batch(sample, *PEN, threads=oracle.max_threads(), cigar=cigar)

@@ -1,7 +1,7 @@
 """ctypes wrapper of oracle/liboracle.so -- TEST INFRASTRUCTURE ONLY.
 
 Imported by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg,
-always as the checker (or the timed CPU baseline), never by the product.
+always to check results (or as the timed CPU baseline), never by the product.
 See wfa_oracle.c for the algorithm and the reference lines it follows.
 """
 from __future__ import annotations

@@ -2,7 +2,7 @@
  * wfa2p_oracle.c -- CPU oracle for gap-affine 2-piece WaveFront Alignment.
  *
  * TEST INFRASTRUCTURE ONLY.  Nothing in the product links, loads or calls
- * this file; it is used by tests/ (as the checker) and by bench.py's
+ * this file; it is used by tests/ (to check results) and by bench.py's
  * cpu_baseline leg.
  *
  * Gap-affine 2-piece scoring: match 0, mismatch x, a gap of length L costs

@@ -3,7 +3,7 @@
  *
  * TEST INFRASTRUCTURE ONLY.  Nothing in the product (the HIP library, the
  * wfa_gpu / generate_dataset CLIs, the Python package) links, loads or calls
- * this file.  It is used by tests/ (as the checker), by __graft_entry__.smoke()
+ * this file.  It is used by tests/ (to check results), by __graft_entry__.smoke()
  * and by bench.py's cpu_baseline leg (timed as a "port" baseline).
  *
  * Contents
