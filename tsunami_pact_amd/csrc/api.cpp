@@ -816,8 +816,11 @@ static const tsu_tune_entry* tune_lookup(const tsu_ctx* ctx, int32_t bound) { re
 // prediction only places a pair; a pair that overflows is re-run wider, so
 // scores do not depend on it.  (On 400 pairs of 4-20 kbp at 1-15%, the ratio
 // of the true to the predicted score had median 1.01, 95th percentile 1.16,
-// maximum 1.29.)
-constexpr double kProbeMargin = 1.2;
+// maximum 1.29.)  A margin of 1.05-1.1 measured fastest on 500,000 pairs of
+// 100-20,000 bp at 1-15%: about 1.5% below 1.2 (more pairs in the narrower
+// windows outweigh the extra re-runs), 1.0 and 1.3 slower.  The served form
+// keeps 1.2: there a re-run waits for its flight to end, and at 1.1 the
+// median latency at 0.9 of capacity rose from 54-59 to 74 ms.
 
 // Capacity a probed pair gets: the predicted range with the margin, rounded
 // to the window of the kernel that will take it (64..512 one wave, 1024 two
@@ -826,10 +829,10 @@ constexpr double kProbeMargin = 1.2;
 // in device memory remains, several times slower per diagonal: a pair goes
 // there directly only when even its unmargined prediction is well beyond
 // 16,384, and otherwise tries the 16-chunk kernel's widest window first.
-int32_t tsu_host::probe_capacity(const tsu_ctx* ctx, const tsu_pair& pr, int32_t progress) {
+int32_t tsu_host::probe_capacity(const tsu_ctx* ctx, const tsu_pair& pr, int32_t progress, double margin) {
     const double s_end = (double)kProbeCap * (double)(pr.p_len + pr.t_len) / (double)std::max(progress, 1);
     const double need = std::min(2.0 * (s_end - ctx->os) / ctx->es + 3.0, 1e9);
-    const int64_t w = (int64_t)std::ceil(kProbeMargin * need);
+    const int64_t w = (int64_t)std::ceil(margin * need);
     int64_t c;
     if (w <= 512) {
         for (c = 64; c < w; c *= 2) {}

@@ -424,7 +424,7 @@ void retire_probe(tsu_server* s, Slot& sl) {
         if (v >= 0) { finish_entry(en.r, en.i, v, now); continue; }
         const tsu_pair& pr = sl.fpairs[(size_t)k];
         const int bucket = tsu_host::bucket_of(std::max(pr.p_len, pr.t_len));
-        const int32_t W = v == tsu::kProbeStopped ? tsu_host::probe_capacity(ctx, pr, sl.h_probe[k]) : 0;
+        const int32_t W = v == tsu::kProbeStopped ? tsu_host::probe_capacity(ctx, pr, sl.h_probe[k], tsu_host::kProbeMarginServed) : 0;
         s->pending[queue_key(bucket, W)].push_back(Entry{en.r, en.i, W});
         ++s->n_pending;
     }

@@ -148,7 +148,12 @@ int ml_waves(const tsu_ctx* ctx, int32_t W, int32_t max_len);
 // the range that far), and the furthest h + v it reached gives the capacity
 // the pair is placed at (probe_capacity).
 constexpr int32_t kProbeCap = 240;
-int32_t probe_capacity(const tsu_ctx* ctx, const tsu_pair& pr, int32_t progress);
+// margin: the factor of spare range (kProbeMargin for throughput; the
+// server passes kProbeMarginServed, since a re-run adds to its requests'
+// latency).
+constexpr double kProbeMargin = 1.1;
+constexpr double kProbeMarginServed = 1.2;
+int32_t probe_capacity(const tsu_ctx* ctx, const tsu_pair& pr, int32_t progress, double margin = kProbeMargin);
 // Pairs of a length bucket are probed when the register kernels take them
 // (their sequences, up to max_len bases, fit in LDS) and the bucket's first
 // capacity (W) needs several waves.
