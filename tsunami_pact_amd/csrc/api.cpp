@@ -819,8 +819,8 @@ static const tsu_tune_entry* tune_lookup(const tsu_ctx* ctx, int32_t bound) { re
 // maximum 1.29.)  A margin of 1.05-1.1 measured fastest on 500,000 pairs of
 // 100-20,000 bp at 1-15%: about 1.5% below 1.2 (more pairs in the narrower
 // windows outweigh the extra re-runs), 1.0 and 1.3 slower.  The served form
-// keeps 1.2: there a re-run waits for its flight to end, and at 1.1 the
-// median latency at 0.9 of capacity rose from 54-59 to 74 ms.
+// keeps 1.2: there a re-run waits for its flight to end and adds to its
+// request's latency (the two margins measured alike there, within noise).
 
 // Capacity a probed pair gets: the predicted range with the margin, rounded
 // to the window of the kernel that will take it (64..512 one wave, 1024 two
