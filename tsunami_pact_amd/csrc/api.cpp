@@ -1707,9 +1707,21 @@ int CigarCall::launch_wide_async(WidePlan& wp, const std::vector<int32_t>& ids) 
 // rather than its bucket's 16,384.
 int CigarCall::plan_records(std::vector<RecordPlan>& plans, int64_t& runs_need) {
     const int nb = tsu_host::bucket_count();
+    // one pass for every pair's bucket and each bucket's count and longest
+    // sequence, then the buckets' lists in pair order
+    std::vector<uint8_t> bof((size_t)b->n);
+    std::vector<int64_t> cnt((size_t)nb, 0);
+    std::vector<int32_t> bmax((size_t)nb, 0);
+    for (int32_t i = 0; i < b->n; ++i) {
+        const int32_t L = std::max(b->pairs[(size_t)i].p_len, b->pairs[(size_t)i].t_len);
+        const int bi = tsu_host::bucket_of(L);
+        bof[(size_t)i] = (uint8_t)bi;
+        ++cnt[(size_t)bi];
+        bmax[(size_t)bi] = std::max(bmax[(size_t)bi], L);
+    }
     std::vector<std::vector<int32_t>> bk((size_t)nb);
-    for (int32_t i = 0; i < b->n; ++i)
-        bk[(size_t)tsu_host::bucket_of(std::max(b->pairs[(size_t)i].p_len, b->pairs[(size_t)i].t_len))].push_back(i);
+    for (int bi = 0; bi < nb; ++bi) bk[(size_t)bi].reserve((size_t)cnt[(size_t)bi]);
+    for (int32_t i = 0; i < b->n; ++i) bk[bof[(size_t)i]].push_back(i);
     runs_need = 0;
     struct Group {
         int32_t W;    // first capacity
@@ -1722,8 +1734,7 @@ int CigarCall::plan_records(std::vector<RecordPlan>& plans, int64_t& runs_need) 
     for (int bi = nb - 1; bi >= 0; --bi) {
         std::vector<int32_t>& ids = bk[(size_t)bi];
         if (ids.empty()) continue;
-        int32_t max_len = 0;
-        for (int32_t i : ids) max_len = std::max(max_len, std::max(b->pairs[(size_t)i].p_len, b->pairs[(size_t)i].t_len));
+        const int32_t max_len = bmax[(size_t)bi];
         const int32_t bound = tsu_host::bucket_bound(bi);
         const tsu_tune_entry* te = tune_lookup(ctx, bound);
         if (te && te->waves_per_pair > 1) continue;  // tuned for the LDS kernel
